@@ -1,0 +1,175 @@
+/*
+ * ins_poisson.h - C ABI of the MI355X-native DG Poisson solve.
+ *
+ * Drop-in for the Poisson linear-solver interface that INS-Solver calls on
+ * the (un-vendored) OP2-DG-Toolkit. The reference call sequence is
+ *
+ *   PoissonMatrixFreeDiag2D(mesh) / PETScPMultigrid(mesh)   src/solvers/2d/ins_solver.cpp:91-96
+ *   matrix->set_bc_types(bc_types)                          src/solvers/2d/ins_solver.cpp:205-206
+ *   solver->set_nullspace(b); set_tol_and_iter(r,a,it)      src/solvers/2d/ins_solver_base.cpp:154-170
+ *   solver->init()                                          src/solvers/2d/ins_solver.cpp:207
+ *   matrix->set_factor(1/rho)                               src/solvers/2d/mp_ins_solver.cpp:576-578
+ *   solver->set_bcs(bc_data); solver->solve(rhs, ans)       src/solvers/2d/ins_solver.cpp:324-325
+ *
+ * and each entry point below names the call it replaces. Conventions:
+ *  - return values: 0 = success / converged, 1 = not converged (the
+ *    reference's `solve` returning false), < 0 = error (see INS_ERR_*);
+ *    ins_poisson_last_error() describes the last error of the calling thread.
+ *  - vectors are cells x Np nodal values, in one of three layouts:
+ *      INS_LAYOUT_HOST_AOS    host pointer, dat[cell*Np + i] (OP2 CPU dat)
+ *      INS_LAYOUT_DEVICE_SOA  device pointer, dat[i*stride + cell] with the
+ *                             stride from ins_poisson_info (OP2 GPU dat with
+ *                             -DDG_OP2_SOA, Makefile:40-41)
+ *      INS_LAYOUT_DEVICE_AOS  device pointer, dat[cell*Np + i]
+ *  - the rhs is mass-weighted (M f) as assembled by the reference callers
+ *    (src/solvers/2d/ins_solver.cpp:310-319); boundary terms from
+ *    ins_poisson_set_bcs are added inside the solve (viscous_solver.cpp:62-63).
+ *  - `ans` is the initial guess on entry and the solution on exit.
+ *  - one handle per host thread; a handle is not thread-safe.
+ */
+#ifndef INS_POISSON_H
+#define INS_POISSON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ins_poisson* ins_poisson_t;
+typedef struct ins_mesh* ins_mesh_t;
+
+enum {
+  INS_OK = 0,
+  INS_NOT_CONVERGED = 1,
+  INS_ERR_INVALID = -1,
+  INS_ERR_HIP = -2,
+  INS_ERR_BREAKDOWN = -3, /* p.Ap <= 0 or a non-finite residual */
+  INS_ERR_NOMEM = -4,
+  INS_ERR_INTERNAL = -5
+};
+
+enum { INS_LAYOUT_HOST_AOS = 0, INS_LAYOUT_DEVICE_SOA = 1, INS_LAYOUT_DEVICE_AOS = 2 };
+
+/* Numbering follows LinearSolver::Solvers (src/solvers/2d/ins_solver_base.cpp:1294-1309). */
+enum { INS_SOLVER_JACOBI = 1, INS_SOLVER_PMULTIGRID = 4 };
+
+enum { INS_PRECISION_FP64 = 0, INS_PRECISION_MIXED = 1 };
+
+/* Boundary types per bface (include/problem_specific_2d.h:43-44). */
+enum { INS_BC_DIRICHLET = 0, INS_BC_NEUMANN = 1 };
+
+typedef struct {
+  int dim;               /* 2 (triangles) or 3 (tetrahedra) */
+  int64_t n_nodes;
+  const double* coords;  /* n_nodes x dim (OP2 dat node_coords) */
+  int64_t n_cells;
+  const int* cell2nodes; /* n_cells x (dim+1) (OP2 map cell2nodes) */
+} ins_mesh_desc;
+
+typedef struct {
+  int iterations;           /* outer Krylov iterations */
+  int converged;
+  double rel_residual;      /* true |b - A x|_2 / |b|_2 in FP64 */
+  double abs_residual;
+  double bnorm;
+  double seconds;           /* wall time of the solve call */
+  int64_t fine_applies;     /* FP64 applications of the finest operator */
+  int64_t smoother_applies; /* operator applications inside the preconditioner */
+} ins_solve_stats;
+
+/* ---- meshes (stand-in for the toolkit's DGMesh2D/3D construction, src/ins2d.cpp:192) ---- */
+
+/* Unit square, nx x ny cells split into 2 triangles, interior vertices jittered by
+ * +-jitter*h (uniform, seeded). */
+int ins_mesh_generate_square(int nx, int ny, double jitter, uint64_t seed, ins_mesh_t* out);
+/* Unit cube, n^3 hexes split into 6 Kuhn tetrahedra, interior vertices jittered. */
+int ins_mesh_generate_cube(int n, double jitter, uint64_t seed, ins_mesh_t* out);
+/* Mesh from user arrays (cells are re-oriented positively). */
+int ins_mesh_create(const ins_mesh_desc* desc, ins_mesh_t* out);
+int ins_mesh_info(ins_mesh_t m, int* dim, int64_t* n_nodes, int64_t* n_cells, int64_t* n_faces,
+                  int64_t* n_bfaces);
+/* Any output pointer may be NULL. */
+int ins_mesh_get(ins_mesh_t m, double* coords, int* cell2nodes);
+/* OP2 sets faces/bfaces: face2cells (n_faces x 2), faceNum (n_faces x 2),
+ * bface2cells (n_bfaces), bfaceNum (n_bfaces). */
+int ins_mesh_get_faces(ins_mesh_t m, int* face2cells, int* faceNum, int* bface2cells,
+                       int* bfaceNum);
+void ins_mesh_destroy(ins_mesh_t m);
+
+/* ---- solver handle: PoissonMatrixFreeDiag{2D,3D} + LinearSolver ---- */
+
+/* Replaces `new PoissonMatrixFreeDiag2D(mesh)` + `new PETScPMultigrid(mesh)`
+ * (src/solvers/2d/ins_solver.cpp:91-96). Copies the mesh to the device. */
+int ins_poisson_create(ins_mesh_t mesh, int order, ins_poisson_t* out);
+void ins_poisson_destroy(ins_poisson_t h);
+int ins_poisson_info(ins_poisson_t h, int* np, int* nfp, int64_t* n_cells, int64_t* stride,
+                     int64_t* n_bfaces);
+/* Nodal coordinates, dim x n_cells x Np (the toolkit dats x, y, z). */
+int ins_poisson_node_coords(ins_poisson_t h, double* xyz);
+
+/* PoissonMatrix::set_bc_types(op_dat bfaces x 1) - src/solvers/2d/ins_solver.cpp:205. */
+int ins_poisson_set_bc_types(ins_poisson_t h, const int* bc_types, int64_t n_bfaces);
+/* FactorPoissonMatrixFreeDiag::set_factor(op_dat cells x Np) - src/solvers/2d/mp_ins_solver.cpp:577.
+ * NULL restores the constant coefficient. */
+int ins_poisson_set_factor(ins_poisson_t h, const double* factor, int layout);
+/* LinearSolver::set_bcs(op_dat bfaces x Nfp) - src/solvers/2d/ins_solver.cpp:324. Values are
+ * given at the face nodes of each bface in the owner's face-mask order. Layout HOST_AOS or
+ * DEVICE_AOS (row per bface). NULL clears. */
+int ins_poisson_set_bcs(ins_poisson_t h, const double* bc_data, int layout);
+/* LinearSolver::set_nullspace(bool) - src/solvers/2d/ins_solver_base.cpp:156-157. */
+int ins_poisson_set_nullspace(ins_poisson_t h, int nullspace);
+/* LinearSolver::set_tol_and_iter(rtol, atol, max_iter) - src/solvers/2d/ins_solver_base.cpp:170.
+ * Converged when |b - A x|_2 <= max(rtol |b|_2, atol). */
+int ins_poisson_set_tol_and_iter(ins_poisson_t h, double rtol, double atol, int max_iter);
+/* Solver type (INS_SOLVER_*): the reference's set_solver_type enum, ins_solver_base.cpp:1294-1309. */
+int ins_poisson_set_solver(ins_poisson_t h, int type);
+/* INS_PRECISION_FP64, or INS_PRECISION_MIXED: FP32 V-cycle inside the FP64 flexible CG. */
+int ins_poisson_set_precision(ins_poisson_t h, int precision);
+/* p-multigrid options (the [p-multigrid] INI section: orders, pre_it, post_it, cheb_orders,
+ * eigen_val_saftey_factor, cheb_min_lamda_factor, max_iter_eig_approx); per-level arrays
+ * have n_levels entries and may be NULL for defaults. coarse_iters: Jacobi-CG iterations on
+ * the coarsest order. */
+int ins_poisson_set_pmg(ins_poisson_t h, const int* orders, int n_levels, const int* pre_it,
+                        const int* post_it, const int* cheb_orders, double eig_safety,
+                        double cheb_min_factor, int max_iter_eig, int coarse_iters);
+/* LinearSolver::init() - src/solvers/2d/ins_solver.cpp:207: penalties, diagonals and
+ * Chebyshev eigenvalue bounds. Called implicitly by solve when needed. */
+int ins_poisson_init(ins_poisson_t h);
+/* LinearSolver::solve(rhs, ans) - src/solvers/2d/ins_solver.cpp:325. */
+int ins_poisson_solve(ins_poisson_t h, const double* rhs, double* ans, int layout,
+                      ins_solve_stats* stats);
+
+/* PoissonMatrix::mult(in, out) on the finest order (FP64), and its FP32 kernel variant. */
+int ins_poisson_mult(ins_poisson_t h, const double* in, double* out, int layout);
+int ins_poisson_mult_fp32(ins_poisson_t h, const double* in, double* out, int layout);
+/* Diagonal used by Jacobi (FactorViscousMatrix2D::calc_diag, factor_viscous_matrix.cpp:211-256). */
+int ins_poisson_diag(ins_poisson_t h, double* diag, int layout);
+/* rhs += boundary terms (vmf_2d_apply_bc.h / fvmf_2d_apply_bc.h), in place. */
+int ins_poisson_apply_bc(ins_poisson_t h, double* rhs, int layout);
+/* out = J M in per cell (DGMesh::mass). */
+int ins_poisson_mass(ins_poisson_t h, const double* in, double* out, int layout);
+/* Power-iteration estimate of lambda_max(D^-1 A) on p-multigrid level `level`. */
+int ins_poisson_lambda_max(ins_poisson_t h, int level, double* out);
+
+/* Device time of the finest-order SIPG kernels (HIP events on the solver's stream). */
+int ins_poisson_set_profiling(ins_poisson_t h, int on);
+int ins_poisson_take_timing(ins_poisson_t h, double* trace_ms, int64_t* trace_launches,
+                            double* apply_ms, int64_t* apply_launches);
+
+/* Reference-element tables of order `order` (the toolkit's DGConstants: Dr/Ds/Dt, MASS,
+ * FMASK, face mass), row-major; any output may be NULL. rst: dim x Np, D: dim x Np x Np,
+ * M: Np x Np, fmask: Nfaces x Nfp, Mface: Nfaces x Nfp x Nfp. Host only, no device use. */
+int ins_ref_element(int dim, int order, double* rst, double* D, double* M, int* fmask,
+                    double* Mface);
+/* Nodal interpolation from order nc to order nf (Np(nf) x Np(nc)), the p-multigrid
+ * prolongation; host only. */
+int ins_ref_interp(int dim, int nf, int nc, double* P);
+
+const char* ins_poisson_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INS_POISSON_H */

@@ -1,0 +1,54 @@
+// Host-side launchers for the HIP kernels. All vectors use the OP2 GPU
+// struct-of-arrays layout: value i of cell e lives at v[i*stride + e]; entries
+// of padding cells (e >= E) are kept at zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace insp {
+
+struct MeshView {
+  int dim;
+  int64_t E, stride;
+  const double* Kt;    // [NK][stride] metric scaled by J
+  const double* J;     // [stride]
+  const int* nbr;      // [Nf][stride] neighbour or -(bface+1)
+  const int* code;     // [Nf][stride] neighbour face * nperm + vertex permutation
+  const int* bctype;   // [n_bfaces] 0 Dirichlet, 1 Neumann
+};
+
+struct LevelView {
+  int N;
+  const void* C;       // operator constants in the kernel precision
+  const double* C64;   // operator constants in double (set-up kernels)
+  const int* tab;
+  const double* tauS;  // [Nf][stride] penalty * sJ
+};
+
+// Fine-level SIPG operator: trace pass + element pass.
+template <typename T>
+void launch_trace(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* tr,
+                  hipStream_t s);
+template <typename T>
+void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
+                  T* out, hipStream_t s);
+// Exact diagonal of the operator (double).
+void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
+                 hipStream_t s);
+// Penalty tau*sJ per cell face (double), tau = 2(P+1)(P+2) max(fscale k) with P = pen_order.
+void launch_tau(const MeshView& m, const LevelView& l, const double* kfac, int pen_order,
+                double* tauS, hipStream_t s);
+// rhs += boundary-condition terms; bc is bfaces x Nfp (row per boundary face).
+void launch_apply_bc(const MeshView& m, const LevelView& l, const double* bc, const double* kfac,
+                     double* rhs, hipStream_t s);
+// out = J * M * in per cell.
+void launch_mass(const MeshView& m, const LevelView& l, const double* in, double* out,
+                 hipStream_t s);
+// y (+)= A x per cell, A is rows x cols row-major (device, precision T).
+template <typename T>
+void launch_elem_matvec(int dim, int rows_order, int cols_order, int64_t E, int64_t stride,
+                        const T* A, const T* x, T* y, bool add, hipStream_t s);
+
+}  // namespace insp

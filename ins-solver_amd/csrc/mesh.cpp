@@ -1,0 +1,261 @@
+#include "mesh.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <random>
+#include <stdexcept>
+
+#include "refelem.h"
+
+namespace insp {
+
+namespace {
+
+inline double uniform01(std::mt19937_64& g) { return double(g() >> 11) * 0x1.0p-53; }
+
+double signed_measure(const HostMesh& m, int64_t e) {
+  const int d = m.dim;
+  const int* c = &m.cell2nodes[(size_t)e * (d + 1)];
+  if (d == 2) {
+    const double* a = &m.coords[(size_t)c[0] * 2];
+    const double* b = &m.coords[(size_t)c[1] * 2];
+    const double* q = &m.coords[(size_t)c[2] * 2];
+    return (b[0] - a[0]) * (q[1] - a[1]) - (q[0] - a[0]) * (b[1] - a[1]);
+  }
+  const double* a = &m.coords[(size_t)c[0] * 3];
+  double u[3][3];
+  for (int k = 0; k < 3; ++k)
+    for (int x = 0; x < 3; ++x) u[k][x] = m.coords[(size_t)c[k + 1] * 3 + x] - a[x];
+  return u[0][0] * (u[1][1] * u[2][2] - u[1][2] * u[2][1]) -
+         u[0][1] * (u[1][0] * u[2][2] - u[1][2] * u[2][0]) +
+         u[0][2] * (u[1][0] * u[2][1] - u[1][1] * u[2][0]);
+}
+
+}  // namespace
+
+HostMesh make_square_mesh(int nx, int ny, double jitter, uint64_t seed) {
+  if (nx < 1 || ny < 1) throw std::runtime_error("square mesh needs nx, ny >= 1");
+  HostMesh m;
+  m.dim = 2;
+  m.n_nodes = (int64_t)(nx + 1) * (ny + 1);
+  m.coords.resize((size_t)m.n_nodes * 2);
+  std::mt19937_64 g(seed);
+  const double hx = 1.0 / nx, hy = 1.0 / ny;
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const size_t v = (size_t)j * (nx + 1) + i;
+      double x = i * hx, y = j * hy;
+      if (i > 0 && i < nx && j > 0 && j < ny) {
+        x += (2 * uniform01(g) - 1) * jitter * hx;
+        y += (2 * uniform01(g) - 1) * jitter * hy;
+      }
+      m.coords[v * 2] = x;
+      m.coords[v * 2 + 1] = y;
+    }
+  m.n_cells = (int64_t)2 * nx * ny;
+  m.cell2nodes.resize((size_t)m.n_cells * 3);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const int v00 = j * (nx + 1) + i, v10 = v00 + 1, v01 = v00 + nx + 1, v11 = v01 + 1;
+      const size_t e = 2 * ((size_t)j * nx + i);
+      const int t0[3] = {v00, v10, v11}, t1[3] = {v00, v11, v01};
+      std::copy(t0, t0 + 3, &m.cell2nodes[e * 3]);
+      std::copy(t1, t1 + 3, &m.cell2nodes[(e + 1) * 3]);
+    }
+  build_topology(m);
+  build_geometry(m);
+  return m;
+}
+
+HostMesh make_cube_mesh(int n, double jitter, uint64_t seed) {
+  if (n < 1) throw std::runtime_error("cube mesh needs n >= 1");
+  HostMesh m;
+  m.dim = 3;
+  const int n1 = n + 1;
+  m.n_nodes = (int64_t)n1 * n1 * n1;
+  m.coords.resize((size_t)m.n_nodes * 3);
+  std::mt19937_64 g(seed);
+  const double h = 1.0 / n;
+  for (int k = 0; k <= n; ++k)
+    for (int j = 0; j <= n; ++j)
+      for (int i = 0; i <= n; ++i) {
+        const size_t v = ((size_t)k * n1 + j) * n1 + i;
+        double x[3] = {i * h, j * h, k * h};
+        if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+          for (int c = 0; c < 3; ++c) x[c] += (2 * uniform01(g) - 1) * jitter * h;
+        for (int c = 0; c < 3; ++c) m.coords[v * 3 + c] = x[c];
+      }
+  m.n_cells = (int64_t)6 * n * n * n;
+  m.cell2nodes.resize((size_t)m.n_cells * 4);
+  static const int axis_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                      {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t cube = ((size_t)k * n + j) * n + i;
+        for (int t = 0; t < 6; ++t) {
+          int p[3] = {i, j, k};
+          int verts[4];
+          verts[0] = (p[2] * n1 + p[1]) * n1 + p[0];
+          for (int s = 0; s < 3; ++s) {
+            p[axis_perm[t][s]] += 1;
+            verts[s + 1] = (p[2] * n1 + p[1]) * n1 + p[0];
+          }
+          std::copy(verts, verts + 4, &m.cell2nodes[(cube * 6 + t) * 4]);
+        }
+      }
+  build_topology(m);
+  build_geometry(m);
+  return m;
+}
+
+void build_topology(HostMesh& m) {
+  const int d = m.dim, nv = d + 1, Nf = d + 1;
+  const int* fv = face_vertices(d);
+  // Orient every cell positively.
+  for (int64_t e = 0; e < m.n_cells; ++e) {
+    const double vol = signed_measure(m, e);
+    if (vol == 0.0) throw std::runtime_error("degenerate cell");
+    if (vol < 0) std::swap(m.cell2nodes[(size_t)e * nv + d - 1], m.cell2nodes[(size_t)e * nv + d]);
+  }
+  struct FaceRec {
+    std::array<int, 3> key;
+    int64_t cell;
+    int face;
+  };
+  std::vector<FaceRec> recs((size_t)m.n_cells * Nf);
+  for (int64_t e = 0; e < m.n_cells; ++e)
+    for (int f = 0; f < Nf; ++f) {
+      FaceRec& r = recs[(size_t)e * Nf + f];
+      r.key = {-1, -1, -1};
+      for (int k = 0; k < d; ++k) r.key[k] = m.cell2nodes[(size_t)e * nv + fv[f * d + k]];
+      std::sort(r.key.begin(), r.key.begin() + d);
+      r.cell = e;
+      r.face = f;
+    }
+  std::sort(recs.begin(), recs.end(), [](const FaceRec& a, const FaceRec& b) {
+    if (a.key != b.key) return a.key < b.key;
+    if (a.cell != b.cell) return a.cell < b.cell;
+    return a.face < b.face;
+  });
+  m.nbr.assign((size_t)m.n_cells * Nf, INT32_MIN);
+  m.nbr_face.assign((size_t)m.n_cells * Nf, 0);
+  m.nbr_perm.assign((size_t)m.n_cells * Nf, 0);
+  const int nperm = d == 2 ? 2 : 6;
+  static const int perms2[2][3] = {{0, 1, 0}, {1, 0, 0}};
+  static const int perms3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  auto find_perm = [&](int64_t e, int f, int64_t en, int fn) {
+    int mine[3], theirs[3];
+    for (int k = 0; k < d; ++k) {
+      mine[k] = m.cell2nodes[(size_t)e * nv + fv[f * d + k]];
+      theirs[k] = m.cell2nodes[(size_t)en * nv + fv[fn * d + k]];
+    }
+    for (int p = 0; p < nperm; ++p) {
+      bool ok = true;
+      for (int k = 0; k < d; ++k) {
+        const int src = d == 2 ? perms2[p][k] : perms3[p][k];
+        ok &= theirs[k] == mine[src];
+      }
+      if (ok) return p;
+    }
+    throw std::runtime_error("face vertex permutation not found");
+  };
+  std::vector<std::array<int64_t, 4>> interior;  // e1, f1, e2, f2
+  std::vector<std::array<int64_t, 2>> boundary;
+  size_t i = 0;
+  while (i < recs.size()) {
+    size_t j = i + 1;
+    while (j < recs.size() && recs[j].key == recs[i].key) ++j;
+    if (j - i == 1) {
+      boundary.push_back({recs[i].cell, recs[i].face});
+    } else if (j - i == 2) {
+      interior.push_back({recs[i].cell, recs[i].face, recs[i + 1].cell, recs[i + 1].face});
+    } else {
+      throw std::runtime_error("non-manifold face shared by more than two cells");
+    }
+    i = j;
+  }
+  std::sort(interior.begin(), interior.end());
+  std::sort(boundary.begin(), boundary.end());
+  m.n_faces = (int64_t)interior.size();
+  m.face2cells.resize((size_t)m.n_faces * 2);
+  m.faceNum.resize((size_t)m.n_faces * 2);
+  for (size_t k = 0; k < interior.size(); ++k) {
+    const int64_t e1 = interior[k][0], e2 = interior[k][2];
+    const int f1 = (int)interior[k][1], f2 = (int)interior[k][3];
+    m.face2cells[k * 2] = (int)e1;
+    m.face2cells[k * 2 + 1] = (int)e2;
+    m.faceNum[k * 2] = f1;
+    m.faceNum[k * 2 + 1] = f2;
+    m.nbr[(size_t)e1 * Nf + f1] = (int)e2;
+    m.nbr_face[(size_t)e1 * Nf + f1] = f2;
+    m.nbr_perm[(size_t)e1 * Nf + f1] = find_perm(e1, f1, e2, f2);
+    m.nbr[(size_t)e2 * Nf + f2] = (int)e1;
+    m.nbr_face[(size_t)e2 * Nf + f2] = f1;
+    m.nbr_perm[(size_t)e2 * Nf + f2] = find_perm(e2, f2, e1, f1);
+  }
+  m.n_bfaces = (int64_t)boundary.size();
+  m.bface2cells.resize(m.n_bfaces);
+  m.bfaceNum.resize(m.n_bfaces);
+  for (size_t k = 0; k < boundary.size(); ++k) {
+    m.bface2cells[k] = (int)boundary[k][0];
+    m.bfaceNum[k] = (int)boundary[k][1];
+    m.nbr[(size_t)boundary[k][0] * Nf + boundary[k][1]] = -(int)k - 1;
+  }
+}
+
+void build_geometry(HostMesh& m) {
+  const int d = m.dim, nv = d + 1;
+  const int nk = d == 2 ? 3 : 6;
+  m.J.resize(m.n_cells);
+  m.Kt.resize((size_t)m.n_cells * nk);
+  for (int64_t e = 0; e < m.n_cells; ++e) {
+    const int* c = &m.cell2nodes[(size_t)e * nv];
+    double xr[3][3];  // xr[ref dir][phys coord]
+    for (int a = 0; a < d; ++a)
+      for (int x = 0; x < d; ++x)
+        xr[a][x] = 0.5 * (m.coords[(size_t)c[a + 1] * d + x] - m.coords[(size_t)c[0] * d + x]);
+    double G[3][3];  // G[ref dir][phys coord] = d xi_a / d x
+    double J;
+    if (d == 2) {
+      J = xr[0][0] * xr[1][1] - xr[1][0] * xr[0][1];
+      G[0][0] = xr[1][1] / J;
+      G[0][1] = -xr[1][0] / J;
+      G[1][0] = -xr[0][1] / J;
+      G[1][1] = xr[0][0] / J;
+    } else {
+      auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+      };
+      double c12[3], c20[3], c01[3];
+      cross(xr[1], xr[2], c12);
+      cross(xr[2], xr[0], c20);
+      cross(xr[0], xr[1], c01);
+      J = xr[0][0] * c12[0] + xr[0][1] * c12[1] + xr[0][2] * c12[2];
+      for (int x = 0; x < 3; ++x) {
+        G[0][x] = c12[x] / J;
+        G[1][x] = c20[x] / J;
+        G[2][x] = c01[x] / J;
+      }
+    }
+    if (!(J > 0)) throw std::runtime_error("non-positive element Jacobian");
+    m.J[e] = J;
+    auto dotg = [&](int a, int b) {
+      double s = 0;
+      for (int x = 0; x < d; ++x) s += G[a][x] * G[b][x];
+      return s * J;
+    };
+    double* K = &m.Kt[(size_t)e * nk];
+    if (d == 2) {
+      K[0] = dotg(0, 0); K[1] = dotg(0, 1); K[2] = dotg(1, 1);
+    } else {
+      K[0] = dotg(0, 0); K[1] = dotg(0, 1); K[2] = dotg(0, 2);
+      K[3] = dotg(1, 1); K[4] = dotg(1, 2); K[5] = dotg(2, 2);
+    }
+  }
+}
+
+}  // namespace insp

@@ -1,0 +1,46 @@
+// Unstructured simplex meshes for the DG Poisson solve: generation of the
+// jittered benchmark meshes, face connectivity (the OP2 sets faces/bfaces and
+// maps face2cells/bface2cells with local face numbers), and per-element
+// geometric metrics of straight-sided simplices.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace insp {
+
+struct HostMesh {
+  int dim = 0;
+  int64_t n_nodes = 0, n_cells = 0;
+  std::vector<double> coords;   // n_nodes x dim
+  std::vector<int> cell2nodes;  // n_cells x (dim+1), positively oriented
+
+  // Derived connectivity (build_topology).
+  std::vector<int> nbr;         // n_cells x Nf: neighbour cell, or -(bface+1)
+  std::vector<int> nbr_face;    // n_cells x Nf: neighbour's local face (0 on boundary)
+  std::vector<int> nbr_perm;    // n_cells x Nf: vertex permutation index (0 on boundary)
+  int64_t n_faces = 0;          // interior faces
+  std::vector<int> face2cells;  // n_faces x 2 (left cell < right cell)
+  std::vector<int> faceNum;     // n_faces x 2 local face numbers
+  int64_t n_bfaces = 0;
+  std::vector<int> bface2cells; // n_bfaces
+  std::vector<int> bfaceNum;    // n_bfaces
+
+  // Geometry (build_geometry): per cell J and the metric K[r][s] = grad(xi_r).grad(xi_s)
+  // scaled by J (Ktilde), stored as 3 (2D: rr, rs, ss) or 6 (3D: rr, rs, rt, ss, st, tt).
+  std::vector<double> J;        // n_cells
+  std::vector<double> Kt;       // n_cells x (3|6)
+};
+
+// Unit square, nx*ny cells, each split into two triangles along the (i,j)-(i+1,j+1)
+// diagonal; interior vertices jittered uniformly by +-jitter*h per coordinate.
+HostMesh make_square_mesh(int nx, int ny, double jitter, uint64_t seed);
+// Unit cube, n^3 hexes, each split into 6 Kuhn tetrahedra sharing the main
+// diagonal; interior vertices jittered by +-jitter*h per coordinate.
+HostMesh make_cube_mesh(int n, double jitter, uint64_t seed);
+
+// Orients cells positively, then builds faces and neighbour tables.
+void build_topology(HostMesh& m);
+void build_geometry(HostMesh& m);
+
+}  // namespace insp

@@ -1,0 +1,725 @@
+#include "solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace insp {
+
+namespace {
+// Device scalar slots.
+enum : int {
+  S_RZ = 0, S_PAP = 1, S_RR = 2, S_RZN = 3, S_BNUM = 4,
+  S_CG_A = 8, S_CG_PAP = 9, S_CG_B = 10,
+  S_EIG_TOP = 12, S_EIG_BOT = 13, S_EIG_NRM = 14,
+  S_MEAN = 16, S_BB = 17,
+  S_COUNT = 32
+};
+
+int64_t round_up(int64_t v, int64_t m) { return ((v + m - 1) / m) * m; }
+}  // namespace
+
+template <typename T>
+struct PoissonSolver::Work {
+  DBuf<T> x, b, ax, d;     // V-cycle vectors
+  DBuf<T> r, z, p, Ap;     // coarse CG vectors (coarsest level only)
+  DBuf<T> tr;              // face traces
+  DBuf<T> inv;             // inverse diagonal
+  DBuf<T> k;               // factor (empty = constant)
+  DBuf<T> C;               // operator constants
+  DBuf<T> P, Pt;           // transfer to/from the next coarser level
+};
+
+struct PoissonSolver::Level {
+  int N = 0;
+  OpLayout L{};
+  RefElement re;
+  DBuf<double> C64;
+  DBuf<int> tab;
+  DBuf<double> tauS, diag, Ik;
+  double lmax = 0;
+  Work<double> w64;
+  Work<float> w32;
+  template <typename T>
+  Work<T>& work();
+};
+template <>
+PoissonSolver::Work<double>& PoissonSolver::Level::work<double>() { return w64; }
+template <>
+PoissonSolver::Work<float>& PoissonSolver::Level::work<float>() { return w32; }
+
+PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
+    : mesh_(std::move(mesh)), N_(order), stream_(stream) {
+  if (mesh_.dim != 2 && mesh_.dim != 3) throw InvalidArgument("mesh dimension must be 2 or 3");
+  if (order < 1 || order > kMaxOrder) throw InvalidArgument("order must be in 1..4");
+  if (mesh_.n_cells < 1) throw InvalidArgument("mesh has no cells");
+  if (mesh_.nbr.empty()) build_topology(mesh_);
+  if (mesh_.J.empty()) build_geometry(mesh_);
+  fine_re_ = make_ref_element(mesh_.dim, order);
+  if (!stream_) {
+    INSP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  const int64_t E = mesh_.n_cells;
+  stride_ = round_up(E, 64);
+  const int d = mesh_.dim, Nf = d + 1, NK = d == 2 ? 3 : 6;
+  const int nperm = d == 2 ? 2 : 6;
+  std::vector<double> kt((size_t)NK * stride_, 0.0), jj((size_t)stride_, 0.0);
+  std::vector<int> nb((size_t)Nf * stride_, 0), cd((size_t)Nf * stride_, 0);
+  for (int64_t e = 0; e < E; ++e) {
+    for (int k = 0; k < NK; ++k) kt[(size_t)k * stride_ + e] = mesh_.Kt[(size_t)e * NK + k];
+    jj[e] = mesh_.J[e];
+    for (int f = 0; f < Nf; ++f) {
+      const int n = mesh_.nbr[(size_t)e * Nf + f];
+      nb[(size_t)f * stride_ + e] = n;
+      cd[(size_t)f * stride_ + e] =
+          n >= 0 ? mesh_.nbr_face[(size_t)e * Nf + f] * nperm + mesh_.nbr_perm[(size_t)e * Nf + f] : 0;
+    }
+  }
+  Kt_.upload(kt.data(), kt.size());
+  J_.upload(jj.data(), jj.size());
+  nbr_.upload(nb.data(), nb.size());
+  code_.upload(cd.data(), cd.size());
+  std::vector<int> bt((size_t)std::max<int64_t>(mesh_.n_bfaces, 1), 0);
+  bctype_.upload(bt.data(), bt.size());
+  scal_.alloc(S_COUNT);
+  partial_.alloc((size_t)kRedBlocks * 4);
+  INSP_HIP(hipHostMalloc((void**)&host_scal_, S_COUNT * sizeof(double), hipHostMallocDefault));
+}
+
+PoissonSolver::~PoissonSolver() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& pr : ev_trace_) { ev_pool_.push_back(pr.first); ev_pool_.push_back(pr.second); }
+  for (auto& pr : ev_apply_) { ev_pool_.push_back(pr.first); ev_pool_.push_back(pr.second); }
+  std::sort(ev_pool_.begin(), ev_pool_.end());
+  ev_pool_.erase(std::unique(ev_pool_.begin(), ev_pool_.end()), ev_pool_.end());
+  for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
+  if (host_scal_) (void)hipHostFree(host_scal_);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+MeshView PoissonSolver::mesh_view() const {
+  MeshView m;
+  m.dim = mesh_.dim;
+  m.E = mesh_.n_cells;
+  m.stride = stride_;
+  m.Kt = Kt_.p;
+  m.J = J_.p;
+  m.nbr = nbr_.p;
+  m.code = code_.p;
+  m.bctype = bctype_.p;
+  return m;
+}
+
+LevelView PoissonSolver::level_view(int l, bool fp32) const {
+  const Level& lv = *levels_[l];
+  LevelView v;
+  v.N = lv.N;
+  v.C = fp32 ? (const void*)lv.w32.C.p : (const void*)lv.w64.C.p;
+  v.C64 = lv.C64.p;
+  v.tab = lv.tab.p;
+  v.tauS = lv.tauS.p;
+  return v;
+}
+
+void PoissonSolver::set_bc_types(const int* types, int64_t n) {
+  if (n != mesh_.n_bfaces) throw InvalidArgument("bc_types length must equal the number of bfaces");
+  if (n == 0) return;
+  if (!types) throw InvalidArgument("bc_types is null");
+  bctype_.upload(types, (size_t)n);
+  initialised_ = false;
+}
+
+void PoissonSolver::set_factor(const double* k, Layout layout) {
+  if (!k) {
+    var_factor_ = false;
+    factor_.release();
+  } else {
+    factor_.alloc((size_t)np() * stride_);
+    import_vec(k, factor_.p, layout, np());
+    var_factor_ = true;
+  }
+  initialised_ = false;
+}
+
+void PoissonSolver::set_bcs(const double* bc, Layout layout) {
+  if (!bc) {
+    have_bc_ = false;
+    return;
+  }
+  const size_t n = (size_t)mesh_.n_bfaces * nfp();
+  if (n == 0) {
+    have_bc_ = false;
+    return;
+  }
+  bc_.alloc(n);
+  if (layout == Layout::HOST_AOS)
+    INSP_HIP(hipMemcpyAsync(bc_.p, bc, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+  else
+    INSP_HIP(hipMemcpyAsync(bc_.p, bc, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  have_bc_ = true;
+}
+
+void PoissonSolver::set_tol_and_iter(double rtol, double atol, int max_iter) {
+  if (!(rtol >= 0) || !(atol >= 0) || max_iter < 0) throw InvalidArgument("invalid tolerances");
+  rtol_ = rtol;
+  atol_ = atol;
+  max_iter_ = max_iter;
+}
+
+void PoissonSolver::build_levels() {
+  std::vector<int> orders;
+  if (type_ == SolverType::PMULTIGRID) {
+    orders = pmg_.orders;
+    if (orders.empty())
+      for (int n = N_; n >= 1; --n) orders.push_back(n);
+    if (orders.front() != N_) throw InvalidArgument("p-multigrid orders must start at the solver order");
+    for (size_t i = 1; i < orders.size(); ++i)
+      if (orders[i] >= orders[i - 1] || orders[i] < 1)
+        throw InvalidArgument("p-multigrid orders must decrease strictly");
+  } else {
+    orders = {N_};
+  }
+  const int d = mesh_.dim;
+  levels_.clear();
+  for (int N : orders) {
+    auto lv = std::make_unique<Level>();
+    lv->N = N;
+    lv->L = op_layout(d, N);
+    lv->re = make_ref_element(d, N);
+    const OpLayout& L = lv->L;
+    const RefElement& re = lv->re;
+    const int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+    const FmaskTable fm = make_fmask(d, N);
+    for (int i = 0; i < Nf * Nfp; ++i)
+      if (fm.v[i] != re.fmask[i]) throw std::runtime_error("face mask tables disagree");
+    std::vector<double> C((size_t)L.size, 0.0);
+    for (int s = 0; s < d; ++s)
+      for (int j = 0; j < Np; ++j)
+        for (int m = 0; m < Np; ++m)
+          C[L.off_D + (s * Np + j) * Np + m] = re.D[s][(size_t)j * Np + m];
+    for (int r = 0; r < d; ++r)
+      for (int j = 0; j < Np; ++j)
+        for (int i = 0; i < Np; ++i) {
+          double a = 0;
+          for (int k = 0; k < Np; ++k) a += re.D[r][(size_t)k * Np + i] * re.M[(size_t)k * Np + j];
+          C[L.off_Bt + (r * Np + j) * Np + i] = a;
+        }
+    for (int f = 0; f < Nf; ++f)
+      for (int j = 0; j < Nfp; ++j)
+        for (int i = 0; i < Nfp; ++i)
+          C[L.off_MfT + (f * Nfp + j) * Nfp + i] = re.Mface[((size_t)f * Nfp + i) * Nfp + j];
+    for (int f = 0; f < Nf; ++f)
+      for (int r = 0; r < d; ++r)
+        for (int j = 0; j < Nfp; ++j)
+          for (int i = 0; i < Np; ++i) {
+            double a = 0;
+            for (int l = 0; l < Nfp; ++l)
+              a += re.D[r][(size_t)re.fmask[f * Nfp + l] * Np + i] *
+                   re.Mface[((size_t)f * Nfp + l) * Nfp + j];
+            C[L.off_Wt + ((f * d + r) * Nfp + j) * Np + i] = a;
+          }
+    for (int i = 0; i < Np * Np; ++i) C[L.off_M + i] = re.M[i];
+    lv->C64.upload(C.data(), C.size());
+    std::vector<int> tab((size_t)L.tab_size, -1);
+    for (int i = 0; i < Nf * Nfp; ++i) tab[L.tab_fmask + i] = re.fmask[i];
+    for (int f = 0; f < Nf; ++f)
+      for (int fn = 0; fn < Nf; ++fn)
+        for (int p = 0; p < L.nperm; ++p)
+          for (int j = 0; j < Nfp; ++j) {
+            const size_t idx = (((size_t)f * Nf + fn) * L.nperm + p) * Nfp + j;
+            const int q = re.nbr_face_node[idx];
+            tab[L.tab_nbr + idx] = (re.fmask[fn * Nfp + q] << 6) | (fn * Nfp + q);
+          }
+    for (int f = 0; f < Nf; ++f)
+      for (int l = 0; l < Nfp; ++l) tab[L.tab_finv + f * Np + re.fmask[f * Nfp + l]] = l;
+    lv->tab.upload(tab.data(), tab.size());
+    lv->tauS.alloc((size_t)Nf * stride_);
+    lv->diag.alloc((size_t)Np * stride_);
+    // operator constants in both precisions
+    lv->w64.C.upload(C.data(), C.size());
+    std::vector<float> Cf(C.begin(), C.end());
+    lv->w32.C.upload(Cf.data(), Cf.size());
+    levels_.push_back(std::move(lv));
+  }
+  for (size_t l = 0; l + 1 < levels_.size(); ++l) {
+    Level& F = *levels_[l];
+    Level& Cc = *levels_[l + 1];
+    const int npf = F.L.Np, npc = Cc.L.Np;
+    std::vector<double> P = interp_matrix(F.re, Cc.re);  // npf x npc
+    std::vector<double> Pt = mat_transpose(P, npf, npc);
+    std::vector<double> Ik = interp_matrix(Cc.re, F.re);  // npc x npf
+    F.w64.P.upload(P.data(), P.size());
+    F.w64.Pt.upload(Pt.data(), Pt.size());
+    std::vector<float> Pf(P.begin(), P.end()), Ptf(Pt.begin(), Pt.end());
+    F.w32.P.upload(Pf.data(), Pf.size());
+    F.w32.Pt.upload(Ptf.data(), Ptf.size());
+    F.Ik.upload(Ik.data(), Ik.size());
+  }
+}
+
+void PoissonSolver::ensure_factor_levels() {
+  const int64_t E = mesh_.n_cells;
+  for (size_t l = 0; l < levels_.size(); ++l) {
+    Level& lv = *levels_[l];
+    if (!var_factor_) {
+      lv.w64.k.release();
+      lv.w32.k.release();
+      continue;
+    }
+    const size_t n = (size_t)lv.L.Np * stride_;
+    lv.w64.k.alloc(n);
+    if (l == 0) {
+      copy<double>((int64_t)n, factor_.p, lv.w64.k.p, stream_);
+    } else {
+      Level& F = *levels_[l - 1];
+      launch_elem_matvec<double>(mesh_.dim, lv.N, F.N, E, stride_, F.Ik.p, F.w64.k.p, lv.w64.k.p,
+                                 false, stream_);
+    }
+    if (prec_ == Precision::MIXED || l == 0) {
+      lv.w32.k.alloc(n);
+      convert<double, float>((int64_t)n, lv.w64.k.p, lv.w32.k.p, stream_);
+    }
+  }
+}
+
+void PoissonSolver::compute_diagonals() {
+  const MeshView mv = mesh_view();
+  for (size_t l = 0; l < levels_.size(); ++l) {
+    Level& lv = *levels_[l];
+    const int Np = lv.L.Np;
+    launch_diag(mv, level_view((int)l, false), var_factor_ ? lv.w64.k.p : nullptr, lv.diag.p, stream_);
+    lv.w64.inv.alloc((size_t)Np * stride_);
+    reciprocal<double>(Np, mesh_.n_cells, stride_, lv.diag.p, lv.w64.inv.p, stream_);
+    lv.w32.inv.alloc((size_t)Np * stride_);
+    reciprocal<float>(Np, mesh_.n_cells, stride_, lv.diag.p, lv.w32.inv.p, stream_);
+  }
+}
+
+double PoissonSolver::read_scalar(int slot) {
+  INSP_HIP(hipMemcpyAsync(host_scal_, scal_.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  return host_scal_[slot];
+}
+
+void PoissonSolver::estimate_eigenvalues() {
+  // Power iteration on D^-1 A with a Rayleigh quotient (the reference's
+  // set_rand_vec / vec_norm / vec_normalise / rayleigh_quotient kernels).
+  RedScratch rs{partial_.p};
+  for (size_t l = 0; l + 1 < levels_.size(); ++l) {
+    Level& lv = *levels_[l];
+    Work<double>& w = lv.w64;
+    const int Np = lv.L.Np;
+    const int64_t n = (int64_t)Np * stride_;
+    DBuf<double> v, av, tr_save;
+    v.alloc(n);
+    av.alloc(n);
+    if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
+    rand_fill(Np, mesh_.n_cells, stride_, 0x5eedULL + 977ULL * lv.N, v.p, stream_);
+    double lam = 0;
+    for (int it = 0; it < std::max(1, pmg_.max_iter_eig); ++it) {
+      apply_level<double>((int)l, v.p, av.p);
+      jacobi<double>(n, av.p, w.inv.p, av.p, stream_);
+      dot<double>(n, v.p, av.p, v.p, v.p, 2, scal_.p, S_EIG_TOP, rs, stream_);
+      dot<double>(n, av.p, av.p, nullptr, nullptr, 1, scal_.p, S_EIG_NRM, rs, stream_);
+      read_scalar(0);
+      lam = host_scal_[S_EIG_TOP] / host_scal_[S_EIG_BOT];
+      const double nrm = std::sqrt(host_scal_[S_EIG_NRM]);
+      if (!(nrm > 0) || !std::isfinite(nrm)) break;
+      axpby<double>(n, 1.0 / nrm, av.p, 0.0, v.p, stream_);
+    }
+    lv.lmax = lam;
+  }
+}
+
+void PoissonSolver::init() {
+  build_levels();
+  ensure_factor_levels();
+  const MeshView mv = mesh_view();
+  // Every p-multigrid level uses the finest order in the penalty, as the reference's
+  // kernels use the program-wide DG_ORDER (fvmf_2d_calc_tau_faces.h:9); this keeps the
+  // rediscretised coarse operators equal to the Galerkin products P^T A P.
+  for (size_t l = 0; l < levels_.size(); ++l)
+    launch_tau(mv, level_view((int)l, false), var_factor_ ? levels_[l]->w64.k.p : nullptr, N_,
+               levels_[l]->tauS.p, stream_);
+  compute_diagonals();
+  // work vectors
+  const bool mixed = prec_ == Precision::MIXED;
+  for (size_t l = 0; l < levels_.size(); ++l) {
+    Level& lv = *levels_[l];
+    const size_t n = (size_t)lv.L.Np * stride_;
+    const size_t nt = (size_t)lv.L.Nf * lv.L.Nfp * stride_;
+    auto setup = [&](auto& w) {
+      w.tr.alloc(nt);
+      if (type_ == SolverType::PMULTIGRID) {
+        w.x.alloc(n); w.b.alloc(n); w.ax.alloc(n); w.d.alloc(n);
+        if (l + 1 == levels_.size()) { w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n); }
+      }
+    };
+    setup(lv.w64);
+    if (mixed) setup(lv.w32);
+  }
+  const size_t n0 = (size_t)np() * stride_;
+  b_.alloc(n0); x_.alloc(n0); r_.alloc(n0); z_.alloc(n0); zold_.alloc(n0); p_.alloc(n0); Ap_.alloc(n0);
+  if (type_ == SolverType::PMULTIGRID) estimate_eigenvalues();
+  INSP_HIP(hipStreamSynchronize(stream_));
+  initialised_ = true;
+}
+
+hipEvent_t PoissonSolver::ev_take() {
+  if (!ev_pool_.empty()) {
+    hipEvent_t e = ev_pool_.back();
+    ev_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  INSP_HIP(hipEventCreate(&e));
+  return e;
+}
+
+template <typename T>
+void PoissonSolver::apply_level(int l, const T* u, T* out) {
+  Level& lv = *levels_[l];
+  Work<T>& w = lv.work<T>();
+  if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
+  const MeshView mv = mesh_view();
+  const LevelView v = level_view(l, std::is_same<T, float>::value);
+  const T* k = var_factor_ ? w.k.p : nullptr;
+  const bool prof = profiling_ && l == 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  if (prof) {
+    e0 = ev_take(); e1 = ev_take(); e2 = ev_take();
+    INSP_HIP(hipEventRecord(e0, stream_));
+  }
+  launch_trace<T>(mv, v, u, k, w.tr.p, stream_);
+  if (prof) INSP_HIP(hipEventRecord(e1, stream_));
+  launch_apply<T>(mv, v, u, w.tr.p, k, out, stream_);
+  if (prof) {
+    INSP_HIP(hipEventRecord(e2, stream_));
+    ev_trace_.push_back({e0, e1});
+    ev_apply_.push_back({e1, e2});
+  }
+}
+
+KernelTiming PoissonSolver::take_timing() {
+  INSP_HIP(hipStreamSynchronize(stream_));
+  for (auto& pr : ev_trace_) {
+    float ms = 0;
+    INSP_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    timing_.trace_ms += ms;
+    timing_.trace_launches++;
+  }
+  for (auto& pr : ev_apply_) {
+    float ms = 0;
+    INSP_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    timing_.apply_ms += ms;
+    timing_.apply_launches++;
+  }
+  std::vector<hipEvent_t> used;
+  for (auto& pr : ev_trace_) { used.push_back(pr.first); used.push_back(pr.second); }
+  for (auto& pr : ev_apply_) used.push_back(pr.second);
+  ev_pool_.insert(ev_pool_.end(), used.begin(), used.end());
+  ev_trace_.clear();
+  ev_apply_.clear();
+  KernelTiming t = timing_;
+  timing_ = KernelTiming{};
+  return t;
+}
+
+template <typename T>
+void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
+  Level& lv = *levels_[l];
+  Work<T>& w = lv.work<T>();
+  const int64_t n = (int64_t)lv.L.Np * stride_;
+  const int K = l < (int)pmg_.cheb_orders.size() ? std::max(1, pmg_.cheb_orders[l]) : 3;
+  const double hi = pmg_.eig_safety * lv.lmax;
+  const double lo = pmg_.cheb_min_factor * hi;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  if (zero_init) {
+    cheb_step<T>(n, f, nullptr, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), true, stream_);
+  } else {
+    apply_level<T>(l, x, w.ax.p);
+    ++smoother_applies_;
+    cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), false, stream_);
+  }
+  for (int k = 2; k <= K; ++k) {
+    apply_level<T>(l, x, w.ax.p);
+    ++smoother_applies_;
+    const double rho = 1.0 / (2.0 * sigma - rho_old);
+    cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, (T)(rho * rho_old), (T)(2.0 * rho / delta),
+                 false, stream_);
+    rho_old = rho;
+  }
+}
+
+template <typename T>
+void PoissonSolver::coarse_cg(int l) {
+  Level& lv = *levels_[l];
+  Work<T>& w = lv.work<T>();
+  const int64_t n = (int64_t)lv.L.Np * stride_;
+  RedScratch rs{partial_.p};
+  fill<T>(n, w.x.p, (T)0, stream_);
+  copy<T>(n, w.b.p, w.r.p, stream_);
+  jacobi<T>(n, w.r.p, w.inv.p, w.z.p, stream_);
+  copy<T>(n, w.z.p, w.p.p, stream_);
+  dot<T>(n, w.r.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
+  int a = S_CG_A, b = S_CG_B;
+  for (int it = 0; it < pmg_.coarse_iters; ++it) {
+    apply_level<T>(l, w.p.p, w.Ap.p);
+    ++smoother_applies_;
+    dot<T>(n, w.p.p, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
+    cg_update_xr<T>(n, w.x.p, w.r.p, w.p.p, w.Ap.p, w.inv.p, w.z.p, scal_.p, a, S_CG_PAP, b, rs,
+                    stream_);
+    cg_update_p<T>(n, w.p.p, w.z.p, scal_.p, b, a, stream_);
+    std::swap(a, b);
+  }
+}
+
+template <typename T>
+void PoissonSolver::vcycle(int l) {
+  Level& lv = *levels_[l];
+  Work<T>& w = lv.work<T>();
+  if (l + 1 == (int)levels_.size()) {
+    coarse_cg<T>(l);
+    return;
+  }
+  const int64_t n = (int64_t)lv.L.Np * stride_;
+  const int pre = l < (int)pmg_.pre_it.size() ? pmg_.pre_it[l] : 1;
+  const int post = l < (int)pmg_.post_it.size() ? pmg_.post_it[l] : 1;
+  if (pre <= 0) fill<T>(n, w.x.p, (T)0, stream_);
+  for (int s = 0; s < pre; ++s) chebyshev<T>(l, w.b.p, w.x.p, s == 0);
+  apply_level<T>(l, w.x.p, w.ax.p);
+  ++smoother_applies_;
+  axpby<T>(n, (T)1, w.b.p, (T)-1, w.ax.p, stream_);  // ax = b - A x
+  Level& cl = *levels_[l + 1];
+  Work<T>& cw = cl.work<T>();
+  launch_elem_matvec<T>(mesh_.dim, cl.N, lv.N, mesh_.n_cells, stride_, w.Pt.p, w.ax.p, cw.b.p,
+                        false, stream_);
+  vcycle<T>(l + 1);
+  launch_elem_matvec<T>(mesh_.dim, lv.N, cl.N, mesh_.n_cells, stride_, w.P.p, cw.x.p, w.x.p, true,
+                        stream_);
+  for (int s = 0; s < post; ++s) chebyshev<T>(l, w.b.p, w.x.p, false);
+}
+
+void PoissonSolver::precondition(const double* r, double* z) {
+  const int64_t n = (int64_t)np() * stride_;
+  Level& l0 = *levels_[0];
+  if (type_ == SolverType::JACOBI_PCG) {
+    jacobi<double>(n, r, l0.w64.inv.p, z, stream_);
+  } else if (prec_ == Precision::FP64) {
+    copy<double>(n, r, l0.w64.b.p, stream_);
+    vcycle<double>(0);
+    copy<double>(n, l0.w64.x.p, z, stream_);
+  } else {
+    convert<double, float>(n, r, l0.w32.b.p, stream_);
+    vcycle<float>(0);
+    convert<float, double>(n, l0.w32.x.p, z, stream_);
+  }
+  if (nullspace_) {
+    RedScratch rs{partial_.p};
+    vsum<double>(n, z, scal_.p, S_MEAN, rs, stream_);
+    sub_mean<double>(np(), mesh_.n_cells, stride_, z, scal_.p, S_MEAN,
+                     (double)np() * mesh_.n_cells, stream_);
+  }
+}
+
+void PoissonSolver::import_vec(const double* src, double* dst, Layout layout, int rows) {
+  const int64_t E = mesh_.n_cells;
+  if (!src) throw InvalidArgument("null vector");
+  if (layout == Layout::HOST_AOS) {
+    DBuf<double> tmp;
+    tmp.alloc((size_t)rows * E);
+    INSP_HIP(hipMemcpyAsync(tmp.p, src, (size_t)rows * E * sizeof(double), hipMemcpyHostToDevice,
+                            stream_));
+    soa_from_aos(rows, E, stride_, tmp.p, dst, stream_);
+    INSP_HIP(hipStreamSynchronize(stream_));
+  } else if (layout == Layout::DEVICE_AOS) {
+    soa_from_aos(rows, E, stride_, src, dst, stream_);
+  } else {
+    if (src != dst)
+      INSP_HIP(hipMemcpyAsync(dst, src, (size_t)rows * stride_ * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
+    zero_padding<double>(rows, E, stride_, dst, stream_);
+  }
+}
+
+void PoissonSolver::export_vec(const double* src, double* dst, Layout layout, int rows) {
+  const int64_t E = mesh_.n_cells;
+  if (!dst) throw InvalidArgument("null vector");
+  if (layout == Layout::HOST_AOS) {
+    DBuf<double> tmp;
+    tmp.alloc((size_t)rows * E);
+    aos_from_soa(rows, E, stride_, src, tmp.p, stream_);
+    INSP_HIP(hipMemcpyAsync(dst, tmp.p, (size_t)rows * E * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    INSP_HIP(hipStreamSynchronize(stream_));
+  } else if (layout == Layout::DEVICE_AOS) {
+    aos_from_soa(rows, E, stride_, src, dst, stream_);
+  } else if (src != dst) {
+    INSP_HIP(hipMemcpyAsync(dst, src, (size_t)rows * stride_ * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream_));
+  }
+}
+
+int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveStats* stats) {
+  if (!initialised_) init();
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = (int64_t)np() * stride_;
+  RedScratch rs{partial_.p};
+  smoother_applies_ = 0;
+  int64_t fine_applies = 0;
+  import_vec(rhs, b_.p, layout, np());
+  if (have_bc_)
+    launch_apply_bc(mesh_view(), level_view(0, false), bc_.p, var_factor_ ? levels_[0]->w64.k.p : nullptr,
+                    b_.p, stream_);
+  if (nullspace_) {
+    vsum<double>(n, b_.p, scal_.p, S_MEAN, rs, stream_);
+    sub_mean<double>(np(), mesh_.n_cells, stride_, b_.p, scal_.p, S_MEAN,
+                     (double)np() * mesh_.n_cells, stream_);
+  }
+  import_vec(ans, x_.p, layout, np());
+  dot<double>(n, b_.p, b_.p, nullptr, nullptr, 1, scal_.p, S_BB, rs, stream_);
+  const double bnorm = std::sqrt(read_scalar(S_BB));
+  SolveStats st;
+  st.bnorm = bnorm;
+  int status = 1;
+  int it = 0;
+  double rr_true = 0;
+  if (!std::isfinite(bnorm)) {
+    status = -3;
+  } else {
+    const double tol = std::max(rtol_ * bnorm, atol_);
+    for (int restart = 0;; ++restart) {
+      apply_level<double>(0, x_.p, Ap_.p);
+      ++fine_applies;
+      residual(n, b_.p, Ap_.p, r_.p, scal_.p, S_RR, rs, stream_);
+      rr_true = read_scalar(S_RR);
+      if (!std::isfinite(rr_true)) { status = -3; break; }
+      if (std::sqrt(rr_true) <= tol) { status = 0; break; }
+      if (it >= max_iter_ || restart > 50) { status = 1; break; }
+      precondition(r_.p, z_.p);
+      dot<double>(n, r_.p, z_.p, nullptr, nullptr, 1, scal_.p, S_RZ, rs, stream_);
+      copy<double>(n, z_.p, p_.p, stream_);
+      bool breakdown = false;
+      while (it < max_iter_) {
+        apply_level<double>(0, p_.p, Ap_.p);
+        ++fine_applies;
+        dot<double>(n, p_.p, Ap_.p, nullptr, nullptr, 1, scal_.p, S_PAP, rs, stream_);
+        cg_update_xr<double>(n, x_.p, r_.p, p_.p, Ap_.p, nullptr, nullptr, scal_.p, S_RZ, S_PAP,
+                             S_RR, rs, stream_);
+        ++it;
+        read_scalar(0);
+        const double pap = host_scal_[S_PAP], rr = host_scal_[S_RR];
+        if (!(pap > 0) || !std::isfinite(rr) || !std::isfinite(pap)) { breakdown = true; break; }
+        if (std::sqrt(rr) <= tol) break;
+        std::swap(z_.p, zold_.p);
+        precondition(r_.p, z_.p);
+        dot<double>(n, r_.p, z_.p, nullptr, nullptr, 1, scal_.p, S_RZN, rs, stream_);
+        dot_flex<double>(n, r_.p, z_.p, zold_.p, scal_.p, S_BNUM, rs, stream_);
+        cg_update_p<double>(n, p_.p, z_.p, scal_.p, S_BNUM, S_RZ, stream_);
+        scal_copy(scal_.p, S_RZ, S_RZN, stream_);
+      }
+      if (breakdown) { status = -3; break; }
+    }
+  }
+  if (status != -3 && nullspace_) {
+    vsum<double>(n, x_.p, scal_.p, S_MEAN, rs, stream_);
+    sub_mean<double>(np(), mesh_.n_cells, stride_, x_.p, scal_.p, S_MEAN,
+                     (double)np() * mesh_.n_cells, stream_);
+  }
+  export_vec(x_.p, ans, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+  const auto t1 = std::chrono::steady_clock::now();
+  st.iterations = it;
+  st.converged = status == 0;
+  st.abs_residual = std::sqrt(rr_true);
+  st.rel_residual = bnorm > 0 ? st.abs_residual / bnorm : 0.0;
+  st.seconds = std::chrono::duration<double>(t1 - t0).count();
+  st.fine_applies = fine_applies;
+  st.smoother_applies = smoother_applies_;
+  if (stats) *stats = st;
+  return status;
+}
+
+void PoissonSolver::mult(const double* in, double* out, Layout layout, bool fp32) {
+  if (!initialised_) init();
+  const int64_t n = (int64_t)np() * stride_;
+  import_vec(in, b_.p, layout, np());
+  if (!fp32) {
+    apply_level<double>(0, b_.p, Ap_.p);
+  } else {
+    Level& l0 = *levels_[0];
+    DBuf<float> xf, yf;
+    xf.alloc(n);
+    yf.alloc(n);
+    if (var_factor_ && l0.w32.k.empty()) {
+      l0.w32.k.alloc(n);
+      convert<double, float>(n, l0.w64.k.p, l0.w32.k.p, stream_);
+    }
+    convert<double, float>(n, b_.p, xf.p, stream_);
+    apply_level<float>(0, xf.p, yf.p);
+    convert<float, double>(n, yf.p, Ap_.p, stream_);
+    INSP_HIP(hipStreamSynchronize(stream_));
+  }
+  export_vec(Ap_.p, out, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::diag(double* out, Layout layout) {
+  if (!initialised_) init();
+  export_vec(levels_[0]->diag.p, out, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::apply_bc(double* rhs, Layout layout) {
+  if (!initialised_) init();
+  import_vec(rhs, b_.p, layout, np());
+  if (have_bc_)
+    launch_apply_bc(mesh_view(), level_view(0, false), bc_.p,
+                    var_factor_ ? levels_[0]->w64.k.p : nullptr, b_.p, stream_);
+  export_vec(b_.p, rhs, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::mass(const double* in, double* out, Layout layout) {
+  if (!initialised_) init();
+  import_vec(in, b_.p, layout, np());
+  launch_mass(mesh_view(), level_view(0, false), b_.p, Ap_.p, stream_);
+  export_vec(Ap_.p, out, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::node_coords(double* xyz) const {
+  const int d = mesh_.dim, Np = np();
+  const RefElement& re = fine_re_;
+  const int64_t E = mesh_.n_cells;
+  for (int64_t e = 0; e < E; ++e) {
+    const int* c = &mesh_.cell2nodes[(size_t)e * (d + 1)];
+    for (int i = 0; i < Np; ++i) {
+      double lam[4];
+      const double r = re.r[i], s = re.s[i], t = d == 3 ? re.t[i] : 0.0;
+      if (d == 2) {
+        lam[0] = -(r + s) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2;
+      } else {
+        lam[0] = -(1 + r + s + t) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2; lam[3] = (1 + t) / 2;
+      }
+      for (int x = 0; x < d; ++x) {
+        double v = 0;
+        for (int k = 0; k <= d; ++k) v += lam[k] * mesh_.coords[(size_t)c[k] * d + x];
+        xyz[((size_t)x * E + e) * Np + i] = v;
+      }
+    }
+  }
+}
+
+double PoissonSolver::level_lambda_max(int l) const {
+  if (l < 0 || l >= (int)levels_.size()) throw InvalidArgument("level out of range");
+  return levels_[l]->lmax;
+}
+
+}  // namespace insp

@@ -1,0 +1,204 @@
+// C++ host side of the Poisson solve: the matrix-free SIPG operator and the
+// linear solvers behind the C ABI. It mirrors the OP2-DG-Toolkit interface the
+// reference calls (src/solvers/2d/ins_solver.cpp:91-96,205-207,324-325):
+// PoissonMatrix::{set_bc_types, set_factor, mult} and
+// LinearSolver::{set_bcs, set_nullspace, set_tol_and_iter, init, solve}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dg_layout.h"
+#include "kernels.h"
+#include "mesh.h"
+#include "refelem.h"
+#include "vec_kernels.h"
+
+namespace insp {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct InvalidArgument : std::runtime_error {
+  explicit InvalidArgument(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define INSP_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t err_ = (call);                                                            \
+    if (err_ != hipSuccess)                                                              \
+      throw ::insp::HipError(std::string(#call) + ": " + hipGetErrorString(err_));       \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    INSP_HIP(hipMalloc(&p, count * sizeof(T)));
+    INSP_HIP(hipMemset(p, 0, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* h, size_t count) {
+    alloc(count);
+    INSP_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  bool empty() const { return p == nullptr; }
+};
+
+enum class SolverType { JACOBI_PCG = 1, PMULTIGRID = 4 };
+enum class Precision { FP64 = 0, MIXED = 1 };
+enum class Layout { HOST_AOS = 0, DEVICE_SOA = 1, DEVICE_AOS = 2 };
+
+struct PMGConfig {
+  std::vector<int> orders;        // finest first, e.g. {3,2,1}; empty = N..1
+  std::vector<int> pre_it, post_it, cheb_orders;
+  double eig_safety = 1.1;        // eigen_val_saftey_factor
+  double cheb_min_factor = 0.1;   // lower Chebyshev bound = factor * upper
+  int max_iter_eig = 15;          // max_iter_eig_approx
+  int coarse_iters = 40;          // Jacobi-CG iterations on the coarsest order
+};
+
+struct SolveStats {
+  int iterations = 0;
+  int converged = 0;
+  double rel_residual = 0;        // true |b - A x| / |b| in FP64
+  double abs_residual = 0;
+  double bnorm = 0;
+  double seconds = 0;
+  int64_t fine_applies = 0;       // FP64 fine-level operator applications
+  int64_t smoother_applies = 0;   // applications inside the preconditioner (all levels)
+};
+
+// Accumulated device time of the fine-level apply kernel, measured with HIP
+// events on the solver's stream while profiling is enabled.
+struct KernelTiming {
+  double trace_ms = 0, apply_ms = 0;
+  int64_t trace_launches = 0, apply_launches = 0;
+};
+
+class PoissonSolver {
+ public:
+  PoissonSolver(HostMesh mesh, int order, hipStream_t stream = nullptr);
+  ~PoissonSolver();
+
+  const HostMesh& mesh() const { return mesh_; }
+  int order() const { return N_; }
+  int np() const { return fine_re_.Np; }
+  int nfp() const { return fine_re_.Nfp; }
+  int64_t stride() const { return stride_; }
+  int64_t n_cells() const { return mesh_.n_cells; }
+  hipStream_t stream() const { return stream_; }
+  const RefElement& ref() const { return fine_re_; }
+
+  void set_bc_types(const int* types, int64_t n);
+  void set_factor(const double* k, Layout layout);  // nullptr = constant coefficient 1
+  void set_bcs(const double* bc, Layout layout);    // bfaces x Nfp, nullptr = none
+  void set_nullspace(bool ns) { nullspace_ = ns; }
+  void set_tol_and_iter(double rtol, double atol, int max_iter);
+  void set_solver(SolverType t) { type_ = t; initialised_ = false; }
+  void set_precision(Precision p) { prec_ = p; initialised_ = false; }
+  void set_pmg(const PMGConfig& c) { pmg_ = c; initialised_ = false; }
+  void init();
+
+  // ans is the initial guess on entry (as in the reference) and the solution on exit.
+  // Returns 0 converged, 1 not converged, -3 breakdown.
+  int solve(const double* rhs, double* ans, Layout layout, SolveStats* stats);
+
+  // out = A in on the finest order (FP64 or FP32 kernels).
+  void mult(const double* in, double* out, Layout layout, bool fp32 = false);
+  void diag(double* out, Layout layout);
+  void apply_bc(double* rhs, Layout layout);
+  void mass(const double* in, double* out, Layout layout);
+  void node_coords(double* xyz) const;  // host, dim x cells x Np (AoS per coordinate)
+  double level_lambda_max(int l) const;
+  int n_levels() const { return (int)levels_.size(); }
+
+  void set_profiling(bool on) { profiling_ = on; }
+  KernelTiming take_timing();
+
+ private:
+  struct Level;
+  template <typename T>
+  struct Work;
+
+  void build_levels();
+  void ensure_factor_levels();
+  void compute_diagonals();
+  void estimate_eigenvalues();
+  template <typename T>
+  void apply_level(int l, const T* u, T* out);
+  template <typename T>
+  void vcycle(int l);
+  template <typename T>
+  void chebyshev(int l, const T* f, T* x, bool zero_init);
+  template <typename T>
+  void coarse_cg(int l);
+  void precondition(const double* r, double* z);
+  void import_vec(const double* src, double* dst, Layout layout, int rows);
+  void export_vec(const double* src, double* dst, Layout layout, int rows);
+  double read_scalar(int slot);
+  MeshView mesh_view() const;
+  LevelView level_view(int l, bool fp32) const;
+
+  HostMesh mesh_;
+  int N_;
+  RefElement fine_re_;
+  hipStream_t stream_;
+  bool own_stream_ = false;
+  int64_t stride_;
+  // device mesh
+  DBuf<double> Kt_, J_;
+  DBuf<int> nbr_, code_, bctype_;
+  DBuf<double> bc_;
+  bool have_bc_ = false;
+  bool var_factor_ = false;
+  DBuf<double> factor_;  // fine-level factor, SoA
+
+  std::vector<std::unique_ptr<Level>> levels_;
+  SolverType type_ = SolverType::PMULTIGRID;
+  Precision prec_ = Precision::FP64;
+  PMGConfig pmg_;
+  bool nullspace_ = false;
+  double rtol_ = 1e-8, atol_ = 1e-9;
+  int max_iter_ = 500;
+  bool initialised_ = false;
+  bool levels_dirty_ = true;
+
+  // outer FCG vectors (FP64, fine level)
+  DBuf<double> b_, x_, r_, z_, zold_, p_, Ap_;
+  DBuf<double> scal_;
+  DBuf<double> partial_;
+  double* host_scal_ = nullptr;  // pinned
+  int64_t smoother_applies_ = 0;
+
+  bool profiling_ = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_trace_, ev_apply_;
+  std::vector<hipEvent_t> ev_pool_;
+  KernelTiming timing_;
+  hipEvent_t ev_take();
+};
+
+}  // namespace insp

@@ -1,0 +1,393 @@
+#include "vec_kernels.h"
+
+#include <algorithm>
+
+namespace insp {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of K values per thread; every thread gets the totals.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+  __shared__ double sh[K][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = wave_sum(v[k]);
+    if (lane == 0) sh[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) t += sh[k][i];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
+template <int K>
+__device__ __forceinline__ void write_partials(double (&v)[K], double* partial) {
+  block_sum<K>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[k * kRedBlocks + blockIdx.x] = v[k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_finalize(int nb, int nvals, const double* __restrict__ partial, double* __restrict__ scal,
+           int slot) {
+  for (int k = 0; k < nvals; ++k) {
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < nb; i += kBlock) v[0] += partial[k * kRedBlocks + i];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) scal[slot + k] = v[0];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dot(int64_t n, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+      const T* __restrict__ d, int nd, double* __restrict__ partial) {
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    v[0] += (double)a[i] * (double)b[i];
+    if (nd > 1) v[1] += (double)c[i] * (double)d[i];
+  }
+  write_partials<2>(v, partial);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dot_flex(int64_t n, const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ zold,
+           double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    v[0] += (double)r[i] * ((double)z[i] - (double)zold[i]);
+  write_partials<1>(v, partial);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ Ax,
+           double* __restrict__ r, double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double t = b[i] - Ax[i];
+    r[i] = t;
+    v[0] += t * t;
+  }
+  write_partials<1>(v, partial);
+}
+
+__device__ __forceinline__ double safe_div(double a, double b) { return b != 0.0 ? a / b : 0.0; }
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_xr(int64_t n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+               const T* __restrict__ Ap, const T* __restrict__ invdiag, T* __restrict__ z,
+               const double* __restrict__ scal, int num, int den, double* __restrict__ partial) {
+  const double alpha = safe_div(scal[num], scal[den]);
+  const T a = (T)alpha;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    x[i] = fma(a, p[i], x[i]);
+    const T ri = fma(-a, Ap[i], r[i]);
+    r[i] = ri;
+    if (invdiag) {
+      const T zi = invdiag[i] * ri;
+      z[i] = zi;
+      v[0] += (double)ri * (double)zi;
+    } else {
+      v[0] += (double)ri * (double)ri;
+    }
+  }
+  write_partials<1>(v, partial);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_p(int64_t n, T* __restrict__ p, const T* __restrict__ z, const double* __restrict__ scal,
+              int num, int den) {
+  const T beta = (T)safe_div(scal[num], scal[den]);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = fma(beta, p[i], z[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_jacobi(int64_t n, const T* __restrict__ r, const T* __restrict__ invdiag, T* __restrict__ z) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    z[i] = invdiag[i] * r[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cheb_step(int64_t n, const T* __restrict__ f, const T* __restrict__ Ax,
+            const T* __restrict__ invdiag, T* __restrict__ d, T* __restrict__ x, T c1, T c2,
+            bool x_zero) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T res = invdiag[i] * (Ax ? f[i] - Ax[i] : f[i]);
+    const T di = c1 == (T)0 ? c2 * res : fma(c1, d[i], c2 * res);
+    d[i] = di;
+    x[i] = x_zero ? di : x[i] + di;
+  }
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kBlock)
+k_convert(int64_t n, const Tin* __restrict__ x, Tout* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = (Tout)x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(int64_t n, T* __restrict__ x, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    x[i] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_axpby(int64_t n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = fma(a, x[i], b * y[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sub_mean(int rows, int64_t E, int64_t stride, T* __restrict__ x, const double* __restrict__ scal,
+           int slot, double count) {
+  const T mean = (T)(scal[slot] / count);
+  const int64_t n = (int64_t)rows * stride;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    if (i % stride < E) x[i] -= mean;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rand_fill(int rows, int64_t E, int64_t stride, uint64_t seed, double* __restrict__ x) {
+  const int64_t n = (int64_t)rows * stride;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    // splitmix64 of (seed, index)
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    const double u = (double)(z >> 11) * 0x1.0p-53;
+    x[i] = (i % stride < E) ? 2.0 * u - 1.0 : 0.0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_vsum(int64_t n, const T* __restrict__ x, double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    v[0] += (double)x[i];
+  write_partials<1>(v, partial);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_soa_from_aos(int rows, int64_t E, int64_t stride, const double* __restrict__ aos,
+               double* __restrict__ soa) {
+  const int64_t n = (int64_t)rows * stride;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t e = i % stride, r = i / stride;
+    soa[i] = e < E ? aos[e * rows + r] : 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_aos_from_soa(int rows, int64_t E, int64_t stride, const double* __restrict__ soa,
+               double* __restrict__ aos) {
+  const int64_t n = (int64_t)rows * E;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t e = i / rows, r = i % rows;
+    aos[i] = soa[r * stride + e];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_zero_padding(int rows, int64_t E, int64_t stride, T* __restrict__ x) {
+  const int64_t pad = stride - E;
+  const int64_t n = (int64_t)rows * pad;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    x[(i / pad) * stride + E + i % pad] = (T)0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_reciprocal(int rows, int64_t E, int64_t stride, const double* __restrict__ diag, T* __restrict__ inv) {
+  const int64_t n = (int64_t)rows * stride;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    inv[i] = (i % stride < E) ? (T)(1.0 / diag[i]) : (T)0;
+}
+
+__global__ void k_scal_copy(double* scal, int dst, int src) {
+  if (threadIdx.x == 0) scal[dst] = scal[src];
+}
+
+int grid_stream(int64_t n) {
+  const int64_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
+}
+
+}  // namespace
+
+int red_blocks(int64_t n) {
+  const int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
+template <typename T>
+void dot(int64_t n, const T* a, const T* b, const T* c, const T* d, int nd, double* scal, int slot,
+         RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(kBlock), 0, s, n, a, b, c, d, nd, rs.partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nd, rs.partial, scal, slot);
+}
+
+template <typename T>
+void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, int slot,
+              RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_dot_flex<T>), dim3(nb), dim3(kBlock), 0, s, n, r, z, zold, rs.partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+}
+
+void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
+              RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+}
+
+template <typename T>
+void cg_update_xr(int64_t n, T* x, T* r, const T* p, const T* Ap, const T* invdiag, T* z,
+                  double* scal, int num, int den, int out, RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(nb), dim3(kBlock), 0, s, n, x, r, p, Ap, invdiag, z,
+                     scal, num, den, rs.partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, out);
+}
+
+template <typename T>
+void cg_update_p(int64_t n, T* p, const T* z, const double* scal, int num, int den, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update_p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p, z, scal,
+                     num, den);
+}
+
+template <typename T>
+void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s) {
+  hipLaunchKernelGGL((k_jacobi<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, invdiag, z);
+}
+
+template <typename T>
+void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x, T c1, T c2,
+               bool x_zero, hipStream_t s) {
+  hipLaunchKernelGGL((k_cheb_step<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, f, Ax, invdiag,
+                     d, x, c1, c2, x_zero);
+}
+
+template <typename Tin, typename Tout>
+void convert(int64_t n, const Tin* x, Tout* y, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y);
+}
+
+template <typename T>
+void fill(int64_t n, T* x, T v, hipStream_t s) {
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, v);
+}
+
+template <typename T>
+void copy(int64_t n, const T* x, T* y, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y);
+}
+
+template <typename T>
+void axpby(int64_t n, T a, const T* x, T b, T* y, hipStream_t s) {
+  hipLaunchKernelGGL((k_axpby<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+}
+
+template <typename T>
+void sub_mean(int rows, int64_t E, int64_t stride, T* x, const double* scal, int slot,
+              double count, hipStream_t s) {
+  hipLaunchKernelGGL((k_sub_mean<T>), dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
+                     rows, E, stride, x, scal, slot, count);
+}
+
+void rand_fill(int rows, int64_t E, int64_t stride, uint64_t seed, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_rand_fill, dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
+                     rows, E, stride, seed, x);
+}
+
+template <typename T>
+void vsum(int64_t n, const T* x, double* scal, int slot, RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_vsum<T>), dim3(nb), dim3(kBlock), 0, s, n, x, rs.partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+}
+
+void soa_from_aos(int rows, int64_t E, int64_t stride, const double* aos, double* soa,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_soa_from_aos, dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
+                     rows, E, stride, aos, soa);
+}
+
+void aos_from_soa(int rows, int64_t E, int64_t stride, const double* soa, double* aos,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_aos_from_soa, dim3(grid_stream((int64_t)rows * E)), dim3(kBlock), 0, s,
+                     rows, E, stride, soa, aos);
+}
+
+template <typename T>
+void zero_padding(int rows, int64_t E, int64_t stride, T* x, hipStream_t s) {
+  if (stride == E) return;
+  hipLaunchKernelGGL((k_zero_padding<T>), dim3(grid_stream((int64_t)rows * (stride - E))),
+                     dim3(kBlock), 0, s, rows, E, stride, x);
+}
+
+template <typename T>
+void reciprocal(int rows, int64_t E, int64_t stride, const double* diag, T* inv, hipStream_t s) {
+  hipLaunchKernelGGL((k_reciprocal<T>), dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0,
+                     s, rows, E, stride, diag, inv);
+}
+
+void scal_copy(double* scal, int dst, int src, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, scal, dst, src);
+}
+
+#define INSP_INST(T)                                                                              \
+  template void dot<T>(int64_t, const T*, const T*, const T*, const T*, int, double*, int,        \
+                       RedScratch, hipStream_t);                                                  \
+  template void dot_flex<T>(int64_t, const T*, const T*, const T*, double*, int, RedScratch,      \
+                            hipStream_t);                                                         \
+  template void cg_update_xr<T>(int64_t, T*, T*, const T*, const T*, const T*, T*, double*, int,  \
+                                int, int, RedScratch, hipStream_t);                               \
+  template void cg_update_p<T>(int64_t, T*, const T*, const double*, int, int, hipStream_t);      \
+  template void jacobi<T>(int64_t, const T*, const T*, T*, hipStream_t);                          \
+  template void cheb_step<T>(int64_t, const T*, const T*, const T*, T*, T*, T, T, bool,           \
+                             hipStream_t);                                                        \
+  template void fill<T>(int64_t, T*, T, hipStream_t);                                             \
+  template void copy<T>(int64_t, const T*, T*, hipStream_t);                                      \
+  template void axpby<T>(int64_t, T, const T*, T, T*, hipStream_t);                               \
+  template void sub_mean<T>(int, int64_t, int64_t, T*, const double*, int, double, hipStream_t); \
+  template void vsum<T>(int64_t, const T*, double*, int, RedScratch, hipStream_t);                \
+  template void zero_padding<T>(int, int64_t, int64_t, T*, hipStream_t);                          \
+  template void reciprocal<T>(int, int64_t, int64_t, const double*, T*, hipStream_t);
+INSP_INST(double)
+INSP_INST(float)
+template void convert<double, float>(int64_t, const double*, float*, hipStream_t);
+template void convert<float, double>(int64_t, const float*, double*, hipStream_t);
+
+}  // namespace insp

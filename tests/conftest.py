@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "ins-solver_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (MI355X)")
+
+
+@pytest.fixture(scope="session")
+def ip():
+    import ins_poisson
+    return ins_poisson
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+    return O

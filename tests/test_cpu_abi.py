@@ -1,0 +1,88 @@
+"""The C-ABI library loads and exports what include/ins_poisson.h declares;
+host-side pieces (reference element, mesh topology) agree with the oracle."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "ins_poisson.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ins_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(ip):
+    lib = ctypes.CDLL(ip.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(ip.EXPORTED) == names
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_product_reference_element_matches_oracle(ip, oracle, dim, N):
+    a, b = oracle.ref_element(dim, N), ip.ref_element(dim, N)
+    assert np.array_equal(a["fmask"], b["fmask"])
+    for k in ("rst", "D", "M", "Mface"):
+        scale = max(1.0, np.abs(a[k]).max())
+        assert np.abs(a[k] - b[k]).max() < 1e-12 * scale, k
+    for nc in range(1, N):
+        assert np.abs(oracle.interp(dim, N, nc) - ip.ref_interp(dim, N, nc)).max() < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["square", "cube"])
+def test_mesh_topology_matches_oracle(ip, oracle, kind):
+    m = ip.Mesh.square(9, 7, jitter=0.1, seed=3) if kind == "square" else ip.Mesh.cube(4, 0.1, 3)
+    coords, c2n = m.arrays()
+    o = oracle.Oracle(coords, c2n, 2)
+    f2c, fnum, b2c, bnum = m.faces()
+    of2c, ofnum = o.faces()
+    ob2c, obnum = o.bfaces()
+    assert np.array_equal(f2c, of2c) and np.array_equal(fnum, ofnum)
+    assert np.array_equal(b2c, ob2c) and np.array_equal(bnum, obnum)
+    nf = m.dim + 1
+    assert 2 * m.n_faces + m.n_bfaces == nf * m.n_cells
+
+
+def test_benchmark_mesh_sizes(ip):
+    m = ip.Mesh.square(32, jitter=0.1, seed=1)
+    assert (m.n_cells, m.n_faces, m.n_bfaces) == (2048, 3008, 128)
+    m = ip.Mesh.cube(8, jitter=0.1, seed=1)
+    assert m.n_cells == 6 * 512 and m.n_bfaces == 6 * 2 * 64
+
+
+def test_jitter_is_seeded_and_bounded(ip):
+    a, _ = ip.Mesh.cube(5, 0.1, 7).arrays()
+    b, _ = ip.Mesh.cube(5, 0.1, 7).arrays()
+    c, _ = ip.Mesh.cube(5, 0.1, 8).arrays()
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    g = np.stack(np.meshgrid(*[np.arange(6) / 5] * 3, indexing="ij"), -1)[..., ::-1]
+    g = g.transpose(2, 1, 0, 3).reshape(-1, 3)[:, ::-1]
+    assert np.abs(a - g).max() <= 0.1 / 5 + 1e-15
+
+
+def test_invalid_arguments_are_reported(ip):
+    with pytest.raises(ip.InsPoissonError) as e:
+        ip.Mesh.square(0)
+    assert e.value.code == ip.ERR_INVALID
+    with pytest.raises(ip.InsPoissonError):
+        ip.ref_element(2, 7)
+    coords = np.array([[0.0, 0], [1, 0], [0, 1]])
+    with pytest.raises(ip.InsPoissonError):
+        ip.Mesh.from_arrays(coords, np.array([[0, 1, 5]]))
+
+
+def test_user_mesh_orientation_fixed(ip):
+    coords = np.array([[0.0, 0], [1, 0], [0, 1], [1, 1]])
+    m = ip.Mesh.from_arrays(coords, np.array([[0, 2, 1], [1, 2, 3]]))
+    _, c2n = m.arrays()
+    for c in c2n:
+        a, b, d = coords[c]
+        assert (b[0] - a[0]) * (d[1] - a[1]) - (d[0] - a[0]) * (b[1] - a[1]) > 0
