@@ -23,11 +23,20 @@
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
+#include <string>
 
 #include "dg_layout.h"
 #include "kernels.h"
 
 namespace insp {
+
+namespace {
+inline void launch_check(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("kernel launch failed: ") + what + ": " + hipGetErrorString(e));
+}
+}  // namespace
 
 namespace {
 
@@ -423,6 +432,7 @@ inline dim3 grid_for(int64_t E) { return dim3((unsigned)((E + 255) / 256)); }
         default: throw std::runtime_error("unsupported order");         \
       }                                                                 \
     }                                                                   \
+    launch_check("SIPG kernel");                                        \
   } while (0)
 
 }  // namespace
@@ -533,6 +543,7 @@ void matvec_rc(int64_t E, int64_t stride, const T* A, const T* x, T* y, bool add
   else
     hipLaunchKernelGGL((k_elem_matvec<T, R, K, false>), grid_for(E), dim3(256), 0, s, E, stride, A,
                        x, y);
+  launch_check("k_elem_matvec");
 }
 
 template <typename T, int DIM, int NR>

@@ -331,6 +331,7 @@ void PoissonSolver::estimate_eigenvalues() {
       if (!(nrm > 0) || !std::isfinite(nrm)) break;
       axpby<double>(n, 1.0 / nrm, av.p, 0.0, v.p, stream_);
     }
+    INSP_HIP(hipStreamSynchronize(stream_));  // v, av are released below
     lv.lmax = lam;
   }
 }
@@ -533,7 +534,7 @@ void PoissonSolver::import_vec(const double* src, double* dst, Layout layout, in
   if (!src) throw InvalidArgument("null vector");
   if (layout == Layout::HOST_AOS) {
     DBuf<double> tmp;
-    tmp.alloc((size_t)rows * E);
+    tmp.alloc((size_t)rows * E, false);
     INSP_HIP(hipMemcpyAsync(tmp.p, src, (size_t)rows * E * sizeof(double), hipMemcpyHostToDevice,
                             stream_));
     soa_from_aos(rows, E, stride_, tmp.p, dst, stream_);
@@ -553,7 +554,7 @@ void PoissonSolver::export_vec(const double* src, double* dst, Layout layout, in
   if (!dst) throw InvalidArgument("null vector");
   if (layout == Layout::HOST_AOS) {
     DBuf<double> tmp;
-    tmp.alloc((size_t)rows * E);
+    tmp.alloc((size_t)rows * E, false);
     aos_from_soa(rows, E, stride_, src, tmp.p, stream_);
     INSP_HIP(hipMemcpyAsync(dst, tmp.p, (size_t)rows * E * sizeof(double), hipMemcpyDeviceToHost,
                             stream_));

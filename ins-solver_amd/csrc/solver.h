@@ -48,12 +48,17 @@ struct DBuf {
     return *this;
   }
   ~DBuf() { release(); }
-  void alloc(size_t count) {
+  // Allocates and zeroes. The zeroing runs on the null stream, which is not ordered with
+  // the solver's non-blocking stream, so it is completed before returning.
+  void alloc(size_t count, bool zero = true) {
     if (count == n && p) return;
     release();
     if (count == 0) return;
     INSP_HIP(hipMalloc(&p, count * sizeof(T)));
-    INSP_HIP(hipMemset(p, 0, count * sizeof(T)));
+    if (zero) {
+      INSP_HIP(hipMemset(p, 0, count * sizeof(T)));
+      INSP_HIP(hipDeviceSynchronize());
+    }
     n = count;
   }
   void upload(const T* h, size_t count) {

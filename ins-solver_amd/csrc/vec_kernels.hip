@@ -1,8 +1,18 @@
 #include "vec_kernels.h"
 
 #include <algorithm>
+#include <stdexcept>
+#include <string>
 
 namespace insp {
+
+namespace {
+inline void launch_check(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("kernel launch failed: ") + what + ": " + hipGetErrorString(e));
+}
+}  // namespace
 
 namespace {
 
@@ -253,23 +263,23 @@ template <typename T>
 void dot(int64_t n, const T* a, const T* b, const T* c, const T* d, int nd, double* scal, int slot,
          RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(kBlock), 0, s, n, a, b, c, d, nd, rs.partial);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nd, rs.partial, scal, slot);
+  hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(kBlock), 0, s, n, a, b, c, d, nd, rs.partial); launch_check("k_dot");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nd, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 template <typename T>
 void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, int slot,
               RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL((k_dot_flex<T>), dim3(nb), dim3(kBlock), 0, s, n, r, z, zold, rs.partial);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+  hipLaunchKernelGGL((k_dot_flex<T>), dim3(nb), dim3(kBlock), 0, s, n, r, z, zold, rs.partial); launch_check("k_dot_flex");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
               RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial); launch_check("k_residual");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 template <typename T>
@@ -277,94 +287,94 @@ void cg_update_xr(int64_t n, T* x, T* r, const T* p, const T* Ap, const T* invdi
                   double* scal, int num, int den, int out, RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
   hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(nb), dim3(kBlock), 0, s, n, x, r, p, Ap, invdiag, z,
-                     scal, num, den, rs.partial);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, out);
+                     scal, num, den, rs.partial); launch_check("k_cg_update_xr");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
 }
 
 template <typename T>
 void cg_update_p(int64_t n, T* p, const T* z, const double* scal, int num, int den, hipStream_t s) {
   hipLaunchKernelGGL((k_cg_update_p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p, z, scal,
-                     num, den);
+                     num, den); launch_check("k_cg_update_p");
 }
 
 template <typename T>
 void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s) {
-  hipLaunchKernelGGL((k_jacobi<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, invdiag, z);
+  hipLaunchKernelGGL((k_jacobi<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, invdiag, z); launch_check("k_jacobi");
 }
 
 template <typename T>
 void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x, T c1, T c2,
                bool x_zero, hipStream_t s) {
   hipLaunchKernelGGL((k_cheb_step<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, f, Ax, invdiag,
-                     d, x, c1, c2, x_zero);
+                     d, x, c1, c2, x_zero); launch_check("k_cheb_step");
 }
 
 template <typename Tin, typename Tout>
 void convert(int64_t n, const Tin* x, Tout* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y);
+  hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y); launch_check("k_convert");
 }
 
 template <typename T>
 void fill(int64_t n, T* x, T v, hipStream_t s) {
-  hipLaunchKernelGGL((k_fill<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, v);
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, v); launch_check("k_fill");
 }
 
 template <typename T>
 void copy(int64_t n, const T* x, T* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y);
+  hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y); launch_check("k_convert");
 }
 
 template <typename T>
 void axpby(int64_t n, T a, const T* x, T b, T* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_axpby<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+  hipLaunchKernelGGL((k_axpby<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, x, b, y); launch_check("k_axpby");
 }
 
 template <typename T>
 void sub_mean(int rows, int64_t E, int64_t stride, T* x, const double* scal, int slot,
               double count, hipStream_t s) {
   hipLaunchKernelGGL((k_sub_mean<T>), dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
-                     rows, E, stride, x, scal, slot, count);
+                     rows, E, stride, x, scal, slot, count); launch_check("k_sub_mean");
 }
 
 void rand_fill(int rows, int64_t E, int64_t stride, uint64_t seed, double* x, hipStream_t s) {
   hipLaunchKernelGGL(k_rand_fill, dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
-                     rows, E, stride, seed, x);
+                     rows, E, stride, seed, x); launch_check("k_rand_fill");
 }
 
 template <typename T>
 void vsum(int64_t n, const T* x, double* scal, int slot, RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL((k_vsum<T>), dim3(nb), dim3(kBlock), 0, s, n, x, rs.partial);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot);
+  hipLaunchKernelGGL((k_vsum<T>), dim3(nb), dim3(kBlock), 0, s, n, x, rs.partial); launch_check("k_vsum");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 void soa_from_aos(int rows, int64_t E, int64_t stride, const double* aos, double* soa,
                   hipStream_t s) {
   hipLaunchKernelGGL(k_soa_from_aos, dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0, s,
-                     rows, E, stride, aos, soa);
+                     rows, E, stride, aos, soa); launch_check("k_soa_from_aos");
 }
 
 void aos_from_soa(int rows, int64_t E, int64_t stride, const double* soa, double* aos,
                   hipStream_t s) {
   hipLaunchKernelGGL(k_aos_from_soa, dim3(grid_stream((int64_t)rows * E)), dim3(kBlock), 0, s,
-                     rows, E, stride, soa, aos);
+                     rows, E, stride, soa, aos); launch_check("k_aos_from_soa");
 }
 
 template <typename T>
 void zero_padding(int rows, int64_t E, int64_t stride, T* x, hipStream_t s) {
   if (stride == E) return;
   hipLaunchKernelGGL((k_zero_padding<T>), dim3(grid_stream((int64_t)rows * (stride - E))),
-                     dim3(kBlock), 0, s, rows, E, stride, x);
+                     dim3(kBlock), 0, s, rows, E, stride, x); launch_check("k_zero_padding");
 }
 
 template <typename T>
 void reciprocal(int rows, int64_t E, int64_t stride, const double* diag, T* inv, hipStream_t s) {
   hipLaunchKernelGGL((k_reciprocal<T>), dim3(grid_stream((int64_t)rows * stride)), dim3(kBlock), 0,
-                     s, rows, E, stride, diag, inv);
+                     s, rows, E, stride, diag, inv); launch_check("k_reciprocal");
 }
 
 void scal_copy(double* scal, int dst, int src, hipStream_t s) {
-  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, scal, dst, src);
+  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, scal, dst, src); launch_check("k_scal_copy");
 }
 
 #define INSP_INST(T)                                                                              \

@@ -218,7 +218,7 @@ class Mesh:
         return f2c, fnum, b2c, bnum
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.ins_mesh_destroy(self._h)
             self._h = None
 
@@ -241,7 +241,7 @@ class PoissonSolver:
         self.last_stats = None
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.ins_poisson_destroy(self._h)
             self._h = None
 

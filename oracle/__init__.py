@@ -116,7 +116,7 @@ class Oracle:
         self.n_bfaces = lib.oracle_n_bfaces(self._h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.oracle_destroy(self._h)
             self._h = None
 
