@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--coarse-iters", type=int, default=40)
-    ap.add_argument("--cpu-n", type=int, default=12, help="cube size of the CPU baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=20, help="cube size of the CPU baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-mixed-leg", action="store_true")
     return ap.parse_args()
