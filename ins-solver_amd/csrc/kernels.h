@@ -27,13 +27,16 @@ struct LevelView {
   const double* tauS;  // [Nf][stride] penalty * sJ
 };
 
-// Fine-level SIPG operator: trace pass + element pass.
+// SIPG operator: trace pass + element pass (order 1: element pass only). With
+// `partial` the element pass also writes per-block partial sums of u.(A u)
+// (apply_blocks(E) of them) for a deterministic finalize.
 template <typename T>
 void launch_trace(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* tr,
                   hipStream_t s);
 template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
-                  T* out, hipStream_t s);
+                  T* out, hipStream_t s, double* partial = nullptr);
+int apply_blocks(int64_t E);
 // Exact diagonal of the operator (double).
 void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
                  hipStream_t s);

@@ -27,6 +27,7 @@
 
 #include "dg_layout.h"
 #include "kernels.h"
+#include "reduce.h"
 
 namespace insp {
 
@@ -98,18 +99,17 @@ k_trace(int64_t E, int64_t stride, const T* __restrict__ u, const double* __rest
   }
 }
 
+// One cell of the element pass; returns u.(A u) over the cell.
 template <int DIM, int N, typename T, bool VAR>
-__global__ void __launch_bounds__(256)
-k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
-        const double* __restrict__ Kt, const double* __restrict__ tauS,
-        const int* __restrict__ nbr, const int* __restrict__ code,
-        const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
-        const int* __restrict__ tab, T* __restrict__ out) {
+__device__ __forceinline__ double
+apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
+           const double* __restrict__ Kt, const double* __restrict__ tauS,
+           const int* __restrict__ nbr, const int* __restrict__ code,
+           const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
+           const int* __restrict__ tab, T* __restrict__ out) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, N);
-  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= E) return;
   T uu[Np], acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -193,8 +193,179 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
       }
     }
   }
+  double dotv = 0.0;
 #pragma unroll
-  for (int i = 0; i < Np; ++i) out[i * stride + e] = acc[i];
+  for (int i = 0; i < Np; ++i) {
+    out[i * stride + e] = acc[i];
+    dotv += (double)uu[i] * (double)acc[i];
+  }
+  return dotv;
+}
+
+template <int DIM, int N, typename T, bool VAR>
+__global__ void __launch_bounds__(256)
+k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
+        const double* __restrict__ Kt, const double* __restrict__ tauS,
+        const int* __restrict__ nbr, const int* __restrict__ code,
+        const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
+        const int* __restrict__ tab, T* __restrict__ out, double* __restrict__ partial) {
+  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
+  double dotv = 0.0;
+  if (e < E) dotv = apply_cell<DIM, N, T, VAR>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, C, tab, out);
+  // optional fused u.(A u) partial sum for CG step lengths
+  if (partial) {
+    const double t = block_sum1<256>(dotv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+
+// Order 1 in one pass: the gradient of a P1 cell is constant, so the neighbour's
+// flux trace k n.grad(u) is recomputed from its Np nodal values and metric instead
+// of being stored by a trace pass. Same operator as apply_cell.
+__device__ __forceinline__ double face_alpha_rt(int dim, int f, int r) {
+  if (dim == 2) {
+    switch (f) {
+      case 0: return r == 1 ? -1.0 : 0.0;
+      case 1: return 1.0;
+      default: return r == 0 ? -1.0 : 0.0;
+    }
+  }
+  switch (f) {
+    case 0: return r == 2 ? -1.0 : 0.0;
+    case 1: return r == 1 ? -1.0 : 0.0;
+    case 2: return 1.0;
+    default: return r == 0 ? -1.0 : 0.0;
+  }
+}
+
+template <int DIM, typename T, bool VAR>
+__global__ void __launch_bounds__(256)
+k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
+           const double* __restrict__ tauS, const int* __restrict__ nbr,
+           const int* __restrict__ code, const int* __restrict__ bctype,
+           const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
+           T* __restrict__ out, double* __restrict__ partial) {
+  constexpr OpLayout L = op_layout(DIM, 1);
+  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  constexpr FmaskTable FM = make_fmask(DIM, 1);
+  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
+  double dotv = 0.0;
+  if (e < E) {
+    T uu[Np], acc[Np], kk[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      uu[m] = u[m * stride + e];
+      acc[m] = 0;
+      kk[m] = VAR ? kfac[m * stride + e] : (T)1;
+    }
+    T K[L.NK];
+#pragma unroll
+    for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
+    T g[DIM];
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T a = 0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], uu[m], a);
+      g[s] = a;
+    }
+    T h[DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      T a = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
+      h[r] = a;
+    }
+#pragma unroll
+    for (int j = 0; j < Np; ++j)
+#pragma unroll
+      for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int i = 0; i < Np; ++i)
+          acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h[r] * kk[j], acc[i]);
+#pragma unroll
+    for (int f = 0; f < Nf; ++f) {
+      T beta[DIM];
+      T tm0 = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) {
+        T b = 0;
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
+        beta[s] = b;
+        tm0 = fma(b, g[s], tm0);
+      }
+      const int nb = nbr[f * stride + e];
+      int mode = 0;
+      int tbase = 0;
+      T tn0 = 0;
+      if (nb >= 0) {
+        const int c = code[f * stride + e];
+        const int fn = c / L.nperm;
+        tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
+        T gn[DIM];
+#pragma unroll
+        for (int s = 0; s < DIM; ++s) gn[s] = 0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m) {
+          const T um = u[m * stride + nb];
+#pragma unroll
+          for (int s = 0; s < DIM; ++s) gn[s] = fma(C[L.off_D + (s * Np) * Np + m], um, gn[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < DIM; ++s) {
+          T b = 0;
+#pragma unroll
+          for (int r = 0; r < DIM; ++r)
+            b += (T)face_alpha_rt(DIM, fn, r) * (T)Kt[kidx(DIM, r, s) * stride + nb];
+          tn0 = fma(b, gn[s], tn0);
+        }
+      } else {
+        mode = bctype[-nb - 1] == 0 ? 1 : 2;
+      }
+      if (mode == 2) continue;
+      const T ts = (T)tauS[f * stride + e];
+#pragma unroll
+      for (int j = 0; j < Nfp; ++j) {
+        const int p = FM.v[f * Nfp + j];
+        const T um = uu[p];
+        const T Tm = kk[p] * tm0;
+        T up, Tp;
+        if (mode == 0) {
+          const int node = tab[tbase + j] >> 6;
+          up = u[(int64_t)node * stride + nb];
+          Tp = (VAR ? kfac[(int64_t)node * stride + nb] : (T)1) * tn0;
+        } else {
+          up = -um;
+          Tp = -Tm;
+        }
+        const T jump = um - up;
+        const T F = (T)0.5 * (ts * jump - (Tm - Tp));
+        const T sig = (T)-0.5 * jump * kk[p];
+#pragma unroll
+        for (int i = 0; i < Nfp; ++i)
+          acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+          const T bs = beta[r] * sig;
+#pragma unroll
+          for (int i = 0; i < Np; ++i)
+            acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < Np; ++i) {
+      out[i * stride + e] = acc[i];
+      dotv += (double)uu[i] * (double)acc[i];
+    }
+  }
+  if (partial) {
+    const double t = block_sum1<256>(dotv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
 }
 
 template <int DIM, int N, bool VAR>
@@ -440,7 +611,7 @@ inline dim3 grid_for(int64_t E) { return dim3((unsigned)((E + 255) / 256)); }
 template <typename T>
 void launch_trace(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* tr,
                   hipStream_t s) {
-  if (m.E == 0) return;
+  if (m.E == 0 || l.N == 1) return;  // order 1 runs in one pass (k_apply_p1)
   const T* C = static_cast<const T*>(l.C);
   if (kfac)
     INSP_DISPATCH_N(m.dim, l.N,
@@ -454,29 +625,50 @@ void launch_trace(const MeshView& m, const LevelView& l, const T* u, const T* kf
 
 template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
-                  T* out, hipStream_t s) {
+                  T* out, hipStream_t s, double* partial) {
   if (m.E == 0) return;
   const T* C = static_cast<const T*>(l.C);
+  if (l.N == 1) {
+    if (m.dim == 2) {
+      if (kfac)
+        hipLaunchKernelGGL((k_apply_p1<2, T, true>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
+                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+      else
+        hipLaunchKernelGGL((k_apply_p1<2, T, false>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
+                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+    } else {
+      if (kfac)
+        hipLaunchKernelGGL((k_apply_p1<3, T, true>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
+                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+      else
+        hipLaunchKernelGGL((k_apply_p1<3, T, false>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
+                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+    }
+    launch_check("k_apply_p1");
+    return;
+  }
   if (kfac)
     INSP_DISPATCH_N(m.dim, l.N,
                     hipLaunchKernelGGL((k_apply<D_, N_, T, true>), grid_for(m.E), dim3(256), 0, s,
                                        m.E, m.stride, u, tr, m.Kt, l.tauS, m.nbr, m.code,
-                                       m.bctype, kfac, C, l.tab, out));
+                                       m.bctype, kfac, C, l.tab, out, partial));
   else
     INSP_DISPATCH_N(m.dim, l.N,
                     hipLaunchKernelGGL((k_apply<D_, N_, T, false>), grid_for(m.E), dim3(256), 0, s,
                                        m.E, m.stride, u, tr, m.Kt, l.tauS, m.nbr, m.code,
-                                       m.bctype, kfac, C, l.tab, out));
+                                       m.bctype, kfac, C, l.tab, out, partial));
 }
+
+int apply_blocks(int64_t E) { return (int)((E + 255) / 256); }
 
 template void launch_trace<double>(const MeshView&, const LevelView&, const double*, const double*,
                                    double*, hipStream_t);
 template void launch_trace<float>(const MeshView&, const LevelView&, const float*, const float*,
                                   float*, hipStream_t);
 template void launch_apply<double>(const MeshView&, const LevelView&, const double*, const double*,
-                                   const double*, double*, hipStream_t);
+                                   const double*, double*, hipStream_t, double*);
 template void launch_apply<float>(const MeshView&, const LevelView&, const float*, const float*,
-                                  const float*, float*, hipStream_t);
+                                  const float*, float*, hipStream_t, double*);
 
 void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
                  hipStream_t s) {

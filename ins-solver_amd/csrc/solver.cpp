@@ -84,7 +84,7 @@ PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
   std::vector<int> bt((size_t)std::max<int64_t>(mesh_.n_bfaces, 1), 0);
   bctype_.upload(bt.data(), bt.size());
   scal_.alloc(S_COUNT);
-  partial_.alloc((size_t)kRedBlocks * 4);
+  partial_.alloc((size_t)std::max<int64_t>(4 * kRedBlocks, apply_blocks(E)));
   INSP_HIP(hipHostMalloc((void**)&host_scal_, S_COUNT * sizeof(double), hipHostMallocDefault));
 }
 
@@ -382,7 +382,7 @@ hipEvent_t PoissonSolver::ev_take() {
 }
 
 template <typename T>
-void PoissonSolver::apply_level(int l, const T* u, T* out) {
+void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
   Level& lv = *levels_[l];
   Work<T>& w = lv.work<T>();
   if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
@@ -397,7 +397,7 @@ void PoissonSolver::apply_level(int l, const T* u, T* out) {
   }
   launch_trace<T>(mv, v, u, k, w.tr.p, stream_);
   if (prof) INSP_HIP(hipEventRecord(e1, stream_));
-  launch_apply<T>(mv, v, u, w.tr.p, k, out, stream_);
+  launch_apply<T>(mv, v, u, w.tr.p, k, out, stream_, partial);
   if (prof) {
     INSP_HIP(hipEventRecord(e2, stream_));
     ev_trace_.push_back({e0, e1});
@@ -471,9 +471,9 @@ void PoissonSolver::coarse_cg(int l) {
   dot<T>(n, w.r.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
   int a = S_CG_A, b = S_CG_B;
   for (int it = 0; it < pmg_.coarse_iters; ++it) {
-    apply_level<T>(l, w.p.p, w.Ap.p);
+    apply_level<T>(l, w.p.p, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
     ++smoother_applies_;
-    dot<T>(n, w.p.p, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
+    finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
     cg_update_xr<T>(n, w.x.p, w.r.p, w.p.p, w.Ap.p, w.inv.p, w.z.p, scal_.p, a, S_CG_PAP, b, rs,
                     stream_);
     cg_update_p<T>(n, w.p.p, w.z.p, scal_.p, b, a, stream_);
@@ -608,9 +608,9 @@ int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveSta
       copy<double>(n, z_.p, p_.p, stream_);
       bool breakdown = false;
       while (it < max_iter_) {
-        apply_level<double>(0, p_.p, Ap_.p);
+        apply_level<double>(0, p_.p, Ap_.p, partial_.p);  // p.Ap fused into the element pass
         ++fine_applies;
-        dot<double>(n, p_.p, Ap_.p, nullptr, nullptr, 1, scal_.p, S_PAP, rs, stream_);
+        finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_PAP, stream_);
         cg_update_xr<double>(n, x_.p, r_.p, p_.p, Ap_.p, nullptr, nullptr, scal_.p, S_RZ, S_PAP,
                              S_RR, rs, stream_);
         ++it;

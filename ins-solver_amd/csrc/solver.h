@@ -154,7 +154,7 @@ class PoissonSolver {
   void compute_diagonals();
   void estimate_eigenvalues();
   template <typename T>
-  void apply_level(int l, const T* u, T* out);
+  void apply_level(int l, const T* u, T* out, double* partial = nullptr);
   template <typename T>
   void vcycle(int l);
   template <typename T>

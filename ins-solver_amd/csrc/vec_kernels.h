@@ -9,16 +9,19 @@
 
 #include <cstdint>
 
+#include "reduce.h"
+
 namespace insp {
 
-constexpr int kRedBlocks = 1024;  // upper bound on reduction partials
-
 struct RedScratch {
-  double* partial;  // kRedBlocks * 4 doubles
+  double* partial;  // max(kRedBlocks, apply blocks) * 4 doubles
 };
 
 int red_blocks(int64_t n);
 
+// scal[slot + k] = sum of the first nb partials of value k (partial[k * kRedBlocks + b]),
+// added in index order by one block.
+void finalize(int nb, int nvals, const double* partial, double* scal, int slot, hipStream_t s);
 // scal[slot+k] = sum(a_k * b_k) for k < nd (nd = 1 or 2); c,d may be null for nd = 1.
 template <typename T>
 void dot(int64_t n, const T* a, const T* b, const T* c, const T* d, int nd, double* scal, int slot,

@@ -18,12 +18,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // Block-wide sum of K values per thread; every thread gets the totals.
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K]) {
@@ -257,6 +251,11 @@ int grid_stream(int64_t n) {
 int red_blocks(int64_t n) {
   const int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
+void finalize(int nb, int nvals, const double* partial, double* scal, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nvals, partial, scal, slot);
+  launch_check("k_finalize");
 }
 
 template <typename T>
