@@ -9,6 +9,18 @@ relative residual). A step is one solve from a zero initial guess with rhs and
 solution resident in HBM (device SoA layout).
 
 Prints one JSON line. `value` is GDoF/s = DoF x solves / wall time.
+
+Beside it the line carries the other forms of the same solve:
+  mixed_vs_fp64  the same cube at rel res 1e-8, FP64 V-cycle vs FP32 V-cycle;
+  batch          256 independent 2D problems (log-uniform 500..50k triangles,
+                 N in {2,3,4}, u = sin(k pi x) sin(m pi y), k,m in 1..4) solved
+                 to 1e-8 one at a time on the plain FP64 path, then all at once
+                 by the batch engine with the mixed V-cycle;
+  serving        a request stream (exponential inter-arrival times, 70% 2D /
+                 30% 3D, 1k..200k elements log-uniform, N in 1..4, rho contrast
+                 in {1,10,1000} around a circle/sphere, pure Neumann with the
+                 mean-zero constraint, rel res 1e-8) through the serving loop,
+                 with throughput and p50/p99 latency per arrival rate.
 """
 import argparse
 import json
@@ -31,6 +43,10 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def manufactured_sin(xyz):
     u = np.sin(np.pi * xyz[0]) * np.sin(np.pi * xyz[1]) * np.sin(np.pi * xyz[2])
     return u, 3 * np.pi ** 2 * u
@@ -49,6 +65,11 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=20, help="cube size of the CPU baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-mixed-leg", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-rates", type=str, default="50,400,2000", help="req/s, comma list")
+    ap.add_argument("--skip-serve", action="store_true")
     return ap.parse_args()
 
 
@@ -73,6 +94,219 @@ def cpu_baseline(n, order, rtol):
                       f"rtol {rtol:g}, {it} its, {dt:.1f} s, converged={rc == 0}"}
 
 
+def cube_solver(ip, n, order, precision, rtol, coarse_iters):
+    mesh = ip.Mesh.cube(n, 0.1, 20261202)
+    s = ip.PoissonSolver(mesh, order)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    s.set_solver(ip.SOLVER_PMULTIGRID)
+    orders = list(range(order, 0, -1))
+    s.set_pmg(orders, pre_it=[1] * len(orders), post_it=[1] * len(orders),
+              cheb_orders=[3] * len(orders), coarse_iters=coarse_iters)
+    s.set_tol_and_iter(rtol, 0.0, 500)
+    s.set_precision(ip.PRECISION_MIXED if precision == "mixed" else ip.PRECISION_FP64)
+    s.init()
+    return mesh, s, orders
+
+
+def timed_solves(s, ip, torch, b_d, x_d, steps, warmup, profile=False):
+    """steps solves from a zero guess, rhs/solution resident in HBM; returns wall, latencies, stats."""
+    def one():
+        x_d.zero_()
+        s.solve(b_d.data_ptr(), x_d.data_ptr(), layout=ip.LAYOUT_DEVICE_SOA)
+        return dict(s.last_stats)
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    if profile:
+        s.take_timing()
+        s.set_profiling(True)
+    lat, stats = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        stats.append(one())
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timing = None
+    if profile:
+        s.set_profiling(False)
+        timing = s.take_timing()
+    return wall, lat, stats, timing
+
+
+def mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, steps):
+    """The cube at rel res 1e-8: FP64 V-cycle, then FP32 V-cycle, same solver and rhs."""
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s.set_precision(ip.PRECISION_MIXED if prec == "mixed" else ip.PRECISION_FP64)
+        s.set_tol_and_iter(1e-8, 0.0, 500)
+        s.init()
+        wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1)
+        out[prec] = {"gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+                     "iterations": [st["iterations"] for st in stats],
+                     "rel_residual": max(st["rel_residual"] for st in stats),
+                     "converged": all(st["converged"] for st in stats)}
+    out["speedup_mixed_over_fp64"] = out["mixed"]["gdofs"] / out["fp64"]["gdofs"]
+    out["rtol"] = 1e-8
+    return out
+
+
+def batch_problems(ip, count, seed=20261204):
+    """2D batch set: log-uniform 500..50k triangles, N in {2,3,4}, sin(k pi x) sin(m pi y)."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        tris = float(np.exp(rng.uniform(np.log(500), np.log(50000))))
+        n = max(2, int(round(np.sqrt(tris / 2))))
+        N = int(rng.integers(2, 5))
+        k, m = (int(v) for v in rng.integers(1, 5, size=2))
+        mesh = ip.Mesh.square(n, jitter=0.1, seed=seed + i)
+        xyz = mesh.node_coords(N)
+        f = ((k * k + m * m) * np.pi ** 2 * np.sin(k * np.pi * xyz[0]) * np.sin(m * np.pi * xyz[1]))
+        probs.append(dict(mesh=mesh, order=N, rhs=f, rhs_nodal=True, rtol=1e-8, max_iter=500))
+    return probs
+
+
+def batch_leg(ip, count):
+    """Same problems solved one at a time on the plain FP64 path, then batched (mixed V-cycle)."""
+    probs = batch_problems(ip, count)
+    dofs = sum(p["mesh"].n_cells * ip.np_of(2, p["order"]) for p in probs)
+    # plain FP64 path, one problem at a time (set-up untimed, solve timed)
+    plain_t, plain_its, plain_ok = 0.0, [], True
+    for i, p in enumerate(probs):
+        if i % 64 == 0:
+            log(f"plain FP64 problem {i}/{len(probs)}")
+        s = ip.PoissonSolver(p["mesh"], p["order"])
+        s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+        s.set_solver(ip.SOLVER_PMULTIGRID)
+        s.set_pmg(list(range(p["order"], 0, -1)), coarse_iters=40)
+        s.set_tol_and_iter(1e-8, 0.0, 500)
+        s.init()
+        b = s.mass(p["rhs"])
+        x = np.zeros_like(b)
+        t0 = time.perf_counter()
+        plain_ok &= s.solve(b, x)
+        plain_t += time.perf_counter() - t0
+        plain_its.append(s.last_stats["iterations"])
+        del s
+    # batched, mixed precision: one warm-up batch of a few problems, then the whole set
+    warm = ip.Batch(2, precision=ip.PRECISION_MIXED)
+    warm.add(probs[:4])
+    warm.run()
+    del warm
+    bt = ip.Batch(2, precision=ip.PRECISION_MIXED)
+    t0 = time.perf_counter()
+    ids = bt.add(probs)
+    t1 = time.perf_counter()
+    bt.run()
+    t2 = time.perf_counter()
+    res = [bt.result(i) for i in ids]
+    its = [r[2]["iterations"] for r in res]
+    ok = all(r[0] == ip.OK for r in res)
+    worst = max(r[2]["rel_residual"] for r in res)
+    st = bt.stats()
+    return {
+        "problems": count, "dof": dofs,
+        "plain_fp64": {"gdofs": dofs / plain_t * 1e-9, "seconds": plain_t,
+                       "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
+                       "converged": bool(plain_ok)},
+        "batched_mixed": {"gdofs": dofs / (t2 - t1) * 1e-9, "seconds": t2 - t1,
+                          "setup_seconds": t1 - t0, "solves_per_s": count / (t2 - t1),
+                          "gdofs_incl_setup": dofs / (t2 - t0) * 1e-9,
+                          "lockstep_iterations": st["iterations_run"],
+                          "mean_iterations": float(np.mean(its)), "max_rel_residual": worst,
+                          "converged": bool(ok)},
+        "speedup_batched_mixed_over_plain_fp64": plain_t / (t2 - t1),
+    }
+
+
+def density(xyz, h, contrast):
+    """rho = contrast inside a circle/sphere of radius 0.25 at the centre, 1 outside, smoothed
+    over a band of 2h (the level-set smoothing of src/kernels/ls_step.h:5-6)."""
+    r = np.sqrt(sum((xyz[d] - 0.5) ** 2 for d in range(xyz.shape[0]))) - 0.25
+    step = np.tanh(np.pi * r / (2.0 * h))
+    return 0.5 * (1 + step) + 0.5 * contrast * (1 - step)
+
+
+def serve_requests(ip, count, seed=20261205):
+    """Request set: 70% 2D / 30% 3D, 1k..200k elements, N in 1..4, rho contrast {1,10,1000},
+    pure Neumann with the mean-zero constraint, Gaussian f projected to zero mean."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        dim = 2 if rng.uniform() < 0.7 else 3
+        cells = float(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        N = int(rng.integers(1, 5))
+        contrast = float(rng.choice([1.0, 10.0, 1000.0]))
+        if dim == 2:
+            n = max(2, int(round(np.sqrt(cells / 2))))
+            mesh = ip.Mesh.square(n, jitter=0.1, seed=seed + i)
+        else:
+            n = max(2, int(round((cells / 6) ** (1 / 3))))
+            mesh = ip.Mesh.cube(n, 0.1, seed + i)
+        xyz = mesh.node_coords(N)
+        k = None if contrast == 1.0 else 1.0 / density(xyz, 1.0 / n, contrast)
+        f = rng.standard_normal(xyz[0].shape)
+        f -= f.mean()
+        reqs.append(dict(mesh=mesh, order=N, rhs=f, rhs_nodal=True, factor=k, nullspace=True,
+                         bc_types=np.ones(mesh.n_bfaces, dtype=np.int32), rtol=1e-8, max_iter=500))
+    return reqs
+
+
+def serve_leg(ip, count, rates, budget_gib=64):
+    import threading
+    reqs = serve_requests(ip, count)
+    dofs = [r["mesh"].n_cells * ip.np_of(r["mesh"].dim, r["order"]) for r in reqs]
+    out = {"requests": count, "dof_total": int(sum(dofs)), "hbm_budget_gib": budget_gib, "rates": []}
+    # warm-up: graphs and code objects of both engines
+    srv = ip.Server(budget_gib << 30)
+    for r in reqs[:4]:
+        srv.submit(**r)
+    srv.wait_all()
+    srv.close()
+    for rate in rates:
+        log(f"serving at {rate:g} req/s")
+        srv = ip.Server(budget_gib << 30)
+        rng = np.random.default_rng(int(rate))
+        gaps = rng.exponential(1.0 / rate, size=count)
+        ids = []
+
+        def client():
+            t_next = time.perf_counter()
+            for r, g in zip(reqs, gaps):
+                t_next += g
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids.append(srv.submit(**r))
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        srv.wait_all()
+        wall = time.perf_counter() - t0
+        lat, its, ok = [], [], True
+        for rid in ids:
+            rc, _, st, l, q = srv.wait(rid)
+            lat.append(l)
+            its.append(st["iterations"])
+            ok &= rc == ip.OK
+        peak = srv.stats()["peak_bytes"]
+        srv.close()
+        out["rates"].append({
+            "offered_req_per_s": rate, "achieved_req_per_s": count / wall,
+            "gdofs": sum(dofs) / wall * 1e-9,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                           "p99": float(np.percentile(lat, 99) * 1e3)},
+            "mean_iterations": float(np.mean(its)), "all_converged": bool(ok),
+            "peak_resident_gib": peak / 2 ** 30})
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -81,17 +315,7 @@ def main():
     import torch
     import ins_poisson as ip
 
-    mesh = ip.Mesh.cube(a.n, 0.1, 20261202)
-    s = ip.PoissonSolver(mesh, a.order)
-    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
-    s.set_solver(ip.SOLVER_PMULTIGRID)
-    orders = list(range(a.order, 0, -1))
-    s.set_pmg(orders, pre_it=[1] * len(orders), post_it=[1] * len(orders),
-              cheb_orders=[3] * len(orders), coarse_iters=a.coarse_iters)
-    s.set_tol_and_iter(a.rtol, 0.0, 500)
-    if a.precision == "mixed":
-        s.set_precision(ip.PRECISION_MIXED)
-    s.init()
+    mesh, s, orders = cube_solver(ip, a.n, a.order, a.precision, a.rtol, a.coarse_iters)
     xyz = s.node_coords()
     u, f = manufactured_sin(xyz)
     dof = s.n_cells * s.Np
@@ -106,31 +330,9 @@ def main():
     x_d = torch.zeros_like(f_d)
     torch.cuda.synchronize()
     s.mass((f_d.data_ptr(), b_d.data_ptr()), layout=ip.LAYOUT_DEVICE_SOA)
-    del f, xyz
+    del f, xyz, f_d
 
-    def one_solve():
-        x_d.zero_()
-        ok = s.solve(b_d.data_ptr(), x_d.data_ptr(), layout=ip.LAYOUT_DEVICE_SOA)
-        return ok, dict(s.last_stats)
-
-    for _ in range(a.warmup):
-        one_solve()
-    torch.cuda.synchronize()
-    s.take_timing()
-    s.set_profiling(True)
-    lat, stats = [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ts = time.perf_counter()
-        ok, st = one_solve()
-        lat.append(time.perf_counter() - ts)
-        stats.append(st)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    s.set_profiling(False)
-    timing = s.take_timing()
-    wall = t1 - t0
+    wall, lat, stats, timing = timed_solves(s, ip, torch, b_d, x_d, a.steps, a.warmup, profile=True)
     gdofs = dof * a.steps / wall * 1e-9
 
     # discretisation error of the last solution (nodal mass norm)
@@ -138,6 +340,7 @@ def main():
     e = x - u
     err = float(np.sqrt(np.sum(e * s.mass(e))))
     unorm = float(np.sqrt(np.sum(u * s.mass(u))))
+    del x, e, u
 
     key = (3, a.order)
     cells = s.n_cells
@@ -193,7 +396,21 @@ def main():
         "smoother_applies_per_solve": stats[-1]["smoother_applies"],
         "roofline": roofline,
     }
+    log(f"plain leg: {gdofs:.4f} GDoF/s, its {out['iterations']}")
+    if not a.no_mixed_leg:
+        log("mixed vs fp64 leg")
+        out["mixed_vs_fp64"] = mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, 2)
+    del s, b_d, x_d, mesh
+    torch.cuda.empty_cache()
+    if not a.skip_batch:
+        log("batch leg")
+        out["batch"] = batch_leg(ip, a.batch_problems)
+    if not a.skip_serve:
+        rates = [float(r) for r in a.serve_rates.split(",") if r]
+        log("serving leg")
+        out["serving"] = serve_leg(ip, a.serve_requests, rates)
     if not a.skip_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.order, a.rtol)
     print(json.dumps(out), flush=True)
 

@@ -95,6 +95,9 @@ int ins_mesh_get(ins_mesh_t m, double* coords, int* cell2nodes);
  * bface2cells (n_bfaces), bfaceNum (n_bfaces). */
 int ins_mesh_get_faces(ins_mesh_t m, int* face2cells, int* faceNum, int* bface2cells,
                        int* bfaceNum);
+/* Nodal coordinates of the order-`order` element nodes, dim x n_cells x Np (the toolkit
+ * dats x, y, z that callers evaluate right-hand sides on); host only. */
+int ins_mesh_node_coords(ins_mesh_t m, int order, double* xyz);
 void ins_mesh_destroy(ins_mesh_t m);
 
 /* ---- solver handle: PoissonMatrixFreeDiag{2D,3D} + LinearSolver ---- */
@@ -165,6 +168,81 @@ int ins_ref_element(int dim, int order, double* rst, double* D, double* M, int* 
 /* Nodal interpolation from order nc to order nf (Np(nf) x Np(nc)), the p-multigrid
  * prolongation; host only. */
 int ins_ref_interp(int dim, int nf, int nc, double* P);
+
+/* ---- batched solves: many independent problems advanced in lockstep ----
+ *
+ * The reference solves one problem per call; these entry points put many
+ * (uneven sizes, orders 1..4, one dimension per batch) on the GPU at once. Each
+ * problem runs the same flexible CG as ins_poisson_solve (viscous_solver.cpp:158-266,
+ * in FP64) preconditioned by a two-level p-multigrid V-cycle (order N -> 1, the
+ * reference's "orders = N,1", examples/2d/vortex/config.ini) with Chebyshev
+ * smoothing and Jacobi-CG on order 1. A problem's arithmetic does not depend on
+ * which other problems share the batch. Vectors are host AoS (cells x Np). */
+typedef struct ins_batch* ins_batch_t;
+typedef struct ins_server* ins_server_t;
+
+typedef struct {
+  int precision;          /* INS_PRECISION_FP64, or INS_PRECISION_MIXED: FP32 V-cycle */
+  int cheb_order;         /* Chebyshev degree of one smoothing application (1..8) */
+  int pre_it, post_it;    /* smoothing applications before / after the coarse solve */
+  double eig_safety;      /* lambda_max safety factor (eigen_val_saftey_factor) */
+  double cheb_min_factor; /* lower Chebyshev bound / upper (cheb_min_lamda_factor) */
+  int max_iter_eig;       /* power iterations for lambda_max (max_iter_eig_approx) */
+  int coarse_iters;       /* Jacobi-CG iterations on order 1 */
+  int use_graph;          /* replay one captured HIP graph per lockstep iteration */
+} ins_batch_config;
+
+typedef struct {
+  ins_mesh_t mesh;        /* read during add/submit only; may be destroyed afterwards */
+  int order;              /* 1..4 */
+  const int* bc_types;    /* n_bfaces INS_BC_* values, or NULL = all Dirichlet */
+  const double* factor;   /* cells x Np coefficient (set_factor), or NULL = 1 */
+  int nullspace;          /* set_nullspace */
+  double rtol, atol;      /* set_tol_and_iter */
+  int max_iter;
+  const double* rhs;      /* cells x Np */
+  int rhs_nodal;          /* 0: rhs is mass-weighted (M f); 1: rhs is nodal f, b = J M f */
+} ins_problem;
+
+/* Defaults: MIXED, Chebyshev 3, 1 pre / 1 post, 1.1, 0.1, 15, 40, graphs on. */
+void ins_batch_config_default(ins_batch_config* cfg);
+/* Device bytes one problem holds while resident (admission control). */
+int ins_batch_problem_bytes(int dim, int64_t n_cells, int order, int has_factor, int precision,
+                            int64_t* bytes);
+int ins_batch_create(int dim, const ins_batch_config* cfg, ins_batch_t* out);
+void ins_batch_destroy(ins_batch_t b);
+/* Uploads n problems, sets them up (penalties, diagonals, Chebyshev bounds, in
+ * one batched power iteration) and starts them; ids[i] names problem i. */
+int ins_batch_add(ins_batch_t b, const ins_problem* probs, int n, int* ids);
+/* Runs up to max_iters lockstep iterations, stopping early when none is running;
+ * *running = problems still running afterwards. */
+int ins_batch_iterate(ins_batch_t b, int max_iters, int* running);
+/* Ids finished since the last call, at most cap per call; *n = count returned. */
+int ins_batch_take_finished(ins_batch_t b, int* ids, int cap, int* n);
+/* Result of a finished problem: x (cells x Np, may be NULL) and stats (may be NULL).
+ * Returns INS_OK, INS_NOT_CONVERGED or INS_ERR_BREAKDOWN as ins_poisson_solve would. */
+int ins_batch_result(ins_batch_t b, int id, double* x, ins_solve_stats* stats);
+/* Frees a finished problem's device memory; its id may be reused by a later add. */
+int ins_batch_release(ins_batch_t b, int id);
+int ins_batch_stats(ins_batch_t b, int64_t* bytes_in_use, int64_t* iterations_run, int* running);
+
+/* ---- serving: continuous batching under an HBM budget ----
+ * Requests are queued by ins_server_submit from any thread. A worker thread
+ * admits them in arrival order while their device memory fits the budget,
+ * advances all admitted solves together (one batch per dimension) and retires
+ * each as it finishes, so later arrivals join the running batch. A request
+ * larger than the whole budget is rejected (INS_ERR_NOMEM from wait). */
+int ins_server_create(int64_t hbm_budget_bytes, const ins_batch_config* cfg2d,
+                      const ins_batch_config* cfg3d, ins_server_t* out);
+void ins_server_destroy(ins_server_t s);
+int ins_server_submit(ins_server_t s, const ins_problem* p, int64_t* id);
+int ins_server_poll(ins_server_t s, int64_t id, int* done);
+/* Blocks until request id is finished and collects it (once): x (cells x Np, may be
+ * NULL), stats, arrival -> result latency and arrival -> admission queueing time. */
+int ins_server_wait(ins_server_t s, int64_t id, double* x, ins_solve_stats* stats,
+                    double* latency_s, double* queue_s);
+int ins_server_wait_all(ins_server_t s);
+int ins_server_stats(ins_server_t s, int64_t* completed, int64_t* peak_bytes);
 
 const char* ins_poisson_last_error(void);
 

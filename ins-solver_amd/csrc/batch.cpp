@@ -1,0 +1,618 @@
+#include "batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "dg_layout.h"
+#include "kernels.h"
+#include "refelem.h"
+#include "solver.h"
+#include "vec_kernels.h"
+
+namespace insp {
+
+namespace {
+int64_t round_up(int64_t v, int64_t m) { return ((v + m - 1) / m) * m; }
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+int np_of(int dim, int N) { return dim == 2 ? (N + 1) * (N + 2) / 2 : (N + 1) * (N + 2) * (N + 3) / 6; }
+int nfp_of(int dim, int N) { return dim == 2 ? N + 1 : (N + 1) * (N + 2) / 2; }
+}  // namespace
+
+struct BatchEngine::Prob {
+  int slot = -1, N = 0, Np = 0, Np1 = 0, Nfp = 0, dim = 0;
+  int64_t E = 0, S = 0, nbf = 0;
+  bool var = false, nullspace = false;
+  double rtol = 1e-8, atol = 0, tol = 0, bnorm = 0;
+  int max_iter = 500, iters = 0;
+  void* arena = nullptr;
+  size_t bytes = 0;
+  BatchProb desc{};
+  ProbResult result;
+  bool done = false;
+  std::chrono::steady_clock::time_point t0;
+};
+
+struct BatchEngine::View {
+  DBuf<int2> tiles;
+  DBuf<int> probs;
+  DBuf<int2> ranges;
+  DBuf<int> counts;
+  DBuf<double> partial;
+  int tile_cap = 0, prob_cap = 0;
+  int2* h_tiles = nullptr;
+  int* h_probs = nullptr;
+  int2* h_ranges = nullptr;
+  int* h_counts = nullptr;
+  int max_slots = 0;
+  BatchView bv{};
+  ~View() {
+    if (h_tiles) (void)hipHostFree(h_tiles);
+    if (h_probs) (void)hipHostFree(h_probs);
+    if (h_ranges) (void)hipHostFree(h_ranges);
+    if (h_counts) (void)hipHostFree(h_counts);
+  }
+};
+
+size_t BatchEngine::arena_bytes(int dim, int64_t cells, int N, bool var, bool fp32) {
+  const int64_t S = round_up(std::max<int64_t>(cells, 1), kTile);
+  const int Np = np_of(dim, N), Np1 = dim + 1, Nf = dim + 1, Nfp = nfp_of(dim, N);
+  const int NK = dim == 2 ? 3 : 6;
+  const size_t d = 8, t = fp32 ? 4 : 8;
+  size_t b = 0;
+  b += align256(NK * S * d) + align256(S * d);                    // Kt, J
+  b += 2 * align256(Nf * S * 4);                                  // nbr, code
+  b += align256((size_t)std::max<int64_t>(cells, 1) * Nf * 4);    // bctype (upper bound)
+  b += 2 * align256(Nf * S * d);                                  // tauF, tauC
+  if (var) b += align256(Np * S * d) + align256(Np * S * 4) + align256(Np1 * S * d) + align256(Np1 * S * 4);
+  b += align256(Np * S * d) + align256(Np * S * 4) + align256(Np1 * S * d) + align256(Np1 * S * 4);  // inverses
+  b += 6 * align256(Np * S * d) + align256((size_t)Nf * Nfp * S * d);  // outer FCG + traces
+  b += 4 * align256(Np * S * t) + (fp32 ? align256((size_t)Nf * Nfp * S * t) : 0);
+  b += 6 * align256(Np1 * S * t);
+  return b;
+}
+
+BatchEngine::BatchEngine(const BatchConfig& cfg, hipStream_t stream) : cfg_(cfg), stream_(stream) {
+  if (cfg_.dim != 2 && cfg_.dim != 3) throw InvalidArgument("batch dimension must be 2 or 3");
+  if (cfg_.cheb_order < 1 || cfg_.cheb_order > 8 || cfg_.pre_it < 1 || cfg_.post_it < 0)
+    throw InvalidArgument("invalid batch smoother settings");
+  if (!stream_) {
+    INSP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  max_slots_ = 4096;
+  INSP_HIP(hipMalloc(&d_desc_, sizeof(BatchProb) * max_slots_));
+  INSP_HIP(hipMalloc(&d_scal_, sizeof(double) * BS_COUNT * max_slots_));
+  INSP_HIP(hipMemset(d_scal_, 0, sizeof(double) * BS_COUNT * max_slots_));
+  INSP_HIP(hipDeviceSynchronize());
+  INSP_HIP(hipHostMalloc((void**)&h_scal_, sizeof(double) * BS_COUNT * max_slots_, hipHostMallocDefault));
+  probs_.resize(max_slots_);
+  for (int i = max_slots_ - 1; i >= 0; --i) free_slots_.push_back(i);
+  run_ = std::make_unique<View>();
+  work_ = std::make_unique<View>();
+  build_tables();
+}
+
+BatchEngine::~BatchEngine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (auto& p : probs_)
+    if (p && p->arena) (void)hipFree(p->arena);
+  for (void* b : table_bufs_) (void)hipFree(b);
+  if (d_desc_) (void)hipFree(d_desc_);
+  if (d_scal_) (void)hipFree(d_scal_);
+  if (h_scal_) (void)hipHostFree(h_scal_);
+  run_.reset();
+  work_.reset();
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void BatchEngine::build_tables() {
+  const int d = cfg_.dim;
+  auto up = [&](const void* h, size_t bytes) {
+    void* p = nullptr;
+    INSP_HIP(hipMalloc(&p, bytes));
+    INSP_HIP(hipMemcpy(p, h, bytes, hipMemcpyHostToDevice));
+    table_bufs_.push_back(p);
+    return p;
+  };
+  const RefElement r1 = make_ref_element(d, 1);
+  for (int N = 1; N <= kMaxOrder; ++N) {
+    const RefElement re = make_ref_element(d, N);
+    std::vector<double> C;
+    std::vector<int> tab;
+    make_op_tables(re, C, tab);
+    std::vector<float> Cf(C.begin(), C.end());
+    tables_.C64[N] = (const double*)up(C.data(), C.size() * 8);
+    tables_.C32[N] = (const float*)up(Cf.data(), Cf.size() * 4);
+    tables_.tab[N] = (const int*)up(tab.data(), tab.size() * 4);
+    if (N > 1) {
+      const std::vector<double> P = interp_matrix(re, r1);
+      const std::vector<double> Pt = mat_transpose(P, re.Np, r1.Np);
+      std::vector<float> Pf(P.begin(), P.end()), Ptf(Pt.begin(), Pt.end());
+      tables_.P64[N] = (const double*)up(P.data(), P.size() * 8);
+      tables_.Pt64[N] = (const double*)up(Pt.data(), Pt.size() * 8);
+      tables_.P32[N] = (const float*)up(Pf.data(), Pf.size() * 4);
+      tables_.Pt32[N] = (const float*)up(Ptf.data(), Ptf.size() * 4);
+    }
+  }
+}
+
+void BatchEngine::upload_desc(int slot) {
+  INSP_HIP(hipMemcpyAsync(d_desc_ + slot, &probs_[slot]->desc, sizeof(BatchProb),
+                          hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
+  int ntiles = 0;
+  for (int s : slots) ntiles += (int)(probs_[s]->S / kTile);
+  const int nprobs = (int)slots.size();
+  auto grow = [](int need, int cap) {
+    int c = std::max(cap, 64);
+    while (c < need) c *= 2;
+    return c;
+  };
+  const int tcap = grow(ntiles, v.tile_cap), pcap = grow(nprobs, v.prob_cap);
+  if (tcap != v.tile_cap || !v.tiles.p) {
+    v.tiles.alloc(tcap);
+    v.partial.alloc((size_t)4 * tcap);
+    if (v.h_tiles) (void)hipHostFree(v.h_tiles);
+    INSP_HIP(hipHostMalloc((void**)&v.h_tiles, sizeof(int2) * tcap, hipHostMallocDefault));
+    v.tile_cap = tcap;
+  }
+  if (pcap != v.prob_cap || !v.probs.p) {
+    v.probs.alloc(pcap);
+    if (v.h_probs) (void)hipHostFree(v.h_probs);
+    INSP_HIP(hipHostMalloc((void**)&v.h_probs, sizeof(int) * pcap, hipHostMallocDefault));
+    v.prob_cap = pcap;
+  }
+  if (!v.ranges.p) {
+    v.ranges.alloc(max_slots_);
+    v.counts.alloc(2);
+    INSP_HIP(hipHostMalloc((void**)&v.h_ranges, sizeof(int2) * max_slots_, hipHostMallocDefault));
+    INSP_HIP(hipHostMalloc((void**)&v.h_counts, sizeof(int) * 2, hipHostMallocDefault));
+    std::memset(v.h_ranges, 0, sizeof(int2) * max_slots_);
+  }
+  int t = 0;
+  for (int j = 0; j < nprobs; ++j) {
+    const Prob& P = *probs_[slots[j]];
+    v.h_probs[j] = slots[j];
+    v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / kTile));
+    for (int64_t c = 0; c < P.S; c += kTile) v.h_tiles[t++] = make_int2(slots[j], (int)c);
+  }
+  v.h_counts[0] = ntiles;
+  v.h_counts[1] = nprobs;
+  INSP_HIP(hipMemcpyAsync(v.tiles.p, v.h_tiles, sizeof(int2) * std::max(ntiles, 1),
+                          hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipMemcpyAsync(v.probs.p, v.h_probs, sizeof(int) * std::max(nprobs, 1),
+                          hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipMemcpyAsync(v.ranges.p, v.h_ranges, sizeof(int2) * max_slots_, hipMemcpyHostToDevice,
+                          stream_));
+  INSP_HIP(hipMemcpyAsync(v.counts.p, v.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  v.bv.P = d_desc_;
+  v.bv.tiles = v.tiles.p;
+  v.bv.ntiles = v.counts.p;
+  v.bv.probs = v.probs.p;
+  v.bv.ranges = v.ranges.p;
+  v.bv.nprobs = v.counts.p + 1;
+  v.bv.tile_cap = v.tile_cap;
+  v.bv.prob_cap = v.prob_cap;
+  v.bv.partial = v.partial.p;
+  v.bv.dim = cfg_.dim;
+}
+
+void BatchEngine::read_scal(const std::vector<int>& slots) {
+  int hi = 0;
+  for (int s : slots) hi = std::max(hi, s + 1);
+  if (!hi) return;
+  INSP_HIP(hipMemcpyAsync(h_scal_, d_scal_, sizeof(double) * BS_COUNT * hi, hipMemcpyDeviceToHost,
+                          stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+template <typename T>
+void BatchEngine::vcycle(View& v) {
+  const BatchView& b = v.bv;
+  hipStream_t s = stream_;
+  const bool f32 = std::is_same<T, float>::value;
+  batch::to_t(b, f32, BV_R, BV_VB, s);
+  const int K = cfg_.cheb_order;
+  auto smooth = [&](bool first_zero) {
+    for (int k = 0; k < K; ++k) {
+      if (first_zero && k == 0) {
+        batch::cheb<T>(b, 0, true, s);
+      } else {
+        batch::trace<T>(b, tables_, BV_VX, BV_TRT, s);
+        batch::apply<T>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, s);
+        batch::cheb<T>(b, k, false, s);
+      }
+    }
+  };
+  for (int it = 0; it < cfg_.pre_it; ++it) smooth(it == 0);
+  batch::trace<T>(b, tables_, BV_VX, BV_TRT, s);
+  batch::apply<T>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, s);
+  batch::sub_from_b<T>(b, s);
+  batch::restrict_<T>(b, tables_, s);
+  batch::cg_init<T>(b, s);
+  batch::finalize(b, 1, BS_CA, s);
+  int a = BS_CA, c = BS_CB;
+  for (int it = 0; it < cfg_.coarse_iters; ++it) {
+    batch::apply<T>(b, tables_, 1, BV_CP, BV_TRT, BV_CAP, true, s);
+    batch::finalize(b, 1, BS_CPAP, s);
+    batch::cg_xr<T>(b, a, s);
+    batch::finalize(b, 1, c, s);
+    batch::cg_p<T>(b, c, a, s);
+    std::swap(a, c);
+  }
+  batch::prolong<T>(b, tables_, s);
+  for (int it = 0; it < cfg_.post_it; ++it) smooth(false);
+  batch::from_t(b, f32, BV_VX, BV_Z, s);
+  batch::sum_d(b, BV_Z, s);
+  batch::finalize(b, 1, BS_MEAN, s);
+  batch::sub_mean_d(b, BV_Z, s);
+}
+
+void BatchEngine::precondition(View& v) {
+  if (cfg_.fp32) vcycle<float>(v);
+  else vcycle<double>(v);
+}
+
+void BatchEngine::run_iteration(View& v) {
+  const BatchView& b = v.bv;
+  hipStream_t s = stream_;
+  batch::trace<double>(b, tables_, BV_P, BV_TRD, s);
+  batch::apply<double>(b, tables_, 0, BV_P, BV_TRD, BV_AP, true, s);
+  batch::finalize(b, 1, BS_PAP, s);
+  batch::fcg_update_xr(b, s);
+  batch::finalize(b, 2, BS_RR, s);  // |r|^2 and r.z_old
+  precondition(v);
+  batch::dot<double>(b, 0, BV_R, BV_Z, BV_R, BV_Z, 1, s);
+  batch::finalize(b, 1, BS_RZN, s);
+  batch::fcg_update_p(b, s);
+  batch::scal_copy(b, BS_RZ, BS_RZN, s);
+}
+
+std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
+  const int d = cfg_.dim, Nf = d + 1, NK = d == 2 ? 3 : 6;
+  std::vector<int> ids;
+  for (auto& sp : specs) {
+    if (!sp.mesh || sp.mesh->dim != d) throw InvalidArgument("problem dimension does not match the batch");
+    if (sp.order < 1 || sp.order > kMaxOrder) throw InvalidArgument("order must be in 1..4");
+    if (free_slots_.empty()) throw InvalidArgument("batch is full");
+    HostMesh& m = *sp.mesh;
+    if (m.nbr.empty()) build_topology(m);
+    if (m.J.empty()) build_geometry(m);
+    auto P = std::make_unique<Prob>();
+    P->slot = free_slots_.back();
+    P->N = sp.order;
+    P->dim = d;
+    P->Np = np_of(d, sp.order);
+    P->Np1 = d + 1;
+    P->Nfp = nfp_of(d, sp.order);
+    P->E = m.n_cells;
+    P->S = round_up(m.n_cells, kTile);
+    P->nbf = m.n_bfaces;
+    P->var = !sp.factor.empty();
+    P->nullspace = sp.nullspace;
+    P->rtol = sp.rtol;
+    P->atol = sp.atol;
+    P->max_iter = sp.max_iter;
+    P->t0 = std::chrono::steady_clock::now();
+    P->result.fine_applies = std::max(1, cfg_.max_iter_eig);
+    if ((int64_t)sp.rhs.size() != P->E * P->Np) throw InvalidArgument("rhs size mismatch");
+    if (P->var && (int64_t)sp.factor.size() != P->E * P->Np) throw InvalidArgument("factor size mismatch");
+    if (!sp.bc_types.empty() && (int64_t)sp.bc_types.size() != P->nbf)
+      throw InvalidArgument("bc_types size mismatch");
+    P->bytes = arena_bytes(d, P->E, P->N, P->var, cfg_.fp32);
+    INSP_HIP(hipMalloc(&P->arena, P->bytes));
+    INSP_HIP(hipMemset(P->arena, 0, P->bytes));
+    INSP_HIP(hipDeviceSynchronize());
+    char* base = (char*)P->arena;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      void* p = base + off;
+      off += align256(bytes);
+      return p;
+    };
+    const int64_t S = P->S, E = P->E;
+    const int Np = P->Np, Np1 = P->Np1, Nfp = P->Nfp;
+    const size_t t = cfg_.fp32 ? 4 : 8;
+    BatchProb& D = P->desc;
+    D.N = P->N; D.Np = Np; D.Np1 = Np1; D.var = P->var; D.nullspace = P->nullspace;
+    D.E = E; D.S = S; D.dofs = (double)E * Np;
+    double* Kt = (double*)take(NK * S * 8);
+    double* J = (double*)take(S * 8);
+    int* nbr = (int*)take(Nf * S * 4);
+    int* code = (int*)take(Nf * S * 4);
+    int* bct = (int*)take((size_t)std::max<int64_t>(E, 1) * Nf * 4);
+    double* tauF = (double*)take(Nf * S * 8);
+    double* tauC = (double*)take(Nf * S * 8);
+    D.Kt = Kt; D.J = J; D.nbr = nbr; D.code = code; D.bctype = bct; D.tauF = tauF; D.tauC = tauC;
+    double *kF64 = nullptr, *kC64 = nullptr;
+    float *kF32 = nullptr, *kC32 = nullptr;
+    if (P->var) {
+      kF64 = (double*)take(Np * S * 8); kF32 = (float*)take(Np * S * 4);
+      kC64 = (double*)take(Np1 * S * 8); kC32 = (float*)take(Np1 * S * 4);
+    }
+    D.kF[0] = kF64; D.kF[1] = kF32; D.kC[0] = kC64; D.kC[1] = kC32;
+    double* iF64 = (double*)take(Np * S * 8);
+    float* iF32 = (float*)take(Np * S * 4);
+    double* iC64 = (double*)take(Np1 * S * 8);
+    float* iC32 = (float*)take(Np1 * S * 4);
+    D.invF[0] = iF64; D.invF[1] = iF32; D.invC[0] = iC64; D.invC[1] = iC32;
+    for (int f = BV_B; f <= BV_AP; ++f) D.v[f] = take(Np * S * 8);
+    D.v[BV_TRD] = take((size_t)Nf * Nfp * S * 8);
+    for (int f = BV_VX; f <= BV_VD; ++f) D.v[f] = take(Np * S * t);
+    D.v[BV_TRT] = cfg_.fp32 ? take((size_t)Nf * Nfp * S * t) : D.v[BV_TRD];
+    for (int f = BV_CX; f <= BV_CAP; ++f) D.v[f] = take(Np1 * S * t);
+    D.scal = d_scal_ + (size_t)P->slot * BS_COUNT;
+    if (off > P->bytes) throw std::runtime_error("arena layout overflow");
+    // geometry and connectivity, SoA with stride S
+    const int nperm = d == 2 ? 2 : 6;
+    std::vector<double> kt((size_t)NK * S, 0.0), jj((size_t)S, 1.0);
+    std::vector<int> nb((size_t)Nf * S, 0), cd((size_t)Nf * S, 0);
+    for (int64_t e = 0; e < E; ++e) {
+      for (int k = 0; k < NK; ++k) kt[(size_t)k * S + e] = m.Kt[(size_t)e * NK + k];
+      jj[e] = m.J[e];
+      for (int f = 0; f < Nf; ++f) {
+        const int n = m.nbr[(size_t)e * Nf + f];
+        nb[(size_t)f * S + e] = n;
+        cd[(size_t)f * S + e] = n >= 0 ? m.nbr_face[(size_t)e * Nf + f] * nperm + m.nbr_perm[(size_t)e * Nf + f] : 0;
+      }
+    }
+    INSP_HIP(hipMemcpy(Kt, kt.data(), kt.size() * 8, hipMemcpyHostToDevice));
+    INSP_HIP(hipMemcpy(J, jj.data(), jj.size() * 8, hipMemcpyHostToDevice));
+    INSP_HIP(hipMemcpy(nbr, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
+    INSP_HIP(hipMemcpy(code, cd.data(), cd.size() * 4, hipMemcpyHostToDevice));
+    std::vector<int> bt = sp.bc_types;
+    if (bt.empty()) bt.assign((size_t)std::max<int64_t>(P->nbf, 1), 0);
+    INSP_HIP(hipMemcpy(bct, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+    const RefElement reN = make_ref_element(d, P->N), re1 = make_ref_element(d, 1);
+    if (P->var) {
+      std::vector<double> kf((size_t)Np * S, 0.0), kc((size_t)Np1 * S, 0.0);
+      const std::vector<double> Ik = interp_matrix(re1, reN);  // Np1 x Np
+      for (int64_t e = 0; e < E; ++e) {
+        for (int i = 0; i < Np; ++i) kf[(size_t)i * S + e] = sp.factor[(size_t)e * Np + i];
+        for (int i = 0; i < Np1; ++i) {
+          double a = 0;
+          for (int j = 0; j < Np; ++j) a += Ik[(size_t)i * Np + j] * sp.factor[(size_t)e * Np + j];
+          kc[(size_t)i * S + e] = a;
+        }
+      }
+      std::vector<float> kff(kf.begin(), kf.end()), kcf(kc.begin(), kc.end());
+      INSP_HIP(hipMemcpy(kF64, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
+      INSP_HIP(hipMemcpy(kF32, kff.data(), kff.size() * 4, hipMemcpyHostToDevice));
+      INSP_HIP(hipMemcpy(kC64, kc.data(), kc.size() * 8, hipMemcpyHostToDevice));
+      INSP_HIP(hipMemcpy(kC32, kcf.data(), kcf.size() * 4, hipMemcpyHostToDevice));
+    }
+    // penalties (finest order in the penalty on both levels) and inverse diagonals
+    MeshView mv{d, E, S, Kt, J, nbr, code, bct};
+    LevelView lvN{P->N, nullptr, tables_.C64[P->N], tables_.tab[P->N], tauF};
+    LevelView lv1{1, nullptr, tables_.C64[1], tables_.tab[1], tauC};
+    launch_tau(mv, lvN, kF64, P->N, tauF, stream_);
+    launch_tau(mv, lv1, kC64, P->N, tauC, stream_);
+    double* tmp = (double*)D.v[BV_R];
+    launch_diag(mv, lvN, kF64, tmp, stream_);
+    reciprocal<double>(Np, E, S, tmp, iF64, stream_);
+    reciprocal<float>(Np, E, S, tmp, iF32, stream_);
+    launch_diag(mv, lv1, kC64, tmp, stream_);
+    reciprocal<double>(Np1, E, S, tmp, iC64, stream_);
+    reciprocal<float>(Np1, E, S, tmp, iC32, stream_);
+    fill<double>(Np * S, tmp, 0.0, stream_);
+    // rhs in SoA
+    std::vector<double> rb((size_t)Np * S, 0.0);
+    for (int64_t e = 0; e < E; ++e)
+      for (int i = 0; i < Np; ++i) rb[(size_t)i * S + e] = sp.rhs[(size_t)e * Np + i];
+    INSP_HIP(hipStreamSynchronize(stream_));
+    INSP_HIP(hipMemcpy(D.v[BV_B], rb.data(), rb.size() * 8, hipMemcpyHostToDevice));
+    if (sp.rhs_nodal) {  // b = J M f
+      launch_mass(mv, lvN, (const double*)D.v[BV_B], (double*)D.v[BV_AP], stream_);
+      copy<double>((int64_t)Np * S, (const double*)D.v[BV_AP], (double*)D.v[BV_B], stream_);
+      fill<double>((int64_t)Np * S, (double*)D.v[BV_AP], 0.0, stream_);
+      INSP_HIP(hipStreamSynchronize(stream_));
+    }
+    bytes_in_use_ += P->bytes;
+    const int slot = P->slot;
+    free_slots_.pop_back();
+    probs_[slot] = std::move(P);
+    upload_desc(slot);
+    ids.push_back(slot);
+  }
+  if (ids.empty()) return ids;
+  // Chebyshev bounds: power iteration on D^-1 A for all new problems at once.
+  set_view(*work_, ids);
+  const BatchView& b = work_->bv;
+  hipStream_t s = stream_;
+  batch::rand_d(b, BV_R, s);
+  for (int it = 0; it < std::max(1, cfg_.max_iter_eig); ++it) {
+    batch::trace<double>(b, tables_, BV_R, BV_TRD, s);
+    batch::apply<double>(b, tables_, 0, BV_R, BV_TRD, BV_Z, false, s);
+    batch::inv_apply_d(b, BV_Z, s);
+    batch::dot<double>(b, 0, BV_R, BV_Z, BV_R, BV_R, 2, s);
+    batch::finalize(b, 2, BS_TOP, s);
+    batch::dot<double>(b, 0, BV_Z, BV_Z, BV_Z, BV_Z, 1, s);
+    batch::finalize(b, 1, BS_NRM, s);
+    batch::from_t(b, false, BV_Z, BV_R, s);
+    batch::scale_d(b, BV_R, BS_NRM, s);
+  }
+  read_scal(ids);
+  for (int slot : ids) {
+    Prob& P = *probs_[slot];
+    const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
+    const double lam = sc[BS_BOT] > 0 ? sc[BS_TOP] / sc[BS_BOT] : 1.0;
+    const double hi = cfg_.eig_safety * lam, lo = cfg_.cheb_min_factor * hi;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho_old = 1.0 / sigma;
+    P.desc.c1[0] = 0.0;
+    P.desc.c2[0] = 1.0 / theta;
+    for (int k = 1; k < 8; ++k) {
+      const double rho = 1.0 / (2.0 * sigma - rho_old);
+      P.desc.c1[k] = rho * rho_old;
+      P.desc.c2[k] = 2.0 * rho / delta;
+      rho_old = rho;
+    }
+    INSP_HIP(hipMemcpyAsync(d_desc_ + slot, &P.desc, sizeof(BatchProb), hipMemcpyHostToDevice, stream_));
+  }
+  // clear the power-iteration scratch (x starts at zero)
+  for (int slot : ids) {
+    Prob& P = *probs_[slot];
+    fill<double>((int64_t)P.Np * P.S, (double*)P.desc.v[BV_R], 0.0, stream_);
+    fill<double>((int64_t)P.Np * P.S, (double*)P.desc.v[BV_Z], 0.0, stream_);
+  }
+  INSP_HIP(hipStreamSynchronize(stream_));
+  start(*work_, ids, false);
+  return ids;
+}
+
+// r = b - A x, convergence test, then z = M r, p = z for problems not yet converged.
+void BatchEngine::start(View& v, const std::vector<int>& slots, bool restart) {
+  if (slots.empty()) return;
+  set_view(v, slots);
+  const BatchView& b = v.bv;
+  hipStream_t s = stream_;
+  if (!restart) {
+    batch::sum_d(b, BV_B, s);
+    batch::finalize(b, 1, BS_MEAN, s);
+    batch::sub_mean_d(b, BV_B, s);
+    batch::dot<double>(b, 0, BV_B, BV_B, BV_B, BV_B, 1, s);
+    batch::finalize(b, 1, BS_BB, s);
+  }
+  batch::trace<double>(b, tables_, BV_X, BV_TRD, s);
+  batch::apply<double>(b, tables_, 0, BV_X, BV_TRD, BV_AP, false, s);
+  batch::residual(b, s);
+  batch::finalize(b, 1, BS_RR, s);
+  read_scal(slots);
+  std::vector<int> go;
+  for (int slot : slots) {
+    Prob& P = *probs_[slot];
+    const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
+    ++P.result.fine_applies;
+    if (!restart) {
+      P.bnorm = std::sqrt(sc[BS_BB]);
+      P.tol = std::max(P.rtol * P.bnorm, P.atol);
+    }
+    const double rr = sc[BS_RR];
+    if (!std::isfinite(rr) || !std::isfinite(P.bnorm)) {
+      finish(slot, ProbStatus::BREAKDOWN);
+    } else if (std::sqrt(rr) <= P.tol) {
+      P.result.rel_residual = P.bnorm > 0 ? std::sqrt(rr) / P.bnorm : 0.0;
+      finish(slot, ProbStatus::CONVERGED);
+    } else if (P.iters >= P.max_iter) {
+      P.result.rel_residual = std::sqrt(rr) / P.bnorm;
+      finish(slot, ProbStatus::NOT_CONVERGED);
+    } else {
+      go.push_back(slot);
+    }
+  }
+  if (go.empty()) return;
+  set_view(v, go);
+  precondition(v);
+  batch::dot<double>(v.bv, 0, BV_R, BV_Z, BV_R, BV_Z, 1, s);
+  batch::finalize(v.bv, 1, BS_RZ, s);
+  batch::from_t(v.bv, false, BV_Z, BV_P, s);
+  INSP_HIP(hipStreamSynchronize(stream_));
+  for (int slot : go) {
+    running_slots_.push_back(slot);
+    ++probs_[slot]->result.vcycles;
+  }
+  std::sort(running_slots_.begin(), running_slots_.end());
+  graph_tile_cap_ = -2;  // running set changed: refresh the view before the next replay
+}
+
+void BatchEngine::finish(int slot, ProbStatus st) {
+  Prob& P = *probs_[slot];
+  P.result.status = st;
+  P.result.iterations = P.iters;
+  P.result.bnorm = P.bnorm;
+  P.result.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - P.t0).count();
+  P.result.x.assign((size_t)P.E * P.Np, 0.0);
+  DBuf<double> tmp;
+  tmp.alloc((size_t)P.E * P.Np, false);
+  aos_from_soa(P.Np, P.E, P.S, (const double*)P.desc.v[BV_X], tmp.p, stream_);
+  INSP_HIP(hipMemcpyAsync(P.result.x.data(), tmp.p, sizeof(double) * P.E * P.Np,
+                          hipMemcpyDeviceToHost, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  P.done = true;
+  finished_.push_back(slot);
+}
+
+void BatchEngine::iterate() {
+  if (running_slots_.empty()) return;
+  const bool refresh = graph_tile_cap_ == -2 || !run_->tiles.p;
+  if (refresh) {
+    const int old_t = run_->tile_cap, old_p = run_->prob_cap;
+    set_view(*run_, running_slots_);
+    if (run_->tile_cap != old_t || run_->prob_cap != old_p || !graph_) {
+      if (graph_) {
+        INSP_HIP(hipGraphExecDestroy(graph_));
+        graph_ = nullptr;
+      }
+    }
+    graph_tile_cap_ = run_->tile_cap;
+    graph_prob_cap_ = run_->prob_cap;
+  }
+  if (cfg_.use_graph) {
+    if (!graph_) {
+      hipGraph_t g;
+      INSP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      run_iteration(*run_);
+      INSP_HIP(hipStreamEndCapture(stream_, &g));
+      INSP_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+      INSP_HIP(hipGraphDestroy(g));
+    }
+    INSP_HIP(hipGraphLaunch(graph_, stream_));
+  } else {
+    run_iteration(*run_);
+  }
+  ++iters_run_;
+  read_scal(running_slots_);
+  std::vector<int> keep, verify;
+  for (int slot : running_slots_) {
+    Prob& P = *probs_[slot];
+    ++P.iters;
+    ++P.result.fine_applies;
+    ++P.result.vcycles;
+    const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
+    const double pap = sc[BS_PAP], rr = sc[BS_RR];
+    if (!(pap > 0) || !std::isfinite(rr) || !std::isfinite(pap)) {
+      finish(slot, ProbStatus::BREAKDOWN);
+    } else if (std::sqrt(rr) <= P.tol || P.iters >= P.max_iter) {
+      verify.push_back(slot);
+    } else {
+      keep.push_back(slot);
+    }
+  }
+  if (keep.size() != running_slots_.size()) {
+    running_slots_ = keep;
+    graph_tile_cap_ = -2;
+  }
+  if (!verify.empty()) start(*work_, verify, true);
+}
+
+int BatchEngine::running() const { return (int)running_slots_.size(); }
+
+std::vector<int> BatchEngine::take_finished() {
+  std::vector<int> f;
+  f.swap(finished_);
+  return f;
+}
+
+const ProbResult& BatchEngine::result(int id) const {
+  if (id < 0 || id >= max_slots_ || !probs_[id]) throw InvalidArgument("unknown problem id");
+  return probs_[id]->result;
+}
+
+void BatchEngine::release(int id) {
+  if (id < 0 || id >= max_slots_ || !probs_[id]) throw InvalidArgument("unknown problem id");
+  if (!probs_[id]->done) throw InvalidArgument("problem is still running");
+  INSP_HIP(hipStreamSynchronize(stream_));
+  (void)hipFree(probs_[id]->arena);
+  bytes_in_use_ -= probs_[id]->bytes;
+  probs_[id].reset();
+  free_slots_.push_back(id);
+}
+
+}  // namespace insp

@@ -1,0 +1,589 @@
+// Batched kernels: one block per 256-cell tile, any problem, any order.
+// Grids are sized to a fixed capacity; blocks past the device tile count exit,
+// so a captured graph stays valid while problems are admitted and retired.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "batch.h"
+#include "reduce.h"
+#include "sipg_cell.h"
+
+namespace insp {
+namespace batch {
+
+namespace {
+
+inline void check(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("batched kernel launch failed: ") + what + ": " +
+                             hipGetErrorString(e));
+}
+
+constexpr int kB = kTile;
+
+struct Cell {
+  int t, slot;
+  int64_t e;
+  bool live, valid;
+};
+
+__device__ __forceinline__ Cell locate(const BatchView& b) {
+  Cell c;
+  c.t = (int)xcd_block();
+  c.live = c.t < *b.ntiles;
+  c.slot = 0;
+  c.e = 0;
+  c.valid = false;
+  if (c.live) {
+    const int2 tt = b.tiles[c.t];
+    c.slot = tt.x;
+    c.e = tt.y + threadIdx.x;
+    c.valid = c.e < b.P[c.slot].E;
+  }
+  return c;
+}
+
+template <int K>
+__device__ __forceinline__ void put_partials(const BatchView& b, const Cell& c, const double (&v)[K]) {
+  if (!c.live) return;  // uniform per block
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = block_sum1<kB>(v[k]);
+    if (threadIdx.x == 0) b.partial[(int64_t)k * b.tile_cap + c.t] = s;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ const T* ctab(const BatchTables& tb, int N) {
+  if constexpr (std::is_same<T, float>::value) return tb.C32[N];
+  else return tb.C64[N];
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kB) kb_trace(BatchView b, BatchTables tb, int in, int tr) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const int prec = std::is_same<T, float>::value;
+  const T* u = (const T*)P.v[in];
+  T* trp = (T*)P.v[tr];
+  const T* k = (const T*)P.kF[prec];
+  const T* C = ctab<T>(tb, P.N);
+  switch (P.N) {
+    case 2:
+      if (k) trace_cell<DIM, 2, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
+      else trace_cell<DIM, 2, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
+      break;
+    case 3:
+      if (k) trace_cell<DIM, 3, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
+      else trace_cell<DIM, 3, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
+      break;
+    case 4:
+      if (k) trace_cell<DIM, 4, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
+      else trace_cell<DIM, 4, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
+      break;
+    default:
+      break;  // order 1 needs no trace pass
+  }
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kB)
+kb_apply(BatchView b, BatchTables tb, int level, int in, int tr, int out, int dot) {
+  const Cell c = locate(b);
+  double dv[1] = {0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const int prec = std::is_same<T, float>::value;
+    const int N = level ? 1 : P.N;
+    const T* u = (const T*)P.v[in];
+    T* o = (T*)P.v[out];
+    const double* tau = level ? P.tauC : P.tauF;
+    const T* k = (const T*)(level ? P.kC[prec] : P.kF[prec]);
+    const T* C = ctab<T>(tb, N);
+    const int* tab = tb.tab[N];
+    if (N == 1) {
+      dv[0] = k ? apply_p1_cell<DIM, T, true>(c.e, P.S, u, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o)
+                : apply_p1_cell<DIM, T, false>(c.e, P.S, u, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o);
+    } else {
+      const T* trp = (const T*)P.v[tr];
+#define INSP_BAPPLY(NN)                                                                            \
+  dv[0] = k ? apply_cell<DIM, NN, T, true>(c.e, P.S, u, trp, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o) \
+            : apply_cell<DIM, NN, T, false>(c.e, P.S, u, trp, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o)
+      switch (N) {
+        case 2: INSP_BAPPLY(2); break;
+        case 3: INSP_BAPPLY(3); break;
+        case 4: INSP_BAPPLY(4); break;
+        default: break;
+      }
+#undef INSP_BAPPLY
+    }
+  }
+  if (dot) put_partials<1>(b, c, dv);
+}
+
+__global__ void __launch_bounds__(kB) kb_finalize(BatchView b, int nvals, int slot0) {
+  const int j = blockIdx.x;
+  if (j >= *b.nprobs) return;
+  const int slot = b.probs[j];
+  const int2 r = b.ranges[slot];
+  for (int k = 0; k < nvals; ++k) {
+    double v = 0.0;
+    for (int q = threadIdx.x; q < r.y - r.x; q += kB) v += b.partial[(int64_t)k * b.tile_cap + r.x + q];
+    v = block_sum1<kB>(v);
+    if (threadIdx.x == 0) b.P[slot].scal[slot0 + k] = v;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ int rows_of(const BatchProb& P, int level) { return level ? P.Np1 : P.Np; }
+
+template <typename T>
+__global__ void __launch_bounds__(kB)
+kb_dot(BatchView b, int level, int fa, int fb, int fc, int fd, int nvals) {
+  const Cell c = locate(b);
+  double v[2] = {0.0, 0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const T *a = (const T*)P.v[fa], *bb = (const T*)P.v[fb];
+    const T *cc = nvals > 1 ? (const T*)P.v[fc] : nullptr, *dd = nvals > 1 ? (const T*)P.v[fd] : nullptr;
+    const int rows = rows_of<T>(P, level);
+    for (int i = 0; i < rows; ++i) {
+      const int64_t q = i * P.S + c.e;
+      v[0] += (double)a[q] * (double)bb[q];
+      if (nvals > 1) v[1] += (double)cc[q] * (double)dd[q];
+    }
+  }
+  put_partials<2>(b, c, v);
+}
+
+__device__ __forceinline__ double sdiv(double a, double b) { return b != 0.0 ? a / b : 0.0; }
+
+__global__ void __launch_bounds__(kB) kb_fcg_update_xr(BatchView b) {
+  const Cell c = locate(b);
+  double v[2] = {0.0, 0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const double alpha = sdiv(P.scal[BS_RZ], P.scal[BS_PAP]);
+    double *x = (double*)P.v[BV_X], *r = (double*)P.v[BV_R];
+    const double *p = (const double*)P.v[BV_P], *Ap = (const double*)P.v[BV_AP], *z = (const double*)P.v[BV_Z];
+    for (int i = 0; i < P.Np; ++i) {
+      const int64_t q = i * P.S + c.e;
+      x[q] = fma(alpha, p[q], x[q]);
+      const double ri = fma(-alpha, Ap[q], r[q]);
+      r[q] = ri;
+      v[0] += ri * ri;
+      v[1] += ri * z[q];
+    }
+  }
+  put_partials<2>(b, c, v);
+}
+
+__global__ void __launch_bounds__(kB) kb_fcg_update_p(BatchView b) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  // flexible (Polak-Ribiere) beta = r.(z - z_old) / (r_old.z_old)  (cg_compute_beta_pre_flex.h)
+  const double beta = sdiv(P.scal[BS_RZN] - P.scal[BS_RZO], P.scal[BS_RZ]);
+  double* p = (double*)P.v[BV_P];
+  const double* z = (const double*)P.v[BV_Z];
+  for (int i = 0; i < P.Np; ++i) {
+    const int64_t q = i * P.S + c.e;
+    p[q] = fma(beta, p[q], z[q]);
+  }
+}
+
+__global__ void kb_scal_copy(BatchView b, int dst, int src) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *b.nprobs) return;
+  double* s = b.P[b.probs[j]].scal;
+  s[dst] = s[src];
+}
+
+__global__ void __launch_bounds__(kB) kb_residual(BatchView b) {
+  const Cell c = locate(b);
+  double v[1] = {0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const double *bb = (const double*)P.v[BV_B], *Ap = (const double*)P.v[BV_AP];
+    double* r = (double*)P.v[BV_R];
+    for (int i = 0; i < P.Np; ++i) {
+      const int64_t q = i * P.S + c.e;
+      const double t = bb[q] - Ap[q];
+      r[q] = t;
+      v[0] += t * t;
+    }
+  }
+  put_partials<1>(b, c, v);
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kB) kb_convert(BatchView b, int src, int dst) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const Tin* x = (const Tin*)P.v[src];
+  Tout* y = (Tout*)P.v[dst];
+  for (int i = 0; i < P.Np; ++i) y[i * P.S + c.e] = (Tout)x[i * P.S + c.e];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int step, int zero_init) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const int prec = std::is_same<T, float>::value;
+  const T* inv = (const T*)P.invF[prec];
+  const T *f = (const T*)P.v[BV_VB], *ax = (const T*)P.v[BV_VAX];
+  T *d = (T*)P.v[BV_VD], *x = (T*)P.v[BV_VX];
+  const T c1 = (T)P.c1[step], c2 = (T)P.c2[step];
+  for (int i = 0; i < P.Np; ++i) {
+    const int64_t q = i * P.S + c.e;
+    const T res = inv[q] * (zero_init ? f[q] : f[q] - ax[q]);
+    const T di = step == 0 ? c2 * res : fma(c1, d[q], c2 * res);
+    d[q] = di;
+    x[q] = zero_init ? di : x[q] + di;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_sub_from_b(BatchView b) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const T* f = (const T*)P.v[BV_VB];
+  T* ax = (T*)P.v[BV_VAX];
+  for (int i = 0; i < P.Np; ++i) {
+    const int64_t q = i * P.S + c.e;
+    ax[q] = f[q] - ax[q];
+  }
+}
+
+template <typename T, int R, int K, bool ADD>
+__device__ __forceinline__ void cell_matvec(const T* __restrict__ A, const T* __restrict__ x,
+                                            T* __restrict__ y, int64_t S, int64_t e) {
+  T xv[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) xv[j] = x[j * S + e];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    T a = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) a = fma(A[i * K + j], xv[j], a);
+    y[i * S + e] = ADD ? y[i * S + e] + a : a;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ const T* ptab(const BatchTables& tb, int N, bool restrict_op) {
+  if constexpr (std::is_same<T, float>::value) return restrict_op ? tb.Pt32[N] : tb.P32[N];
+  else return restrict_op ? tb.Pt64[N] : tb.P64[N];
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kB) kb_restrict(BatchView b, BatchTables tb) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const T* r = (const T*)P.v[BV_VAX];
+  T* cb = (T*)P.v[BV_CB];
+  constexpr int n1 = DIM + 1;
+  constexpr int n2 = DIM == 2 ? 6 : 10, n3 = DIM == 2 ? 10 : 20, n4 = DIM == 2 ? 15 : 35;
+  const T* A = ptab<T>(tb, P.N, true);
+  switch (P.N) {
+    case 1: for (int i = 0; i < n1; ++i) cb[i * P.S + c.e] = r[i * P.S + c.e]; break;
+    case 2: cell_matvec<T, n1, n2, false>(A, r, cb, P.S, c.e); break;
+    case 3: cell_matvec<T, n1, n3, false>(A, r, cb, P.S, c.e); break;
+    case 4: cell_matvec<T, n1, n4, false>(A, r, cb, P.S, c.e); break;
+    default: break;
+  }
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kB) kb_prolong(BatchView b, BatchTables tb) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const T* cx = (const T*)P.v[BV_CX];
+  T* x = (T*)P.v[BV_VX];
+  constexpr int n1 = DIM + 1;
+  constexpr int n2 = DIM == 2 ? 6 : 10, n3 = DIM == 2 ? 10 : 20, n4 = DIM == 2 ? 15 : 35;
+  const T* A = ptab<T>(tb, P.N, false);
+  switch (P.N) {
+    case 1: for (int i = 0; i < n1; ++i) x[i * P.S + c.e] += cx[i * P.S + c.e]; break;
+    case 2: cell_matvec<T, n2, n1, true>(A, cx, x, P.S, c.e); break;
+    case 3: cell_matvec<T, n3, n1, true>(A, cx, x, P.S, c.e); break;
+    case 4: cell_matvec<T, n4, n1, true>(A, cx, x, P.S, c.e); break;
+    default: break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_cg_init(BatchView b) {
+  const Cell c = locate(b);
+  double v[1] = {0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const int prec = std::is_same<T, float>::value;
+    const T* inv = (const T*)P.invC[prec];
+    const T* cb = (const T*)P.v[BV_CB];
+    T *x = (T*)P.v[BV_CX], *r = (T*)P.v[BV_CR], *z = (T*)P.v[BV_CZ], *p = (T*)P.v[BV_CP];
+    for (int i = 0; i < P.Np1; ++i) {
+      const int64_t q = i * P.S + c.e;
+      const T ri = cb[q], zi = inv[q] * ri;
+      x[q] = 0;
+      r[q] = ri;
+      z[q] = zi;
+      p[q] = zi;
+      v[0] += (double)ri * (double)zi;
+    }
+  }
+  put_partials<1>(b, c, v);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_cg_xr(BatchView b, int num) {
+  const Cell c = locate(b);
+  double v[1] = {0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const int prec = std::is_same<T, float>::value;
+    const T a = (T)sdiv(P.scal[num], P.scal[BS_CPAP]);
+    const T* inv = (const T*)P.invC[prec];
+    const T *p = (const T*)P.v[BV_CP], *Ap = (const T*)P.v[BV_CAP];
+    T *x = (T*)P.v[BV_CX], *r = (T*)P.v[BV_CR], *z = (T*)P.v[BV_CZ];
+    for (int i = 0; i < P.Np1; ++i) {
+      const int64_t q = i * P.S + c.e;
+      x[q] = fma(a, p[q], x[q]);
+      const T ri = fma(-a, Ap[q], r[q]), zi = inv[q] * ri;
+      r[q] = ri;
+      z[q] = zi;
+      v[0] += (double)ri * (double)zi;
+    }
+  }
+  put_partials<1>(b, c, v);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_cg_p(BatchView b, int num, int den) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const T beta = (T)sdiv(P.scal[num], P.scal[den]);
+  const T* z = (const T*)P.v[BV_CZ];
+  T* p = (T*)P.v[BV_CP];
+  for (int i = 0; i < P.Np1; ++i) {
+    const int64_t q = i * P.S + c.e;
+    p[q] = fma(beta, p[q], z[q]);
+  }
+}
+
+__global__ void __launch_bounds__(kB) kb_sum_d(BatchView b, int field) {
+  const Cell c = locate(b);
+  double v[1] = {0.0};
+  if (c.valid) {
+    const BatchProb& P = b.P[c.slot];
+    const double* x = (const double*)P.v[field];
+    for (int i = 0; i < P.Np; ++i) v[0] += x[i * P.S + c.e];
+  }
+  put_partials<1>(b, c, v);
+}
+
+__global__ void __launch_bounds__(kB) kb_sub_mean_d(BatchView b, int field) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  if (!P.nullspace) return;
+  const double m = P.scal[BS_MEAN] / P.dofs;
+  double* x = (double*)P.v[field];
+  for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] -= m;
+}
+
+__global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int field) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  double* x = (double*)P.v[field];
+  for (int i = 0; i < P.Np; ++i) {
+    const uint64_t idx = (uint64_t)(i * P.S + c.e);
+    uint64_t z = 0x5eedULL + 977ULL * P.N + 0x9E3779B97F4A7C15ull * (idx + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    x[i * P.S + c.e] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+__global__ void __launch_bounds__(kB) kb_inv_apply_d(BatchView b, int field) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const double* inv = (const double*)P.invF[0];
+  double* x = (double*)P.v[field];
+  for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] *= inv[i * P.S + c.e];
+}
+
+__global__ void __launch_bounds__(kB) kb_scale_d(BatchView b, int field, int slot) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const double s = P.scal[slot] > 0 ? 1.0 / sqrt(P.scal[slot]) : 0.0;
+  double* x = (double*)P.v[field];
+  for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] *= s;
+}
+
+dim3 tiles_grid(const BatchView& b) { return dim3((unsigned)b.tile_cap); }
+
+}  // namespace
+
+#define INSP_DIMT(CALL)                     \
+  do {                                      \
+    if (b.dim == 2) { constexpr int D_ = 2; CALL; } \
+    else { constexpr int D_ = 3; CALL; }    \
+  } while (0)
+
+template <typename T>
+void trace(const BatchView& b, const BatchTables& t, int in, int tr, hipStream_t s) {
+  INSP_DIMT(hipLaunchKernelGGL((kb_trace<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t, in, tr));
+  check("kb_trace");
+}
+
+template <typename T>
+void apply(const BatchView& b, const BatchTables& t, int level, int in, int tr, int out, bool dot,
+           hipStream_t s) {
+  INSP_DIMT(hipLaunchKernelGGL((kb_apply<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t, level, in, tr,
+                               out, (int)dot));
+  check("kb_apply");
+}
+
+void finalize(const BatchView& b, int nvals, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(kb_finalize, dim3((unsigned)b.prob_cap), dim3(kB), 0, s, b, nvals, slot);
+  check("kb_finalize");
+}
+
+template <typename T>
+void dot(const BatchView& b, int level, int a, int bb, int c, int d, int nvals, hipStream_t s) {
+  hipLaunchKernelGGL((kb_dot<T>), tiles_grid(b), dim3(kB), 0, s, b, level, a, bb, c, d, nvals);
+  check("kb_dot");
+}
+
+void fcg_update_xr(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL(kb_fcg_update_xr, tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_fcg_update_xr");
+}
+
+void fcg_update_p(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL(kb_fcg_update_p, tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_fcg_update_p");
+}
+
+void scal_copy(const BatchView& b, int dst, int src, hipStream_t s) {
+  hipLaunchKernelGGL(kb_scal_copy, dim3((unsigned)((b.prob_cap + 63) / 64)), dim3(64), 0, s, b, dst, src);
+  check("kb_scal_copy");
+}
+
+void residual(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL(kb_residual, tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_residual");
+}
+
+void to_t(const BatchView& b, bool fp32, int src_d, int dst, hipStream_t s) {
+  if (fp32) hipLaunchKernelGGL((kb_convert<double, float>), tiles_grid(b), dim3(kB), 0, s, b, src_d, dst);
+  else hipLaunchKernelGGL((kb_convert<double, double>), tiles_grid(b), dim3(kB), 0, s, b, src_d, dst);
+  check("kb_convert");
+}
+
+void from_t(const BatchView& b, bool fp32, int src, int dst_d, hipStream_t s) {
+  if (fp32) hipLaunchKernelGGL((kb_convert<float, double>), tiles_grid(b), dim3(kB), 0, s, b, src, dst_d);
+  else hipLaunchKernelGGL((kb_convert<double, double>), tiles_grid(b), dim3(kB), 0, s, b, src, dst_d);
+  check("kb_convert");
+}
+
+template <typename T>
+void cheb(const BatchView& b, int step, bool zero_init, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cheb<T>), tiles_grid(b), dim3(kB), 0, s, b, step, (int)zero_init);
+  check("kb_cheb");
+}
+
+template <typename T>
+void sub_from_b(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL((kb_sub_from_b<T>), tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_sub_from_b");
+}
+
+template <typename T>
+void restrict_(const BatchView& b, const BatchTables& t, hipStream_t s) {
+  INSP_DIMT(hipLaunchKernelGGL((kb_restrict<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t));
+  check("kb_restrict");
+}
+
+template <typename T>
+void prolong(const BatchView& b, const BatchTables& t, hipStream_t s) {
+  INSP_DIMT(hipLaunchKernelGGL((kb_prolong<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t));
+  check("kb_prolong");
+}
+
+template <typename T>
+void cg_init(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cg_init<T>), tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_cg_init");
+}
+
+template <typename T>
+void cg_xr(const BatchView& b, int num, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cg_xr<T>), tiles_grid(b), dim3(kB), 0, s, b, num);
+  check("kb_cg_xr");
+}
+
+template <typename T>
+void cg_p(const BatchView& b, int num, int den, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cg_p<T>), tiles_grid(b), dim3(kB), 0, s, b, num, den);
+  check("kb_cg_p");
+}
+
+void sum_d(const BatchView& b, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_sum_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  check("kb_sum_d");
+}
+
+void sub_mean_d(const BatchView& b, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_sub_mean_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  check("kb_sub_mean_d");
+}
+
+void rand_d(const BatchView& b, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_rand_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  check("kb_rand_d");
+}
+
+void inv_apply_d(const BatchView& b, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_inv_apply_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  check("kb_inv_apply_d");
+}
+
+void scale_d(const BatchView& b, int field, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(kb_scale_d, tiles_grid(b), dim3(kB), 0, s, b, field, slot);
+  check("kb_scale_d");
+}
+
+#define INSP_BINST(T)                                                                             \
+  template void trace<T>(const BatchView&, const BatchTables&, int, int, hipStream_t);            \
+  template void apply<T>(const BatchView&, const BatchTables&, int, int, int, int, bool,          \
+                         hipStream_t);                                                            \
+  template void dot<T>(const BatchView&, int, int, int, int, int, int, hipStream_t);              \
+  template void cheb<T>(const BatchView&, int, bool, hipStream_t);                                \
+  template void sub_from_b<T>(const BatchView&, hipStream_t);                                     \
+  template void restrict_<T>(const BatchView&, const BatchTables&, hipStream_t);                  \
+  template void prolong<T>(const BatchView&, const BatchTables&, hipStream_t);                    \
+  template void cg_init<T>(const BatchView&, hipStream_t);                                        \
+  template void cg_xr<T>(const BatchView&, int, hipStream_t);                                     \
+  template void cg_p<T>(const BatchView&, int, int, hipStream_t);
+INSP_BINST(double)
+INSP_BINST(float)
+
+}  // namespace batch
+}  // namespace insp

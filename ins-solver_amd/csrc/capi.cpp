@@ -5,7 +5,11 @@
 #include <new>
 #include <string>
 
+#include <deque>
+
+#include "batch.h"
 #include "mesh.h"
+#include "server.h"
 #include "solver.h"
 
 struct ins_mesh {
@@ -13,6 +17,15 @@ struct ins_mesh {
 };
 struct ins_poisson {
   std::unique_ptr<insp::PoissonSolver> s;
+};
+struct ins_batch {
+  std::unique_ptr<insp::BatchEngine> e;
+  insp::BatchConfig cfg;
+  std::deque<int> finished;
+};
+struct ins_server {
+  std::unique_ptr<insp::Server> s;
+  insp::ServerConfig cfg;
 };
 
 namespace {
@@ -49,6 +62,95 @@ insp::PoissonSolver& S(ins_poisson_t h) {
 const insp::HostMesh& M(ins_mesh_t m) {
   if (!m) throw insp::InvalidArgument("null mesh handle");
   return m->m;
+}
+
+insp::BatchConfig to_batch_config(int dim, const ins_batch_config* c) {
+  insp::BatchConfig b;
+  b.dim = dim;
+  if (!c) return b;
+  if (c->precision != INS_PRECISION_FP64 && c->precision != INS_PRECISION_MIXED)
+    throw insp::InvalidArgument("unknown precision");
+  if (c->cheb_order < 1 || c->cheb_order > 8 || c->pre_it < 1 || c->post_it < 0 ||
+      c->eig_safety <= 0 || c->cheb_min_factor <= 0 || c->cheb_min_factor >= 1 ||
+      c->max_iter_eig < 1 || c->coarse_iters < 0)
+    throw insp::InvalidArgument("invalid batch configuration");
+  b.fp32 = c->precision == INS_PRECISION_MIXED;
+  b.cheb_order = c->cheb_order;
+  b.pre_it = c->pre_it;
+  b.post_it = c->post_it;
+  b.eig_safety = c->eig_safety;
+  b.cheb_min_factor = c->cheb_min_factor;
+  b.max_iter_eig = c->max_iter_eig;
+  b.coarse_iters = c->coarse_iters;
+  b.use_graph = c->use_graph != 0;
+  return b;
+}
+
+int np_of(int dim, int N) { return dim == 2 ? (N + 1) * (N + 2) / 2 : (N + 1) * (N + 2) * (N + 3) / 6; }
+
+// Copies one request into a problem spec. `own` copies the mesh (the request
+// outlives the call); otherwise the spec borrows it for the call's duration.
+insp::ProblemSpec to_spec(const ins_problem& p, bool own) {
+  if (!p.mesh) throw insp::InvalidArgument("problem without a mesh");
+  if (p.order < 1 || p.order > insp::kMaxOrder) throw insp::InvalidArgument("order must be in 1..4");
+  if (!p.rhs) throw insp::InvalidArgument("problem without a rhs");
+  if (!(p.rtol >= 0) || !(p.atol >= 0) || p.max_iter < 1)
+    throw insp::InvalidArgument("invalid tolerance or iteration limit");
+  const insp::HostMesh& m = p.mesh->m;
+  insp::ProblemSpec sp;
+  sp.mesh = own ? std::make_shared<insp::HostMesh>(m)
+                : insp::HostMeshPtr(insp::HostMeshPtr(), const_cast<insp::HostMesh*>(&m));
+  sp.order = p.order;
+  const int64_t n = m.n_cells * np_of(m.dim, p.order);
+  if (p.bc_types) {
+    for (int64_t i = 0; i < m.n_bfaces; ++i)
+      if (p.bc_types[i] != INS_BC_DIRICHLET && p.bc_types[i] != INS_BC_NEUMANN)
+        throw insp::InvalidArgument("unknown boundary type");
+    sp.bc_types.assign(p.bc_types, p.bc_types + m.n_bfaces);
+  }
+  if (p.factor) sp.factor.assign(p.factor, p.factor + n);
+  sp.nullspace = p.nullspace != 0;
+  sp.rtol = p.rtol;
+  sp.atol = p.atol;
+  sp.max_iter = p.max_iter;
+  sp.rhs.assign(p.rhs, p.rhs + n);
+  sp.rhs_nodal = p.rhs_nodal != 0;
+  return sp;
+}
+
+int fill_result(const insp::BatchConfig& cfg, const insp::ProbResult& r, double* x,
+                ins_solve_stats* st) {
+  if (r.status == insp::ProbStatus::RUNNING) throw insp::InvalidArgument("problem is still running");
+  if (x) std::copy(r.x.begin(), r.x.end(), x);
+  if (st) {
+    st->iterations = r.iterations;
+    st->converged = r.status == insp::ProbStatus::CONVERGED;
+    st->rel_residual = r.rel_residual;
+    st->abs_residual = r.rel_residual * r.bnorm;
+    st->bnorm = r.bnorm;
+    st->seconds = r.seconds;
+    st->fine_applies = r.fine_applies;
+    st->smoother_applies =
+        r.vcycles * ((int64_t)(cfg.pre_it + cfg.post_it) * cfg.cheb_order + cfg.coarse_iters);
+  }
+  switch (r.status) {
+    case insp::ProbStatus::CONVERGED: return INS_OK;
+    case insp::ProbStatus::NOT_CONVERGED:
+      g_last_error = "not converged";
+      return INS_NOT_CONVERGED;
+    default:
+      g_last_error = "breakdown: p.Ap <= 0 or a non-finite residual";
+      return INS_ERR_BREAKDOWN;
+  }
+}
+
+insp::BatchEngine& B(ins_batch_t b) {
+  if (!b || !b->e) throw insp::InvalidArgument("null batch handle");
+  return *b->e;
+}
+insp::Server& SV(ins_server_t s) {
+  if (!s || !s->s) throw insp::InvalidArgument("null server handle");
+  return *s->s;
 }
 }  // namespace
 
@@ -377,6 +479,181 @@ int ins_ref_interp(int dim, int nf, int nc, double* P) {
     const insp::RefElement F = insp::make_ref_element(dim, nf), Cc = insp::make_ref_element(dim, nc);
     const std::vector<double> I = insp::interp_matrix(F, Cc);
     std::copy(I.begin(), I.end(), P);
+    return INS_OK;
+  });
+}
+
+int ins_mesh_node_coords(ins_mesh_t mh, int order, double* xyz) {
+  return guarded([&] {
+    const auto& m = M(mh);
+    if (order < 1 || order > insp::kMaxOrder || !xyz) throw insp::InvalidArgument("invalid order or output");
+    insp::mesh_node_coords(m, insp::make_ref_element(m.dim, order), xyz);
+    return INS_OK;
+  });
+}
+
+void ins_batch_config_default(ins_batch_config* c) {
+  if (!c) return;
+  const insp::BatchConfig d;
+  c->precision = d.fp32 ? INS_PRECISION_MIXED : INS_PRECISION_FP64;
+  c->cheb_order = d.cheb_order;
+  c->pre_it = d.pre_it;
+  c->post_it = d.post_it;
+  c->eig_safety = d.eig_safety;
+  c->cheb_min_factor = d.cheb_min_factor;
+  c->max_iter_eig = d.max_iter_eig;
+  c->coarse_iters = d.coarse_iters;
+  c->use_graph = d.use_graph ? 1 : 0;
+}
+
+int ins_batch_problem_bytes(int dim, int64_t n_cells, int order, int has_factor, int precision,
+                            int64_t* bytes) {
+  return guarded([&] {
+    if ((dim != 2 && dim != 3) || n_cells < 1 || order < 1 || order > insp::kMaxOrder || !bytes)
+      throw insp::InvalidArgument("invalid problem size");
+    *bytes = (int64_t)insp::BatchEngine::arena_bytes(dim, n_cells, order, has_factor != 0,
+                                                     precision == INS_PRECISION_MIXED);
+    return INS_OK;
+  });
+}
+
+int ins_batch_create(int dim, const ins_batch_config* cfg, ins_batch_t* out) {
+  return guarded([&] {
+    if (!out || (dim != 2 && dim != 3)) throw insp::InvalidArgument("invalid batch arguments");
+    auto h = std::make_unique<ins_batch>();
+    h->cfg = to_batch_config(dim, cfg);
+    h->e = std::make_unique<insp::BatchEngine>(h->cfg);
+    *out = h.release();
+    return INS_OK;
+  });
+}
+
+void ins_batch_destroy(ins_batch_t b) { delete b; }
+
+int ins_batch_add(ins_batch_t b, const ins_problem* probs, int n, int* ids) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (n < 0 || (n > 0 && (!probs || !ids))) throw insp::InvalidArgument("invalid problem list");
+    std::vector<insp::ProblemSpec> specs;
+    for (int i = 0; i < n; ++i) {
+      if (!probs[i].mesh || probs[i].mesh->m.dim != b->cfg.dim)
+        throw insp::InvalidArgument("problem dimension does not match the batch");
+      specs.push_back(to_spec(probs[i], false));
+    }
+    const std::vector<int> got = e.add_many(std::move(specs));
+    std::copy(got.begin(), got.end(), ids);
+    return INS_OK;
+  });
+}
+
+int ins_batch_iterate(ins_batch_t b, int max_iters, int* running) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (max_iters < 0) throw insp::InvalidArgument("negative iteration count");
+    for (int i = 0; i < max_iters && e.running() > 0; ++i) e.iterate();
+    if (running) *running = e.running();
+    return INS_OK;
+  });
+}
+
+int ins_batch_take_finished(ins_batch_t b, int* ids, int cap, int* n) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (cap < 0 || (cap > 0 && !ids) || !n) throw insp::InvalidArgument("invalid output buffer");
+    for (int id : e.take_finished()) b->finished.push_back(id);
+    int k = 0;
+    while (k < cap && !b->finished.empty()) {
+      ids[k++] = b->finished.front();
+      b->finished.pop_front();
+    }
+    *n = k;
+    return INS_OK;
+  });
+}
+
+int ins_batch_result(ins_batch_t b, int id, double* x, ins_solve_stats* stats) {
+  return guarded([&] { return fill_result(b ? b->cfg : insp::BatchConfig{}, B(b).result(id), x, stats); });
+}
+
+int ins_batch_release(ins_batch_t b, int id) {
+  return guarded([&] {
+    B(b).release(id);
+    return INS_OK;
+  });
+}
+
+int ins_batch_stats(ins_batch_t b, int64_t* bytes_in_use, int64_t* iterations_run, int* running) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (bytes_in_use) *bytes_in_use = (int64_t)e.bytes_in_use();
+    if (iterations_run) *iterations_run = e.iterations_run();
+    if (running) *running = e.running();
+    return INS_OK;
+  });
+}
+
+int ins_server_create(int64_t hbm_budget_bytes, const ins_batch_config* cfg2d,
+                      const ins_batch_config* cfg3d, ins_server_t* out) {
+  return guarded([&] {
+    if (!out || hbm_budget_bytes <= 0) throw insp::InvalidArgument("invalid server arguments");
+    auto h = std::make_unique<ins_server>();
+    h->cfg.hbm_budget = (size_t)hbm_budget_bytes;
+    h->cfg.cfg2d = to_batch_config(2, cfg2d);
+    h->cfg.cfg3d = to_batch_config(3, cfg3d);
+    h->s = std::make_unique<insp::Server>(h->cfg);
+    *out = h.release();
+    return INS_OK;
+  });
+}
+
+void ins_server_destroy(ins_server_t s) { delete s; }
+
+int ins_server_submit(ins_server_t s, const ins_problem* p, int64_t* id) {
+  return guarded([&] {
+    auto& sv = SV(s);
+    if (!p || !id) throw insp::InvalidArgument("invalid request");
+    *id = sv.submit(to_spec(*p, true));
+    return INS_OK;
+  });
+}
+
+int ins_server_poll(ins_server_t s, int64_t id, int* done) {
+  return guarded([&] {
+    auto& sv = SV(s);
+    if (!done) throw insp::InvalidArgument("null output");
+    *done = sv.poll(id) ? 1 : 0;
+    return INS_OK;
+  });
+}
+
+int ins_server_wait(ins_server_t s, int64_t id, double* x, ins_solve_stats* stats, double* latency_s,
+                    double* queue_s) {
+  return guarded([&] {
+    auto& sv = SV(s);
+    insp::RequestResult r = sv.wait(id);
+    if (latency_s) *latency_s = r.latency_s;
+    if (queue_s) *queue_s = r.queue_s;
+    if (r.rejected) {
+      g_last_error = "request needs more device memory than the server budget";
+      return (int)INS_ERR_NOMEM;
+    }
+    const insp::ServerConfig& c = s->cfg;
+    return fill_result(r.dim == 3 ? c.cfg3d : c.cfg2d, r.result, x, stats);
+  });
+}
+
+int ins_server_wait_all(ins_server_t s) {
+  return guarded([&] {
+    SV(s).wait_all();
+    return INS_OK;
+  });
+}
+
+int ins_server_stats(ins_server_t s, int64_t* completed, int64_t* peak_bytes) {
+  return guarded([&] {
+    auto& sv = SV(s);
+    if (completed) *completed = sv.completed();
+    if (peak_bytes) *peak_bytes = (int64_t)sv.peak_bytes();
     return INS_OK;
   });
 }

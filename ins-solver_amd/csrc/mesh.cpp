@@ -258,4 +258,27 @@ void build_geometry(HostMesh& m) {
   }
 }
 
+void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz) {
+  const int d = m.dim, Np = re.Np;
+  const int64_t E = m.n_cells;
+  for (int64_t e = 0; e < E; ++e) {
+    const int* c = &m.cell2nodes[(size_t)e * (d + 1)];
+    for (int i = 0; i < Np; ++i) {
+      // affine map from the reference simplex: barycentric weights of the vertices
+      double lam[4];
+      const double r = re.r[i], s = re.s[i], t = d == 3 ? re.t[i] : 0.0;
+      if (d == 2) {
+        lam[0] = -(r + s) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2;
+      } else {
+        lam[0] = -(1 + r + s + t) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2; lam[3] = (1 + t) / 2;
+      }
+      for (int x = 0; x < d; ++x) {
+        double v = 0;
+        for (int k = 0; k <= d; ++k) v += lam[k] * m.coords[(size_t)c[k] * d + x];
+        xyz[((size_t)x * E + e) * Np + i] = v;
+      }
+    }
+  }
+}
+
 }  // namespace insp

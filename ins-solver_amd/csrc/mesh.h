@@ -43,4 +43,8 @@ HostMesh make_cube_mesh(int n, double jitter, uint64_t seed);
 void build_topology(HostMesh& m);
 void build_geometry(HostMesh& m);
 
+struct RefElement;
+// Nodal coordinates of an order-N element set, dim x cells x Np.
+void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz);
+
 }  // namespace insp

@@ -1,0 +1,318 @@
+// Per-cell bodies of the SIPG operator kernels (trace pass, element pass, the
+// one-pass order-1 element pass), shared by the single-solve and the batched
+// kernels. See sipg_kernels.hip for the formulation and its reference citations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dg_layout.h"
+#include "reduce.h"
+
+namespace insp {
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X. Give each
+// XCD a contiguous range of cells so that neighbour gathers hit its own L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
+  return (int64_t)(x * q + (x < rem ? x : rem) + k);
+}
+
+// Trace pass for one cell: T = sJ n.(k grad u) at every face node.
+template <int DIM, int N, typename T, bool VAR>
+__device__ __forceinline__ void
+trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
+           const T* __restrict__ kfac, const T* __restrict__ C, T* __restrict__ tr) {
+  constexpr OpLayout L = op_layout(DIM, N);
+  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  constexpr FmaskTable FM = make_fmask(DIM, N);
+  T uu[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) uu[m] = u[m * stride + e];
+  T K[L.NK];
+#pragma unroll
+  for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
+  T beta[Nf][DIM];
+#pragma unroll
+  for (int f = 0; f < Nf; ++f)
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T b = 0;
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
+      beta[f][s] = b;
+    }
+#pragma unroll
+  for (int p = 0; p < Np; ++p) {
+    T g[DIM];
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T a = 0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np + p) * Np + m], uu[m], a);
+      g[s] = a;
+    }
+    T kp = 1;
+    if constexpr (VAR) kp = kfac[p * stride + e];
+#pragma unroll
+    for (int f = 0; f < Nf; ++f)
+#pragma unroll
+      for (int j = 0; j < Nfp; ++j)
+        if (FM.v[f * Nfp + j] == p) {
+          T v = 0;
+#pragma unroll
+          for (int s = 0; s < DIM; ++s) v = fma(beta[f][s], g[s], v);
+          tr[(int64_t)(f * Nfp + j) * stride + e] = kp * v;
+        }
+  }
+}
+
+// One cell of the element pass; returns u.(A u) over the cell.
+template <int DIM, int N, typename T, bool VAR>
+__device__ __forceinline__ double
+apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
+           const double* __restrict__ Kt, const double* __restrict__ tauS,
+           const int* __restrict__ nbr, const int* __restrict__ code,
+           const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
+           const int* __restrict__ tab, T* __restrict__ out) {
+  constexpr OpLayout L = op_layout(DIM, N);
+  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  constexpr FmaskTable FM = make_fmask(DIM, N);
+  T uu[Np], acc[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    uu[m] = u[m * stride + e];
+    acc[m] = 0;
+  }
+  T K[L.NK];
+#pragma unroll
+  for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
+
+  // Volume term, one node column at a time.
+  for (int j = 0; j < Np; ++j) {
+    T g[DIM];
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T a = 0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np + j) * Np + m], uu[m], a);
+      g[s] = a;
+    }
+    T kj = 1;
+    if constexpr (VAR) kj = kfac[j * stride + e];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      T h = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) h = fma(K[kidx(DIM, r, s)], g[s], h);
+      h *= kj;
+#pragma unroll
+      for (int i = 0; i < Np; ++i) acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h, acc[i]);
+    }
+  }
+
+  // Face terms.
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) {
+    T beta[DIM];
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T b = 0;
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
+      beta[s] = b;
+    }
+    const int nb = nbr[f * stride + e];
+    int mode = 0;  // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms)
+    int tbase = 0;
+    if (nb >= 0) {
+      tbase = L.tab_nbr + (f * Nf * L.nperm + code[f * stride + e]) * Nfp;
+    } else {
+      mode = bctype[-nb - 1] == 0 ? 1 : 2;
+    }
+    if (mode == 2) continue;
+    const T ts = (T)tauS[f * stride + e];
+    for (int j = 0; j < Nfp; ++j) {
+      const int p = tab[L.tab_fmask + f * Nfp + j];
+      const T um = u[p * stride + e];
+      const T Tm = tr[(int64_t)(f * Nfp + j) * stride + e];
+      T up, Tp;
+      if (mode == 0) {
+        const int pk = tab[tbase + j];
+        up = u[(int64_t)(pk >> 6) * stride + nb];
+        Tp = tr[(int64_t)(pk & 63) * stride + nb];
+      } else {
+        up = -um;
+        Tp = -Tm;
+      }
+      const T jump = um - up;
+      const T F = (T)0.5 * (ts * jump - (Tm - Tp));
+      T sig = (T)-0.5 * jump;
+      if constexpr (VAR) sig *= kfac[p * stride + e];
+#pragma unroll
+      for (int i = 0; i < Nfp; ++i)
+        acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        const T bs = beta[r] * sig;
+#pragma unroll
+        for (int i = 0; i < Np; ++i)
+          acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
+      }
+    }
+  }
+  double dotv = 0.0;
+#pragma unroll
+  for (int i = 0; i < Np; ++i) {
+    out[i * stride + e] = acc[i];
+    dotv += (double)uu[i] * (double)acc[i];
+  }
+  return dotv;
+}
+
+// Order 1 in one pass: the gradient of a P1 cell is constant, so the neighbour's
+// flux trace k n.grad(u) is recomputed from its Np nodal values and metric instead
+// of being stored by a trace pass. Same operator as apply_cell.
+__device__ __forceinline__ double face_alpha_rt(int dim, int f, int r) {
+  if (dim == 2) {
+    switch (f) {
+      case 0: return r == 1 ? -1.0 : 0.0;
+      case 1: return 1.0;
+      default: return r == 0 ? -1.0 : 0.0;
+    }
+  }
+  switch (f) {
+    case 0: return r == 2 ? -1.0 : 0.0;
+    case 1: return r == 1 ? -1.0 : 0.0;
+    case 2: return 1.0;
+    default: return r == 0 ? -1.0 : 0.0;
+  }
+}
+
+template <int DIM, typename T, bool VAR>
+__device__ __forceinline__ double
+apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
+              const double* __restrict__ tauS, const int* __restrict__ nbr,
+              const int* __restrict__ code, const int* __restrict__ bctype,
+              const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
+              T* __restrict__ out) {
+  constexpr OpLayout L = op_layout(DIM, 1);
+  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  constexpr FmaskTable FM = make_fmask(DIM, 1);
+  double dotv = 0.0;
+  {
+    T uu[Np], acc[Np], kk[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      uu[m] = u[m * stride + e];
+      acc[m] = 0;
+      kk[m] = VAR ? kfac[m * stride + e] : (T)1;
+    }
+    T K[L.NK];
+#pragma unroll
+    for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
+    T g[DIM];
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) {
+      T a = 0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], uu[m], a);
+      g[s] = a;
+    }
+    T h[DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      T a = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
+      h[r] = a;
+    }
+#pragma unroll
+    for (int j = 0; j < Np; ++j)
+#pragma unroll
+      for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int i = 0; i < Np; ++i)
+          acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h[r] * kk[j], acc[i]);
+#pragma unroll
+    for (int f = 0; f < Nf; ++f) {
+      T beta[DIM];
+      T tm0 = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) {
+        T b = 0;
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
+        beta[s] = b;
+        tm0 = fma(b, g[s], tm0);
+      }
+      const int nb = nbr[f * stride + e];
+      int mode = 0;
+      int tbase = 0;
+      T tn0 = 0;
+      if (nb >= 0) {
+        const int c = code[f * stride + e];
+        const int fn = c / L.nperm;
+        tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
+        T gn[DIM];
+#pragma unroll
+        for (int s = 0; s < DIM; ++s) gn[s] = 0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m) {
+          const T um = u[m * stride + nb];
+#pragma unroll
+          for (int s = 0; s < DIM; ++s) gn[s] = fma(C[L.off_D + (s * Np) * Np + m], um, gn[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < DIM; ++s) {
+          T b = 0;
+#pragma unroll
+          for (int r = 0; r < DIM; ++r)
+            b += (T)face_alpha_rt(DIM, fn, r) * (T)Kt[kidx(DIM, r, s) * stride + nb];
+          tn0 = fma(b, gn[s], tn0);
+        }
+      } else {
+        mode = bctype[-nb - 1] == 0 ? 1 : 2;
+      }
+      if (mode == 2) continue;
+      const T ts = (T)tauS[f * stride + e];
+#pragma unroll
+      for (int j = 0; j < Nfp; ++j) {
+        const int p = FM.v[f * Nfp + j];
+        const T um = uu[p];
+        const T Tm = kk[p] * tm0;
+        T up, Tp;
+        if (mode == 0) {
+          const int node = tab[tbase + j] >> 6;
+          up = u[(int64_t)node * stride + nb];
+          Tp = (VAR ? kfac[(int64_t)node * stride + nb] : (T)1) * tn0;
+        } else {
+          up = -um;
+          Tp = -Tm;
+        }
+        const T jump = um - up;
+        const T F = (T)0.5 * (ts * jump - (Tm - Tp));
+        const T sig = (T)-0.5 * jump * kk[p];
+#pragma unroll
+        for (int i = 0; i < Nfp; ++i)
+          acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+          const T bs = beta[r] * sig;
+#pragma unroll
+          for (int i = 0; i < Np; ++i)
+            acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < Np; ++i) {
+      out[i * stride + e] = acc[i];
+      dotv += (double)uu[i] * (double)acc[i];
+    }
+  }
+  return dotv;
+}
+
+
+}  // namespace insp

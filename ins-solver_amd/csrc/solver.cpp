@@ -169,6 +169,55 @@ void PoissonSolver::set_tol_and_iter(double rtol, double atol, int max_iter) {
   max_iter_ = max_iter;
 }
 
+void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<int>& tab) {
+  const int d = re.dim, N = re.N;
+  const OpLayout L = op_layout(d, N);
+  const int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  const FmaskTable fm = make_fmask(d, N);
+  for (int i = 0; i < Nf * Nfp; ++i)
+    if (fm.v[i] != re.fmask[i]) throw std::runtime_error("face mask tables disagree");
+  C.assign((size_t)L.size, 0.0);
+  for (int s = 0; s < d; ++s)
+    for (int j = 0; j < Np; ++j)
+      for (int m = 0; m < Np; ++m)
+        C[L.off_D + (s * Np + j) * Np + m] = re.D[s][(size_t)j * Np + m];
+  for (int r = 0; r < d; ++r)
+    for (int j = 0; j < Np; ++j)
+      for (int i = 0; i < Np; ++i) {
+        double a = 0;
+        for (int k = 0; k < Np; ++k) a += re.D[r][(size_t)k * Np + i] * re.M[(size_t)k * Np + j];
+        C[L.off_Bt + (r * Np + j) * Np + i] = a;
+      }
+  for (int f = 0; f < Nf; ++f)
+    for (int j = 0; j < Nfp; ++j)
+      for (int i = 0; i < Nfp; ++i)
+        C[L.off_MfT + (f * Nfp + j) * Nfp + i] = re.Mface[((size_t)f * Nfp + i) * Nfp + j];
+  for (int f = 0; f < Nf; ++f)
+    for (int r = 0; r < d; ++r)
+      for (int j = 0; j < Nfp; ++j)
+        for (int i = 0; i < Np; ++i) {
+          double a = 0;
+          for (int l = 0; l < Nfp; ++l)
+            a += re.D[r][(size_t)re.fmask[f * Nfp + l] * Np + i] *
+                 re.Mface[((size_t)f * Nfp + l) * Nfp + j];
+          C[L.off_Wt + ((f * d + r) * Nfp + j) * Np + i] = a;
+        }
+  for (int i = 0; i < Np * Np; ++i) C[L.off_M + i] = re.M[i];
+  tab.assign((size_t)L.tab_size, -1);
+  for (int i = 0; i < Nf * Nfp; ++i) tab[L.tab_fmask + i] = re.fmask[i];
+  for (int f = 0; f < Nf; ++f)
+    for (int fn = 0; fn < Nf; ++fn)
+      for (int p = 0; p < L.nperm; ++p)
+        for (int j = 0; j < Nfp; ++j) {
+          const size_t idx = (((size_t)f * Nf + fn) * L.nperm + p) * Nfp + j;
+          const int q = re.nbr_face_node[idx];
+          tab[L.tab_nbr + idx] = (re.fmask[fn * Nfp + q] << 6) | (fn * Nfp + q);
+        }
+  for (int f = 0; f < Nf; ++f)
+    for (int l = 0; l < Nfp; ++l) tab[L.tab_finv + f * Np + re.fmask[f * Nfp + l]] = l;
+}
+
+
 void PoissonSolver::build_levels() {
   std::vector<int> orders;
   if (type_ == SolverType::PMULTIGRID) {
@@ -192,49 +241,10 @@ void PoissonSolver::build_levels() {
     const OpLayout& L = lv->L;
     const RefElement& re = lv->re;
     const int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
-    const FmaskTable fm = make_fmask(d, N);
-    for (int i = 0; i < Nf * Nfp; ++i)
-      if (fm.v[i] != re.fmask[i]) throw std::runtime_error("face mask tables disagree");
-    std::vector<double> C((size_t)L.size, 0.0);
-    for (int s = 0; s < d; ++s)
-      for (int j = 0; j < Np; ++j)
-        for (int m = 0; m < Np; ++m)
-          C[L.off_D + (s * Np + j) * Np + m] = re.D[s][(size_t)j * Np + m];
-    for (int r = 0; r < d; ++r)
-      for (int j = 0; j < Np; ++j)
-        for (int i = 0; i < Np; ++i) {
-          double a = 0;
-          for (int k = 0; k < Np; ++k) a += re.D[r][(size_t)k * Np + i] * re.M[(size_t)k * Np + j];
-          C[L.off_Bt + (r * Np + j) * Np + i] = a;
-        }
-    for (int f = 0; f < Nf; ++f)
-      for (int j = 0; j < Nfp; ++j)
-        for (int i = 0; i < Nfp; ++i)
-          C[L.off_MfT + (f * Nfp + j) * Nfp + i] = re.Mface[((size_t)f * Nfp + i) * Nfp + j];
-    for (int f = 0; f < Nf; ++f)
-      for (int r = 0; r < d; ++r)
-        for (int j = 0; j < Nfp; ++j)
-          for (int i = 0; i < Np; ++i) {
-            double a = 0;
-            for (int l = 0; l < Nfp; ++l)
-              a += re.D[r][(size_t)re.fmask[f * Nfp + l] * Np + i] *
-                   re.Mface[((size_t)f * Nfp + l) * Nfp + j];
-            C[L.off_Wt + ((f * d + r) * Nfp + j) * Np + i] = a;
-          }
-    for (int i = 0; i < Np * Np; ++i) C[L.off_M + i] = re.M[i];
+    std::vector<double> C;
+    std::vector<int> tab;
+    make_op_tables(re, C, tab);
     lv->C64.upload(C.data(), C.size());
-    std::vector<int> tab((size_t)L.tab_size, -1);
-    for (int i = 0; i < Nf * Nfp; ++i) tab[L.tab_fmask + i] = re.fmask[i];
-    for (int f = 0; f < Nf; ++f)
-      for (int fn = 0; fn < Nf; ++fn)
-        for (int p = 0; p < L.nperm; ++p)
-          for (int j = 0; j < Nfp; ++j) {
-            const size_t idx = (((size_t)f * Nf + fn) * L.nperm + p) * Nfp + j;
-            const int q = re.nbr_face_node[idx];
-            tab[L.tab_nbr + idx] = (re.fmask[fn * Nfp + q] << 6) | (fn * Nfp + q);
-          }
-    for (int f = 0; f < Nf; ++f)
-      for (int l = 0; l < Nfp; ++l) tab[L.tab_finv + f * Np + re.fmask[f * Nfp + l]] = l;
     lv->tab.upload(tab.data(), tab.size());
     lv->tauS.alloc((size_t)Nf * stride_);
     lv->diag.alloc((size_t)Np * stride_);
@@ -695,28 +705,7 @@ void PoissonSolver::mass(const double* in, double* out, Layout layout) {
   INSP_HIP(hipStreamSynchronize(stream_));
 }
 
-void PoissonSolver::node_coords(double* xyz) const {
-  const int d = mesh_.dim, Np = np();
-  const RefElement& re = fine_re_;
-  const int64_t E = mesh_.n_cells;
-  for (int64_t e = 0; e < E; ++e) {
-    const int* c = &mesh_.cell2nodes[(size_t)e * (d + 1)];
-    for (int i = 0; i < Np; ++i) {
-      double lam[4];
-      const double r = re.r[i], s = re.s[i], t = d == 3 ? re.t[i] : 0.0;
-      if (d == 2) {
-        lam[0] = -(r + s) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2;
-      } else {
-        lam[0] = -(1 + r + s + t) / 2; lam[1] = (1 + r) / 2; lam[2] = (1 + s) / 2; lam[3] = (1 + t) / 2;
-      }
-      for (int x = 0; x < d; ++x) {
-        double v = 0;
-        for (int k = 0; k <= d; ++k) v += lam[k] * mesh_.coords[(size_t)c[k] * d + x];
-        xyz[((size_t)x * E + e) * Np + i] = v;
-      }
-    }
-  }
-}
+void PoissonSolver::node_coords(double* xyz) const { mesh_node_coords(mesh_, fine_re_, xyz); }
 
 double PoissonSolver::level_lambda_max(int l) const {
   if (l < 0 || l >= (int)levels_.size()) throw InvalidArgument("level out of range");

@@ -73,6 +73,11 @@ struct DBuf {
   bool empty() const { return p == nullptr; }
 };
 
+// Operator constants (D, B^T, face masses, W, M) and the index table
+// (face masks, neighbour node matching, inverse masks) of one order, laid out by
+// op_layout(); values in double.
+void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<int>& tab);
+
 enum class SolverType { JACOBI_PCG = 1, PMULTIGRID = 4 };
 enum class Precision { FP64 = 0, MIXED = 1 };
 enum class Layout { HOST_AOS = 0, DEVICE_SOA = 1, DEVICE_AOS = 2 };

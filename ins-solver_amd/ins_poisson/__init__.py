@@ -40,17 +40,23 @@ NOT_CONVERGED = 1
 ERR_INVALID = -1
 ERR_HIP = -2
 ERR_BREAKDOWN = -3
+ERR_NOMEM = -4
 
 EXPORTED = [
     "ins_mesh_generate_square", "ins_mesh_generate_cube", "ins_mesh_create", "ins_mesh_info",
-    "ins_mesh_get", "ins_mesh_get_faces", "ins_mesh_destroy", "ins_poisson_create",
+    "ins_mesh_get", "ins_mesh_get_faces", "ins_mesh_node_coords", "ins_mesh_destroy", "ins_poisson_create",
     "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords",
     "ins_poisson_set_bc_types", "ins_poisson_set_factor", "ins_poisson_set_bcs",
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
     "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_init", "ins_poisson_solve",
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
-    "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp", "ins_poisson_last_error",
+    "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
+    "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_create", "ins_batch_destroy",
+    "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
+    "ins_batch_release", "ins_batch_stats", "ins_server_create", "ins_server_destroy",
+    "ins_server_submit", "ins_server_poll", "ins_server_wait", "ins_server_wait_all",
+    "ins_server_stats", "ins_poisson_last_error",
 ]
 
 
@@ -74,6 +80,19 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("precision", C.c_int), ("cheb_order", C.c_int), ("pre_it", C.c_int),
+                ("post_it", C.c_int), ("eig_safety", C.c_double), ("cheb_min_factor", C.c_double),
+                ("max_iter_eig", C.c_int), ("coarse_iters", C.c_int), ("use_graph", C.c_int)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("order", C.c_int), ("bc_types", C.c_void_p),
+                ("factor", C.c_void_p), ("nullspace", C.c_int), ("rtol", C.c_double),
+                ("atol", C.c_double), ("max_iter", C.c_int), ("rhs", C.c_void_p),
+                ("rhs_nodal", C.c_int)]
+
+
 def _load():
     if not os.path.exists(_LIB_PATH):
         raise ImportError(f"{_LIB_PATH} is missing: run `python ins-solver_amd/build.py` first")
@@ -87,6 +106,7 @@ def _load():
                           C.POINTER(i64)],
         "ins_mesh_get": [vp, vp, vp],
         "ins_mesh_get_faces": [vp, vp, vp, vp, vp],
+        "ins_mesh_node_coords": [vp, C.c_int, vp],
         "ins_poisson_create": [vp, C.c_int, C.POINTER(vp)],
         "ins_poisson_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(i64),
                              C.POINTER(i64), C.POINTER(i64)],
@@ -111,6 +131,20 @@ def _load():
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
         "ins_ref_interp": [C.c_int, C.c_int, C.c_int, vp],
+        "ins_batch_problem_bytes": [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.POINTER(i64)],
+        "ins_batch_create": [C.c_int, C.POINTER(BatchConfig), C.POINTER(vp)],
+        "ins_batch_add": [vp, C.POINTER(Problem), C.c_int, vp],
+        "ins_batch_iterate": [vp, C.c_int, C.POINTER(C.c_int)],
+        "ins_batch_take_finished": [vp, vp, C.c_int, C.POINTER(C.c_int)],
+        "ins_batch_result": [vp, C.c_int, vp, C.POINTER(SolveStats)],
+        "ins_batch_release": [vp, C.c_int],
+        "ins_batch_stats": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int)],
+        "ins_server_create": [i64, C.POINTER(BatchConfig), C.POINTER(BatchConfig), C.POINTER(vp)],
+        "ins_server_submit": [vp, C.POINTER(Problem), C.POINTER(i64)],
+        "ins_server_poll": [vp, i64, C.POINTER(C.c_int)],
+        "ins_server_wait": [vp, i64, vp, C.POINTER(SolveStats), dp, dp],
+        "ins_server_wait_all": [vp],
+        "ins_server_stats": [vp, C.POINTER(i64), C.POINTER(i64)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -120,6 +154,11 @@ def _load():
     lib.ins_mesh_destroy.restype = None
     lib.ins_poisson_destroy.argtypes = [vp]
     lib.ins_poisson_destroy.restype = None
+    for name in ("ins_batch_destroy", "ins_server_destroy"):
+        getattr(lib, name).argtypes = [vp]
+        getattr(lib, name).restype = None
+    lib.ins_batch_config_default.argtypes = [C.POINTER(BatchConfig)]
+    lib.ins_batch_config_default.restype = None
     lib.ins_poisson_last_error.argtypes = []
     lib.ins_poisson_last_error.restype = C.c_char_p
     return lib
@@ -216,6 +255,12 @@ class Mesh:
         bnum = np.empty(self.n_bfaces, dtype=np.int32)
         _check(lib.ins_mesh_get_faces(self._h, _ptr(f2c), _ptr(fnum), _ptr(b2c), _ptr(bnum)))
         return f2c, fnum, b2c, bnum
+
+    def node_coords(self, order):
+        """Nodal coordinates of order-`order` elements, (dim, cells, Np); host only."""
+        xyz = np.empty((self.dim, self.n_cells, np_of(self.dim, order)))
+        _check(lib.ins_mesh_node_coords(self._h, order, _ptr(xyz)))
+        return xyz
 
     def __del__(self):
         if getattr(self, "_h", None) and lib is not None:
@@ -351,3 +396,155 @@ class PoissonSolver:
                                            C.byref(al)))
         return {"trace_ms": tm.value, "trace_launches": tl.value, "apply_ms": am.value,
                 "apply_launches": al.value}
+
+
+# --- batched solves and serving ----------------------------------------------
+
+def batch_config(**kw):
+    """BatchConfig with library defaults, overridden by keyword (precision, cheb_order, ...)."""
+    c = BatchConfig()
+    lib.ins_batch_config_default(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise TypeError(f"unknown batch option {k}")
+        setattr(c, k, v)
+    return c
+
+
+def problem_bytes(dim, n_cells, order, has_factor=False, precision=PRECISION_MIXED):
+    b = C.c_int64()
+    _check(lib.ins_batch_problem_bytes(dim, n_cells, order, int(bool(has_factor)), precision,
+                                       C.byref(b)))
+    return b.value
+
+
+class _Request:
+    """One problem: keeps its arrays alive while the library may read them."""
+
+    def __init__(self, mesh, order, rhs, bc_types=None, factor=None, nullspace=False, rtol=1e-8,
+                 atol=0.0, max_iter=500, rhs_nodal=False):
+        n = (mesh.n_cells, np_of(mesh.dim, order))
+        self.mesh = mesh
+        self.rhs = _f64(rhs).reshape(n)
+        self.bc = None if bc_types is None else np.ascontiguousarray(bc_types, dtype=np.int32)
+        if self.bc is not None and self.bc.size != mesh.n_bfaces:
+            raise ValueError("bc_types must have one entry per bface")
+        self.factor = None if factor is None else _f64(factor).reshape(n)
+        self.shape = n
+        self.c = Problem(mesh._h, order, _vp(self.bc), _vp(self.factor), int(bool(nullspace)),
+                         rtol, atol, max_iter, _vp(self.rhs), int(bool(rhs_nodal)))
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data
+
+
+class Batch:
+    """Many independent Poisson problems of one dimension solved in lockstep."""
+
+    def __init__(self, dim, **cfg):
+        self.dim = dim
+        self.config = batch_config(**cfg)
+        h = C.c_void_p()
+        _check(lib.ins_batch_create(dim, C.byref(self.config), C.byref(h)))
+        self._h = h
+        self._shape = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ins_batch_destroy(self._h)
+            self._h = None
+
+    def add(self, problems):
+        """problems: list of dicts of _Request keyword arguments. Returns ids."""
+        reqs = [_Request(**p) for p in problems]
+        arr = (Problem * len(reqs))(*[r.c for r in reqs])
+        ids = np.zeros(len(reqs), dtype=np.int32)
+        _check(lib.ins_batch_add(self._h, arr, len(reqs), _ptr(ids)))
+        for i, r in zip(ids, reqs):
+            self._shape[int(i)] = r.shape
+        return [int(i) for i in ids]
+
+    def iterate(self, max_iters=1):
+        run = C.c_int()
+        _check(lib.ins_batch_iterate(self._h, max_iters, C.byref(run)))
+        return run.value
+
+    def run(self, max_iters=1 << 30):
+        return self.iterate(max_iters)
+
+    def take_finished(self):
+        out, buf, n = [], np.zeros(256, dtype=np.int32), C.c_int()
+        while True:
+            _check(lib.ins_batch_take_finished(self._h, _ptr(buf), buf.size, C.byref(n)))
+            out += [int(i) for i in buf[:n.value]]
+            if n.value < buf.size:
+                return out
+
+    def result(self, pid):
+        """(status, x, stats): status OK / NOT_CONVERGED / ERR_BREAKDOWN."""
+        shape = self._shape.get(pid)
+        x = None if shape is None else np.empty(shape)  # unknown ids: the library reports
+        st = SolveStats()
+        rc = lib.ins_batch_result(self._h, pid, _ptr(x), C.byref(st))
+        _check(rc, allow=(OK, NOT_CONVERGED, ERR_BREAKDOWN))
+        return rc, x, st.as_dict()
+
+    def release(self, pid):
+        _check(lib.ins_batch_release(self._h, pid))
+        self._shape.pop(pid, None)
+
+    def stats(self):
+        b, it, run = C.c_int64(), C.c_int64(), C.c_int()
+        _check(lib.ins_batch_stats(self._h, C.byref(b), C.byref(it), C.byref(run)))
+        return {"bytes_in_use": b.value, "iterations_run": it.value, "running": run.value}
+
+
+class Server:
+    """Continuous-batching solve server under an HBM budget (one worker thread)."""
+
+    def __init__(self, hbm_budget_bytes, cfg2d=None, cfg3d=None):
+        self.cfg2d = batch_config(**(cfg2d or {}))
+        self.cfg3d = batch_config(**(cfg3d or {}))
+        h = C.c_void_p()
+        _check(lib.ins_server_create(int(hbm_budget_bytes), C.byref(self.cfg2d),
+                                     C.byref(self.cfg3d), C.byref(h)))
+        self._h = h
+        self._shape = {}
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ins_server_destroy(self._h)
+            self._h = None
+
+    def submit(self, **problem):
+        r = _Request(**problem)
+        rid = C.c_int64()
+        _check(lib.ins_server_submit(self._h, C.byref(r.c), C.byref(rid)))
+        self._shape[rid.value] = r.shape
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int()
+        _check(lib.ins_server_poll(self._h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid):
+        """(status, x, stats, latency_s, queue_s); status ERR_NOMEM (-4) if rejected."""
+        x = np.empty(self._shape[rid])
+        st, lat, q = SolveStats(), C.c_double(), C.c_double()
+        rc = lib.ins_server_wait(self._h, rid, _ptr(x), C.byref(st), C.byref(lat), C.byref(q))
+        _check(rc, allow=(OK, NOT_CONVERGED, ERR_BREAKDOWN, ERR_NOMEM))
+        self._shape.pop(rid, None)
+        return rc, x, st.as_dict(), lat.value, q.value
+
+    def wait_all(self):
+        _check(lib.ins_server_wait_all(self._h))
+
+    def stats(self):
+        c, p = C.c_int64(), C.c_int64()
+        _check(lib.ins_server_stats(self._h, C.byref(c), C.byref(p)))
+        return {"completed": c.value, "peak_bytes": p.value}

@@ -86,3 +86,40 @@ def test_user_mesh_orientation_fixed(ip):
     for c in c2n:
         a, b, d = coords[c]
         assert (b[0] - a[0]) * (d[1] - a[1]) - (d[0] - a[0]) * (b[1] - a[1]) > 0
+
+
+@pytest.mark.parametrize("dim,N", [(2, 1), (2, 4), (3, 2), (3, 3)])
+def test_mesh_node_coords_match_oracle(ip, oracle, dim, N):
+    m = ip.Mesh.square(5, jitter=0.15, seed=4) if dim == 2 else ip.Mesh.cube(2, 0.15, 4)
+    o = oracle.Oracle(*m.arrays(), N)
+    assert np.abs(m.node_coords(N) - o.node_coords()).max() < 1e-14
+
+
+def test_batch_host_side_entry_points(ip):
+    c = ip.batch_config()
+    assert (c.precision, c.cheb_order, c.pre_it, c.post_it, c.max_iter_eig, c.coarse_iters,
+            c.use_graph) == (ip.PRECISION_MIXED, 3, 1, 1, 15, 40, 1)
+    assert abs(c.eig_safety - 1.1) < 1e-15 and abs(c.cheb_min_factor - 0.1) < 1e-15
+    # arena bytes grow with cells, order, a coefficient field and FP64 V-cycle storage
+    b = lambda *a, **k: ip.problem_bytes(*a, **k)
+    assert b(2, 1000, 2) < b(2, 2000, 2) < b(2, 2000, 3)
+    assert b(3, 1000, 3) < b(3, 1000, 3, has_factor=True)
+    assert b(3, 1000, 3) < b(3, 1000, 3, precision=ip.PRECISION_FP64)
+    # cells are padded to whole 256-cell tiles: only the per-bface table grows below a tile
+    assert 0 <= b(2, 256, 2) - b(2, 1, 2) <= 256 * 3 * 4 + 256
+    with pytest.raises(ip.InsPoissonError):
+        b(4, 10, 2)
+    with pytest.raises(ip.InsPoissonError):
+        b(2, 10, 5)
+    # invalid handles and configurations are rejected before any device work
+    h = ctypes.c_void_p()
+    assert ip.lib.ins_batch_create(5, None, ctypes.byref(h)) == ip.ERR_INVALID
+    bad = ip.batch_config(cheb_order=0)
+    assert ip.lib.ins_batch_create(2, ctypes.byref(bad), ctypes.byref(h)) == ip.ERR_INVALID
+    bad = ip.batch_config(precision=7)
+    assert ip.lib.ins_server_create(1 << 30, ctypes.byref(bad), None, ctypes.byref(h)) == ip.ERR_INVALID
+    n = ctypes.c_int()
+    assert ip.lib.ins_batch_iterate(None, 1, ctypes.byref(n)) == ip.ERR_INVALID
+    assert ip.lib.ins_server_poll(None, 0, ctypes.byref(n)) == ip.ERR_INVALID
+    with pytest.raises(TypeError):
+        ip.batch_config(no_such_option=1)
