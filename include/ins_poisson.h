@@ -136,6 +136,22 @@ int ins_poisson_set_precision(ins_poisson_t h, int precision);
 int ins_poisson_set_pmg(ins_poisson_t h, const int* orders, int n_levels, const int* pre_it,
                         const int* post_it, const int* cheb_orders, double eig_safety,
                         double cheb_min_factor, int max_iter_eig, int coarse_iters);
+/* Coarse solve of the p-multigrid V-cycle, in place of the reference's PETSc solve on
+ * the assembled order-1 matrix (PoissonCoarseMatrix + set_coarse_matrix,
+ * src/solvers/2d/ins_solver.cpp:92-96,205):
+ *   INS_COARSE_JACOBI_CG  Jacobi-CG with coarse_iters iterations on order 1 (default);
+ *   INS_COARSE_AMG        Chebyshev smoothing on order 1, then amg_cycles smoothed-aggregation
+ *                         AMG V-cycles on the continuous vertex space (order 1 must be the
+ *                         coarsest p-multigrid order). */
+enum { INS_COARSE_JACOBI_CG = 0, INS_COARSE_AMG = 1 };
+int ins_poisson_set_coarse(ins_poisson_t h, int kind, int amg_cycles);
+/* The assembled coarse matrix of the AMG coarse solve, A_v = P_v^T A_1 P_v on the mesh
+ * vertices (CSR, n x n). With NULL arrays only *n and *nnz are returned. Needs init with
+ * INS_COARSE_AMG. */
+int ins_poisson_coarse_matrix(ins_poisson_t h, int64_t* n, int64_t* nnz, int* rowptr, int* cols,
+                              double* vals);
+/* Number of AMG levels and the size of each (sizes may be NULL; at most 16 levels). */
+int ins_poisson_amg_levels(ins_poisson_t h, int* n_levels, int* sizes);
 /* LinearSolver::init() - src/solvers/2d/ins_solver.cpp:207: penalties, diagonals and
  * Chebyshev eigenvalue bounds. Called implicitly by solve when needed. */
 int ins_poisson_init(ins_poisson_t h);

@@ -355,7 +355,46 @@ int ins_poisson_set_pmg(ins_poisson_t h, const int* orders, int n_levels, const 
     c.cheb_min_factor = cheb_min_factor;
     c.max_iter_eig = max_iter_eig;
     c.coarse_iters = coarse_iters;
+    c.coarse = S(h).pmg().coarse;  // the coarse solve kind is set by ins_poisson_set_coarse
+    c.amg_cycles = S(h).pmg().amg_cycles;
     S(h).set_pmg(c);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_coarse(ins_poisson_t h, int kind, int amg_cycles) {
+  return guarded([&] {
+    auto& s = S(h);
+    if ((kind != INS_COARSE_JACOBI_CG && kind != INS_COARSE_AMG) || amg_cycles < 1 || amg_cycles > 16)
+      throw insp::InvalidArgument("invalid coarse solve settings");
+    insp::PMGConfig c = s.pmg();
+    c.coarse = kind == INS_COARSE_AMG ? insp::CoarseType::AMG : insp::CoarseType::JACOBI_CG;
+    c.amg_cycles = amg_cycles;
+    s.set_pmg(c);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_coarse_matrix(ins_poisson_t h, int64_t* n, int64_t* nnz, int* rowptr, int* cols,
+                              double* vals) {
+  return guarded([&] {
+    const insp::HostCsr& A = S(h).vertex_matrix();
+    if (n) *n = A.n;
+    if (nnz) *nnz = A.nnz();
+    if (rowptr) std::copy(A.ptr.begin(), A.ptr.end(), rowptr);
+    if (cols) std::copy(A.col.begin(), A.col.end(), cols);
+    if (vals) std::copy(A.val.begin(), A.val.end(), vals);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_amg_levels(ins_poisson_t h, int* n_levels, int* sizes) {
+  return guarded([&] {
+    const std::vector<int> z = S(h).amg_level_sizes();
+    if (!n_levels) throw insp::InvalidArgument("null output");
+    *n_levels = (int)z.size();
+    if (sizes)
+      for (size_t i = 0; i < z.size() && i < 16; ++i) sizes[i] = z[i];
     return INS_OK;
   });
 }

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 
@@ -48,6 +49,42 @@ template <>
 PoissonSolver::Work<double>& PoissonSolver::Level::work<double>() { return w64; }
 template <>
 PoissonSolver::Work<float>& PoissonSolver::Level::work<float>() { return w32; }
+
+// Device copy of one AMG level in one precision.
+template <typename T>
+struct AmgLevelBufs {
+  DBuf<T> Aval, Pval, Rval, inv;   // operator, transfers, inverse diagonal
+  DBuf<T> x, b, r, d;              // iterate, rhs, residual / A x, Chebyshev direction
+  DBuf<T> b0, acc;                 // vertex level only: rhs and sum over repeated cycles
+};
+
+struct PoissonSolver::Amg {
+  // vertex space
+  int nv = 0;
+  DBuf<int> vptr, vlist;           // vertex -> SoA entries of the order-1 nodes at it
+  DBuf<int> cv;                    // order-1 SoA entry -> vertex
+  HostCsr Av;                      // vertex operator (host copy)
+  // hierarchy
+  struct Lev {
+    int n = 0, m = 0;              // rows, rows of the next level
+    DBuf<int> Aptr, Acol, Pptr, Pcol, Rptr, Rcol;
+    AmgLevelBufs<double> f64;
+    AmgLevelBufs<float> f32;
+    double lmax = 1.0;
+    template <typename T>
+    AmgLevelBufs<T>& bufs();
+  };
+  std::vector<std::unique_ptr<Lev>> lev;
+  int nc = 0;
+  DBuf<double> cinv64;
+  DBuf<float> cinv32;
+  int cheb_order = 2;
+  double cheb_ratio = 20.0;        // lower Chebyshev bound = upper / ratio
+};
+template <>
+AmgLevelBufs<double>& PoissonSolver::Amg::Lev::bufs<double>() { return f64; }
+template <>
+AmgLevelBufs<float>& PoissonSolver::Amg::Lev::bufs<float>() { return f32; }
 
 PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
     : mesh_(std::move(mesh)), N_(order), stream_(stream) {
@@ -319,7 +356,9 @@ void PoissonSolver::estimate_eigenvalues() {
   // Power iteration on D^-1 A with a Rayleigh quotient (the reference's
   // set_rand_vec / vec_norm / vec_normalise / rayleigh_quotient kernels).
   RedScratch rs{partial_.p};
-  for (size_t l = 0; l + 1 < levels_.size(); ++l) {
+  // With the AMG coarse solve the coarsest order is smoothed as well.
+  const size_t smoothed = pmg_.coarse == CoarseType::AMG ? levels_.size() : levels_.size() - 1;
+  for (size_t l = 0; l < smoothed; ++l) {
     Level& lv = *levels_[l];
     Work<double>& w = lv.w64;
     const int Np = lv.L.Np;
@@ -367,7 +406,9 @@ void PoissonSolver::init() {
       w.tr.alloc(nt);
       if (type_ == SolverType::PMULTIGRID) {
         w.x.alloc(n); w.b.alloc(n); w.ax.alloc(n); w.d.alloc(n);
-        if (l + 1 == levels_.size()) { w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n); }
+        if (l + 1 == levels_.size() && pmg_.coarse == CoarseType::JACOBI_CG) {
+          w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n);
+        }
       }
     };
     setup(lv.w64);
@@ -376,6 +417,8 @@ void PoissonSolver::init() {
   const size_t n0 = (size_t)np() * stride_;
   b_.alloc(n0); x_.alloc(n0); r_.alloc(n0); z_.alloc(n0); zold_.alloc(n0); p_.alloc(n0); Ap_.alloc(n0);
   if (type_ == SolverType::PMULTIGRID) estimate_eigenvalues();
+  amg_.reset();
+  if (type_ == SolverType::PMULTIGRID && pmg_.coarse == CoarseType::AMG) build_coarse_space();
   INSP_HIP(hipStreamSynchronize(stream_));
   initialised_ = true;
 }
@@ -491,12 +534,199 @@ void PoissonSolver::coarse_cg(int l) {
   }
 }
 
+void PoissonSolver::build_coarse_space() {
+  const int l = (int)levels_.size() - 1;
+  Level& lv = *levels_[l];
+  if (lv.N != 1) throw InvalidArgument("the AMG coarse solve needs order 1 as the coarsest p-multigrid order");
+  auto A = std::make_unique<Amg>();
+  const int d1 = mesh_.dim + 1;
+  const int64_t E = mesh_.n_cells;
+  const int nv = (int)mesh_.n_nodes;
+  A->nv = nv;
+  // order-1 nodes are the cell vertices in cell2nodes order
+  std::vector<int> vptr((size_t)nv + 1, 0), vlist((size_t)E * d1), cv((size_t)d1 * stride_, 0);
+  for (int64_t e = 0; e < E; ++e)
+    for (int i = 0; i < d1; ++i) ++vptr[(size_t)mesh_.cell2nodes[e * d1 + i] + 1];
+  for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+  {
+    std::vector<int> pos(vptr.begin(), vptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int i = 0; i < d1; ++i) {
+        const int v = mesh_.cell2nodes[e * d1 + i];
+        vlist[pos[v]++] = (int)(i * stride_ + e);
+        cv[(size_t)i * stride_ + e] = v;
+      }
+  }
+  A->vptr.upload(vptr.data(), vptr.size());
+  A->vlist.upload(vlist.data(), vlist.size());
+  A->cv.upload(cv.data(), cv.size());
+  // assemble A_v = P_v^T A_1 P_v by probing with distance-2 colour classes
+  HostCsr g = vertex_graph(mesh_);
+  int ncol = 0;
+  const std::vector<int> colour = distance2_colouring(g, &ncol);
+  DBuf<int> gptr, gcol, dcol;
+  gptr.upload(g.ptr.data(), g.ptr.size());
+  gcol.upload(g.col.data(), g.col.size());
+  dcol.upload(colour.data(), colour.size());
+  DBuf<double> val, xv, yv, u, y;
+  val.alloc(g.col.size());
+  xv.alloc((size_t)nv);
+  yv.alloc((size_t)nv);
+  u.alloc((size_t)d1 * stride_);
+  y.alloc((size_t)d1 * stride_);
+  for (int c = 0; c < ncol; ++c) {
+    amg::probe_fill(nv, dcol.p, c, xv.p, stream_);
+    amg::vertex_prolong<double>(d1, E, stride_, A->cv.p, xv.p, u.p, false, stream_);
+    apply_level<double>(l, u.p, y.p);
+    amg::vertex_restrict<double>(nv, A->vptr.p, A->vlist.p, y.p, yv.p, stream_);
+    amg::probe_extract(nv, gptr.p, gcol.p, dcol.p, c, yv.p, val.p, stream_);
+  }
+  g.val.resize(g.col.size());
+  INSP_HIP(hipMemcpyAsync(g.val.data(), val.p, g.val.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  {  // symmetrise against rounding in the probes (the pattern is symmetric)
+    HostCsr gt = csr_transpose(g);
+    for (size_t k = 0; k < g.val.size(); ++k) g.val[k] = 0.5 * (g.val[k] + gt.val[k]);
+  }
+  A->Av = g;
+  AmgOptions opt;
+  opt.cheb_order = A->cheb_order;
+  bool singular = nullspace_;
+  AmgHierarchy H = amg_setup(std::move(g), opt, singular);
+  const bool mixed = prec_ == Precision::MIXED;
+  for (size_t k = 0; k < H.levels.size(); ++k) {
+    const AmgHostLevel& h = H.levels[k];
+    auto L = std::make_unique<Amg::Lev>();
+    L->n = h.A.n;
+    L->m = h.P.m;
+    L->lmax = h.lmax;
+    L->Aptr.upload(h.A.ptr.data(), h.A.ptr.size());
+    L->Acol.upload(h.A.col.data(), h.A.col.size());
+    std::vector<double> inv((size_t)h.A.n);
+    for (int i = 0; i < h.A.n; ++i) inv[i] = h.diag[i] != 0 ? 1.0 / h.diag[i] : 0.0;
+    const bool has_next = k + 1 < H.levels.size();
+    if (has_next) {
+      L->Pptr.upload(h.P.ptr.data(), h.P.ptr.size());
+      L->Pcol.upload(h.P.col.data(), h.P.col.size());
+      L->Rptr.upload(h.R.ptr.data(), h.R.ptr.size());
+      L->Rcol.upload(h.R.col.data(), h.R.col.size());
+    }
+    auto fill_bufs = [&](auto& B, auto tag) {
+      using T = decltype(tag);
+      std::vector<T> a(h.A.val.begin(), h.A.val.end()), iv(inv.begin(), inv.end());
+      B.Aval.upload(a.data(), a.size());
+      B.inv.upload(iv.data(), iv.size());
+      if (has_next) {
+        std::vector<T> p(h.P.val.begin(), h.P.val.end()), r(h.R.val.begin(), h.R.val.end());
+        B.Pval.upload(p.data(), p.size());
+        B.Rval.upload(r.data(), r.size());
+      }
+      B.x.alloc((size_t)std::max(h.A.n, 1));
+      B.b.alloc((size_t)std::max(h.A.n, 1));
+      B.r.alloc((size_t)std::max(h.A.n, 1));
+      B.d.alloc((size_t)std::max(h.A.n, 1));
+      if (k == 0 && pmg_.amg_cycles > 1) {
+        B.b0.alloc((size_t)std::max(h.A.n, 1));
+        B.acc.alloc((size_t)std::max(h.A.n, 1));
+      }
+    };
+    fill_bufs(L->f64, double{});
+    if (mixed) fill_bufs(L->f32, float{});
+    A->lev.push_back(std::move(L));
+  }
+  A->nc = H.n_coarse;
+  A->cinv64.upload(H.coarse_inv.data(), H.coarse_inv.size());
+  if (mixed) {
+    std::vector<float> ci(H.coarse_inv.begin(), H.coarse_inv.end());
+    A->cinv32.upload(ci.data(), ci.size());
+  }
+  amg_ = std::move(A);
+}
+
+template <typename T>
+void PoissonSolver::amg_vcycle(int k) {
+  Amg& A = *amg_;
+  Amg::Lev& L = *A.lev[k];
+  AmgLevelBufs<T>& B = L.bufs<T>();
+  const DevCsr<T> Am{L.n, L.n, L.Aptr.p, L.Acol.p, B.Aval.p};
+  if (k + 1 == (int)A.lev.size()) {
+    const T* M = std::is_same<T, float>::value ? (const T*)(const void*)A.cinv32.p
+                                               : (const T*)(const void*)A.cinv64.p;
+    amg::dense_matvec<T>(A.nc, M, B.b.p, B.x.p, stream_);
+    return;
+  }
+  const double hi = 1.1 * L.lmax, lo = hi / A.cheb_ratio;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  auto smooth = [&](bool zero_init) {
+    double rho_old = 1.0 / sigma;
+    if (zero_init) {
+      cheb_step<T>(L.n, B.b.p, nullptr, B.inv.p, B.d.p, B.x.p, (T)0, (T)(1.0 / theta), true, stream_);
+    } else {
+      amg::spmv<T>(Am, B.x.p, B.r.p, nullptr, 0, stream_);
+      cheb_step<T>(L.n, B.b.p, B.r.p, B.inv.p, B.d.p, B.x.p, (T)0, (T)(1.0 / theta), false, stream_);
+    }
+    for (int j = 2; j <= A.cheb_order; ++j) {
+      amg::spmv<T>(Am, B.x.p, B.r.p, nullptr, 0, stream_);
+      const double rho = 1.0 / (2.0 * sigma - rho_old);
+      cheb_step<T>(L.n, B.b.p, B.r.p, B.inv.p, B.d.p, B.x.p, (T)(rho * rho_old), (T)(2.0 * rho / delta),
+                   false, stream_);
+      rho_old = rho;
+    }
+  };
+  smooth(true);
+  amg::spmv<T>(Am, B.x.p, B.r.p, B.b.p, 2, stream_);  // r = b - A x
+  Amg::Lev& C = *A.lev[k + 1];
+  AmgLevelBufs<T>& CB = C.bufs<T>();
+  const DevCsr<T> R{L.m, L.n, L.Rptr.p, L.Rcol.p, B.Rval.p};
+  amg::spmv<T>(R, B.r.p, CB.b.p, nullptr, 0, stream_);
+  amg_vcycle<T>(k + 1);
+  const DevCsr<T> P{L.n, L.m, L.Pptr.p, L.Pcol.p, B.Pval.p};
+  amg::spmv<T>(P, CB.x.p, B.x.p, nullptr, 1, stream_);  // x += P x_c
+  smooth(false);
+}
+
+template <typename T>
+void PoissonSolver::coarse_amg(int l) {
+  Level& lv = *levels_[l];
+  Work<T>& w = lv.work<T>();
+  Amg& A = *amg_;
+  const int64_t n = (int64_t)lv.L.Np * stride_;
+  const int pre = l < (int)pmg_.pre_it.size() ? pmg_.pre_it[l] : 1;
+  const int post = l < (int)pmg_.post_it.size() ? pmg_.post_it[l] : 1;
+  if (pre <= 0) fill<T>(n, w.x.p, (T)0, stream_);
+  for (int s = 0; s < pre; ++s) chebyshev<T>(l, w.b.p, w.x.p, s == 0);
+  apply_level<T>(l, w.x.p, w.ax.p);
+  ++smoother_applies_;
+  axpby<T>(n, (T)1, w.b.p, (T)-1, w.ax.p, stream_);  // ax = b - A x
+  Amg::Lev& V = *A.lev[0];
+  AmgLevelBufs<T>& VB = V.bufs<T>();
+  // b_v = P_v^T (b - A x); then AMG cycles on A_v x_v = b_v
+  amg::vertex_restrict<T>(A.nv, A.vptr.p, A.vlist.p, w.ax.p, VB.b.p, stream_);
+  amg_vcycle<T>(0);
+  if (pmg_.amg_cycles > 1) {
+    // further cycles on the vertex residual: x_v += V(b_v - A_v x_v)
+    const DevCsr<T> Am{V.n, V.n, V.Aptr.p, V.Acol.p, VB.Aval.p};
+    copy<T>(V.n, VB.b.p, VB.b0.p, stream_);
+    copy<T>(V.n, VB.x.p, VB.acc.p, stream_);
+    for (int c = 1; c < pmg_.amg_cycles; ++c) {
+      amg::spmv<T>(Am, VB.acc.p, VB.b.p, VB.b0.p, 2, stream_);
+      amg_vcycle<T>(0);
+      axpby<T>(V.n, (T)1, VB.x.p, (T)1, VB.acc.p, stream_);
+    }
+    copy<T>(V.n, VB.acc.p, VB.x.p, stream_);
+  }
+  amg::vertex_prolong<T>(lv.L.Np, mesh_.n_cells, stride_, A.cv.p, VB.x.p, w.x.p, true, stream_);
+  for (int s = 0; s < post; ++s) chebyshev<T>(l, w.b.p, w.x.p, false);
+}
+
 template <typename T>
 void PoissonSolver::vcycle(int l) {
   Level& lv = *levels_[l];
   Work<T>& w = lv.work<T>();
   if (l + 1 == (int)levels_.size()) {
-    coarse_cg<T>(l);
+    if (amg_) coarse_amg<T>(l);
+    else coarse_cg<T>(l);
     return;
   }
   const int64_t n = (int64_t)lv.L.Np * stride_;
@@ -706,6 +936,18 @@ void PoissonSolver::mass(const double* in, double* out, Layout layout) {
 }
 
 void PoissonSolver::node_coords(double* xyz) const { mesh_node_coords(mesh_, fine_re_, xyz); }
+
+const HostCsr& PoissonSolver::vertex_matrix() const {
+  if (!amg_) throw InvalidArgument("no vertex coarse space: configure the AMG coarse solve and init");
+  return amg_->Av;
+}
+
+std::vector<int> PoissonSolver::amg_level_sizes() const {
+  std::vector<int> s;
+  if (amg_)
+    for (auto& L : amg_->lev) s.push_back(L->n);
+  return s;
+}
 
 double PoissonSolver::level_lambda_max(int l) const {
   if (l < 0 || l >= (int)levels_.size()) throw InvalidArgument("level out of range");

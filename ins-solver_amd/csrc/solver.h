@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "amg.h"
 #include "dg_layout.h"
 #include "kernels.h"
 #include "mesh.h"
@@ -79,6 +80,10 @@ struct DBuf {
 void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<int>& tab);
 
 enum class SolverType { JACOBI_PCG = 1, PMULTIGRID = 4 };
+// Solve on the coarsest p-multigrid order: Jacobi-CG there, or (AMG) Chebyshev
+// smoothing there and a correction from the continuous vertex space solved by
+// smoothed-aggregation AMG.
+enum class CoarseType { JACOBI_CG = 0, AMG = 1 };
 enum class Precision { FP64 = 0, MIXED = 1 };
 enum class Layout { HOST_AOS = 0, DEVICE_SOA = 1, DEVICE_AOS = 2 };
 
@@ -89,6 +94,8 @@ struct PMGConfig {
   double cheb_min_factor = 0.1;   // lower Chebyshev bound = factor * upper
   int max_iter_eig = 15;          // max_iter_eig_approx
   int coarse_iters = 40;          // Jacobi-CG iterations on the coarsest order
+  CoarseType coarse = CoarseType::JACOBI_CG;
+  int amg_cycles = 1;             // AMG V-cycles per coarse correction
 };
 
 struct SolveStats {
@@ -131,6 +138,7 @@ class PoissonSolver {
   void set_solver(SolverType t) { type_ = t; initialised_ = false; }
   void set_precision(Precision p) { prec_ = p; initialised_ = false; }
   void set_pmg(const PMGConfig& c) { pmg_ = c; initialised_ = false; }
+  const PMGConfig& pmg() const { return pmg_; }
   void init();
 
   // ans is the initial guess on entry (as in the reference) and the solution on exit.
@@ -145,6 +153,10 @@ class PoissonSolver {
   void node_coords(double* xyz) const;  // host, dim x cells x Np (AoS per coordinate)
   double level_lambda_max(int l) const;
   int n_levels() const { return (int)levels_.size(); }
+  // Vertex-space operator (CSR, host copy) and AMG level sizes; empty unless the AMG
+  // coarse solve is configured and initialised.
+  const HostCsr& vertex_matrix() const;
+  std::vector<int> amg_level_sizes() const;
 
   void set_profiling(bool on) { profiling_ = on; }
   KernelTiming take_timing();
@@ -166,6 +178,12 @@ class PoissonSolver {
   void chebyshev(int l, const T* f, T* x, bool zero_init);
   template <typename T>
   void coarse_cg(int l);
+  struct Amg;
+  void build_coarse_space();
+  template <typename T>
+  void amg_vcycle(int k);
+  template <typename T>
+  void coarse_amg(int l);
   void precondition(const double* r, double* z);
   void import_vec(const double* src, double* dst, Layout layout, int rows);
   void export_vec(const double* src, double* dst, Layout layout, int rows);
@@ -188,6 +206,7 @@ class PoissonSolver {
   DBuf<double> factor_;  // fine-level factor, SoA
 
   std::vector<std::unique_ptr<Level>> levels_;
+  std::unique_ptr<Amg> amg_;
   SolverType type_ = SolverType::PMULTIGRID;
   Precision prec_ = Precision::FP64;
   PMGConfig pmg_;

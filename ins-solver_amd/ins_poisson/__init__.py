@@ -32,6 +32,8 @@ SOLVER_JACOBI = 1
 SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+COARSE_JACOBI_CG = 0
+COARSE_AMG = 1
 BC_DIRICHLET = 0
 BC_NEUMANN = 1
 
@@ -48,7 +50,8 @@ EXPORTED = [
     "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords",
     "ins_poisson_set_bc_types", "ins_poisson_set_factor", "ins_poisson_set_bcs",
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
-    "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_init", "ins_poisson_solve",
+    "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
+    "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
@@ -119,6 +122,9 @@ def _load():
         "ins_poisson_set_solver": [vp, C.c_int],
         "ins_poisson_set_precision": [vp, C.c_int],
         "ins_poisson_set_pmg": [vp, vp, C.c_int, vp, vp, vp, dbl, dbl, C.c_int, C.c_int],
+        "ins_poisson_set_coarse": [vp, C.c_int, C.c_int],
+        "ins_poisson_coarse_matrix": [vp, C.POINTER(i64), C.POINTER(i64), vp, vp, vp],
+        "ins_poisson_amg_levels": [vp, C.POINTER(C.c_int), vp],
         "ins_poisson_init": [vp],
         "ins_poisson_solve": [vp, vp, vp, C.c_int, C.POINTER(SolveStats)],
         "ins_poisson_mult": [vp, vp, vp, C.c_int],
@@ -334,6 +340,27 @@ class PoissonSolver:
         o, pre, post, ch = (arr(orders, 0), arr(pre_it, 1), arr(post_it, 1), arr(cheb_orders, 3))
         _check(lib.ins_poisson_set_pmg(self._h, _ptr(o), n, _ptr(pre), _ptr(post), _ptr(ch),
                                        eig_safety, cheb_min_factor, max_iter_eig, coarse_iters))
+
+    def set_coarse(self, kind, amg_cycles=1):
+        """COARSE_JACOBI_CG or COARSE_AMG (vertex-space smoothed-aggregation AMG)."""
+        _check(lib.ins_poisson_set_coarse(self._h, kind, amg_cycles))
+
+    def coarse_matrix(self):
+        """Vertex-space coarse operator as (rowptr, cols, vals) CSR arrays."""
+        n, nnz = C.c_int64(), C.c_int64()
+        _check(lib.ins_poisson_coarse_matrix(self._h, C.byref(n), C.byref(nnz), None, None, None))
+        ptr = np.empty(n.value + 1, dtype=np.int32)
+        col = np.empty(nnz.value, dtype=np.int32)
+        val = np.empty(nnz.value)
+        _check(lib.ins_poisson_coarse_matrix(self._h, C.byref(n), C.byref(nnz), _ptr(ptr), _ptr(col),
+                                             _ptr(val)))
+        return ptr, col, val
+
+    def amg_levels(self):
+        nl = C.c_int()
+        sizes = np.zeros(16, dtype=np.int32)
+        _check(lib.ins_poisson_amg_levels(self._h, C.byref(nl), _ptr(sizes)))
+        return [int(v) for v in sizes[:nl.value]]
 
     def init(self):
         _check(lib.ins_poisson_init(self._h))
