@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=20, help="cube size of the CPU baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-mixed-leg", action="store_true")
+    ap.add_argument("--skip-amg", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -151,6 +152,27 @@ def mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, steps):
                      "converged": all(st["converged"] for st in stats)}
     out["speedup_mixed_over_fp64"] = out["mixed"]["gdofs"] / out["fp64"]["gdofs"]
     out["rtol"] = 1e-8
+    return out
+
+
+def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
+    """The cube with the AMG coarse solve (vertex space below order 1) instead of Jacobi-CG:
+    FP64 V-cycle at rel res 1e-10 and mixed at 1e-8; set-up time reported apart."""
+    out = {}
+    for prec, rtol in (("fp64", 1e-10), ("mixed", 1e-8)):
+        s.set_precision(ip.PRECISION_MIXED if prec == "mixed" else ip.PRECISION_FP64)
+        s.set_coarse(ip.COARSE_AMG)
+        s.set_tol_and_iter(rtol, 0.0, 500)
+        t0 = time.perf_counter()
+        s.init()
+        setup = time.perf_counter() - t0
+        wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1)
+        out[prec] = {"rtol": rtol, "gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+                     "setup_s": setup, "iterations": [st["iterations"] for st in stats],
+                     "rel_residual": max(st["rel_residual"] for st in stats),
+                     "converged": all(st["converged"] for st in stats),
+                     "amg_levels": s.amg_levels()}
+    s.set_coarse(ip.COARSE_JACOBI_CG)
     return out
 
 
@@ -400,6 +422,9 @@ def main():
     if not a.no_mixed_leg:
         log("mixed vs fp64 leg")
         out["mixed_vs_fp64"] = mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, 2)
+    if not a.skip_amg:
+        log("AMG coarse leg")
+        out["coarse_amg"] = coarse_amg_leg(ip, torch, s, b_d, x_d, dof, 2)
     del s, b_d, x_d, mesh
     torch.cuda.empty_cache()
     if not a.skip_batch:

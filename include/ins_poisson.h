@@ -52,7 +52,7 @@ enum {
 enum { INS_LAYOUT_HOST_AOS = 0, INS_LAYOUT_DEVICE_SOA = 1, INS_LAYOUT_DEVICE_AOS = 2 };
 
 /* Numbering follows LinearSolver::Solvers (src/solvers/2d/ins_solver_base.cpp:1294-1309). */
-enum { INS_SOLVER_JACOBI = 1, INS_SOLVER_PMULTIGRID = 4 };
+enum { INS_SOLVER_JACOBI = 1, INS_SOLVER_BLOCK_JACOBI = 2, INS_SOLVER_PMULTIGRID = 4 };
 
 enum { INS_PRECISION_FP64 = 0, INS_PRECISION_MIXED = 1 };
 
@@ -164,6 +164,10 @@ int ins_poisson_mult(ins_poisson_t h, const double* in, double* out, int layout)
 int ins_poisson_mult_fp32(ins_poisson_t h, const double* in, double* out, int layout);
 /* Diagonal used by Jacobi (FactorViscousMatrix2D::calc_diag, factor_viscous_matrix.cpp:211-256). */
 int ins_poisson_diag(ins_poisson_t h, double* diag, int layout);
+/* Inverses of the per-cell Np x Np diagonal blocks used by INS_SOLVER_BLOCK_JACOBI
+ * (FactorViscousMatrix2D::calc_inv_block_diag, factor_viscous_matrix.cpp:258-326), host
+ * cells x Np x Np row-major (layout INS_LAYOUT_HOST_AOS only). Initialises when needed. */
+int ins_poisson_inv_block_diag(ins_poisson_t h, double* out, int layout);
 /* rhs += boundary terms (vmf_2d_apply_bc.h / fvmf_2d_apply_bc.h), in place. */
 int ins_poisson_apply_bc(ins_poisson_t h, double* rhs, int layout);
 /* out = J M in per cell (DGMesh::mass). */

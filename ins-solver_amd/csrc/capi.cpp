@@ -319,7 +319,7 @@ int ins_poisson_set_tol_and_iter(ins_poisson_t h, double rtol, double atol, int 
 
 int ins_poisson_set_solver(ins_poisson_t h, int type) {
   return guarded([&] {
-    if (type != INS_SOLVER_JACOBI && type != INS_SOLVER_PMULTIGRID)
+    if (type != INS_SOLVER_JACOBI && type != INS_SOLVER_BLOCK_JACOBI && type != INS_SOLVER_PMULTIGRID)
       throw insp::InvalidArgument("unsupported solver type");
     S(h).set_solver(static_cast<insp::SolverType>(type));
     return INS_OK;
@@ -358,6 +358,14 @@ int ins_poisson_set_pmg(ins_poisson_t h, const int* orders, int n_levels, const 
     c.coarse = S(h).pmg().coarse;  // the coarse solve kind is set by ins_poisson_set_coarse
     c.amg_cycles = S(h).pmg().amg_cycles;
     S(h).set_pmg(c);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_inv_block_diag(ins_poisson_t h, double* out, int layout) {
+  return guarded([&] {
+    if (!out || layout != INS_LAYOUT_HOST_AOS) throw insp::InvalidArgument("host output required");
+    S(h).inv_block_diag(out);
     return INS_OK;
   });
 }

@@ -54,4 +54,14 @@ template <typename T>
 void launch_elem_matvec(int dim, int rows_order, int cols_order, int64_t E, int64_t stride,
                         const T* A, const T* x, T* y, bool add, hipStream_t s);
 
+// Block-Jacobi (block_kernels.hip): blocks SoA over cells, (i, j) of cell e at
+// blk[(i*rows + j)*stride + e].
+void block_probe_unit(int rows, int64_t E, int64_t stride, const int* colour, int c, int j, double* u,
+                      hipStream_t s);
+void block_extract_column(int rows, int64_t E, int64_t stride, const int* colour, int c, int j,
+                          const double* y, double* blk, hipStream_t s);
+void block_invert(int rows, int64_t E, int64_t stride, double* blk, hipStream_t s);
+void block_apply(int rows, int64_t E, int64_t stride, const double* blk, const double* r, double* z,
+                 hipStream_t s);
+
 }  // namespace insp

@@ -419,6 +419,8 @@ void PoissonSolver::init() {
   if (type_ == SolverType::PMULTIGRID) estimate_eigenvalues();
   amg_.reset();
   if (type_ == SolverType::PMULTIGRID && pmg_.coarse == CoarseType::AMG) build_coarse_space();
+  if (type_ == SolverType::BLOCK_JACOBI_PCG) build_block_jacobi();
+  else blk_.release();
   INSP_HIP(hipStreamSynchronize(stream_));
   initialised_ = true;
 }
@@ -532,6 +534,53 @@ void PoissonSolver::coarse_cg(int l) {
     cg_update_p<T>(n, w.p.p, w.z.p, scal_.p, b, a, stream_);
     std::swap(a, b);
   }
+}
+
+void PoissonSolver::build_block_jacobi() {
+  // Greedy colouring of the face-neighbour graph: no two neighbours share a colour, so a
+  // unit vector on every cell of one colour returns each of those cells' own block column.
+  const int64_t E = mesh_.n_cells;
+  const int Nf = mesh_.dim + 1, Np = np();
+  std::vector<int> colour((size_t)E, -1);
+  int ncol = 0;
+  for (int64_t e = 0; e < E; ++e) {
+    unsigned used = 0;
+    for (int f = 0; f < Nf; ++f) {
+      const int n = mesh_.nbr[(size_t)e * Nf + f];
+      if (n >= 0 && colour[n] >= 0) used |= 1u << colour[n];
+    }
+    int c = 0;
+    while (used & (1u << c)) ++c;
+    colour[e] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  DBuf<int> dcol;
+  dcol.upload(colour.data(), colour.size());
+  blk_.alloc((size_t)Np * Np * stride_);
+  DBuf<double> u, y;
+  u.alloc((size_t)Np * stride_);
+  y.alloc((size_t)Np * stride_);
+  for (int c = 0; c < ncol; ++c)
+    for (int j = 0; j < Np; ++j) {
+      block_probe_unit(Np, E, stride_, dcol.p, c, j, u.p, stream_);
+      apply_level<double>(0, u.p, y.p);
+      block_extract_column(Np, E, stride_, dcol.p, c, j, y.p, blk_.p, stream_);
+    }
+  block_invert(Np, E, stride_, blk_.p, stream_);
+  INSP_HIP(hipStreamSynchronize(stream_));  // u, y, dcol are released on return
+}
+
+void PoissonSolver::inv_block_diag(double* out) {
+  if (!initialised_) init();
+  if (type_ != SolverType::BLOCK_JACOBI_PCG || blk_.empty())
+    throw InvalidArgument("inverse blocks exist only for the block-Jacobi solver type");
+  const int Np = np();
+  const int64_t E = mesh_.n_cells;
+  std::vector<double> h((size_t)Np * Np * stride_);
+  INSP_HIP(hipMemcpyAsync(h.data(), blk_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  for (int64_t e = 0; e < E; ++e)
+    for (int k = 0; k < Np * Np; ++k) out[(size_t)e * Np * Np + k] = h[(size_t)k * stride_ + e];
 }
 
 void PoissonSolver::build_coarse_space() {
@@ -752,6 +801,8 @@ void PoissonSolver::precondition(const double* r, double* z) {
   Level& l0 = *levels_[0];
   if (type_ == SolverType::JACOBI_PCG) {
     jacobi<double>(n, r, l0.w64.inv.p, z, stream_);
+  } else if (type_ == SolverType::BLOCK_JACOBI_PCG) {
+    block_apply(np(), mesh_.n_cells, stride_, blk_.p, r, z, stream_);
   } else if (prec_ == Precision::FP64) {
     copy<double>(n, r, l0.w64.b.p, stream_);
     vcycle<double>(0);

@@ -79,7 +79,7 @@ struct DBuf {
 // op_layout(); values in double.
 void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<int>& tab);
 
-enum class SolverType { JACOBI_PCG = 1, PMULTIGRID = 4 };
+enum class SolverType { JACOBI_PCG = 1, BLOCK_JACOBI_PCG = 2, PMULTIGRID = 4 };
 // Solve on the coarsest p-multigrid order: Jacobi-CG there, or (AMG) Chebyshev
 // smoothing there and a correction from the continuous vertex space solved by
 // smoothed-aggregation AMG.
@@ -133,7 +133,10 @@ class PoissonSolver {
   void set_bc_types(const int* types, int64_t n);
   void set_factor(const double* k, Layout layout);  // nullptr = constant coefficient 1
   void set_bcs(const double* bc, Layout layout);    // bfaces x Nfp, nullptr = none
-  void set_nullspace(bool ns) { nullspace_ = ns; }
+  void set_nullspace(bool ns) {
+    if (ns != nullspace_ && pmg_.coarse == CoarseType::AMG) initialised_ = false;  // coarsest inverse
+    nullspace_ = ns;
+  }
   void set_tol_and_iter(double rtol, double atol, int max_iter);
   void set_solver(SolverType t) { type_ = t; initialised_ = false; }
   void set_precision(Precision p) { prec_ = p; initialised_ = false; }
@@ -148,6 +151,8 @@ class PoissonSolver {
   // out = A in on the finest order (FP64 or FP32 kernels).
   void mult(const double* in, double* out, Layout layout, bool fp32 = false);
   void diag(double* out, Layout layout);
+  // Inverses of the per-cell diagonal blocks (block-Jacobi solver type), host cells x Np x Np.
+  void inv_block_diag(double* out);
   void apply_bc(double* rhs, Layout layout);
   void mass(const double* in, double* out, Layout layout);
   void node_coords(double* xyz) const;  // host, dim x cells x Np (AoS per coordinate)
@@ -180,6 +185,7 @@ class PoissonSolver {
   void coarse_cg(int l);
   struct Amg;
   void build_coarse_space();
+  void build_block_jacobi();
   template <typename T>
   void amg_vcycle(int k);
   template <typename T>
@@ -207,6 +213,7 @@ class PoissonSolver {
 
   std::vector<std::unique_ptr<Level>> levels_;
   std::unique_ptr<Amg> amg_;
+  DBuf<double> blk_;  // inverse diagonal blocks (block-Jacobi)
   SolverType type_ = SolverType::PMULTIGRID;
   Precision prec_ = Precision::FP64;
   PMGConfig pmg_;

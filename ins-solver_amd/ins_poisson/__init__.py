@@ -29,6 +29,7 @@ LAYOUT_HOST_AOS = 0
 LAYOUT_DEVICE_SOA = 1
 LAYOUT_DEVICE_AOS = 2
 SOLVER_JACOBI = 1
+SOLVER_BLOCK_JACOBI = 2
 SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -52,7 +53,7 @@ EXPORTED = [
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
     "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
     "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
-    "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_apply_bc",
+    "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_create", "ins_batch_destroy",
@@ -130,6 +131,7 @@ def _load():
         "ins_poisson_mult": [vp, vp, vp, C.c_int],
         "ins_poisson_mult_fp32": [vp, vp, vp, C.c_int],
         "ins_poisson_diag": [vp, vp, C.c_int],
+        "ins_poisson_inv_block_diag": [vp, vp, C.c_int],
         "ins_poisson_apply_bc": [vp, vp, C.c_int],
         "ins_poisson_mass": [vp, vp, vp, C.c_int],
         "ins_poisson_lambda_max": [vp, C.c_int, dp],
@@ -402,6 +404,12 @@ class PoissonSolver:
     def diag(self):
         out = np.empty((self.n_cells, self.Np))
         _check(lib.ins_poisson_diag(self._h, _ptr(out), LAYOUT_HOST_AOS))
+        return out
+
+    def inv_block_diag(self):
+        """Inverse diagonal blocks of the block-Jacobi solver, (cells, Np, Np)."""
+        out = np.empty((self.n_cells, self.Np, self.Np))
+        _check(lib.ins_poisson_inv_block_diag(self._h, _ptr(out), LAYOUT_HOST_AOS))
         return out
 
     def apply_bc(self, rhs):

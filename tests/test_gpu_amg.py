@@ -1,0 +1,149 @@
+"""GPU parity of the AMG coarse solve (vertex space below order 1), through the C ABI.
+
+  - The assembled vertex operator equals P_v^T A_1 P_v formed from the oracle's dense
+    order-1 matrix (penalty of the finest order), within 1e-12 of its largest entry.
+  - p-multigrid solves with the AMG coarse solve reach the oracle's FP64 solution within
+    1e-10 relative L2 (FP64) / 1e-8 (mixed), including a variable coefficient with pure
+    Neumann boundaries and the mean-zero constraint.
+  - Outer iteration counts stay flat under refinement and undercut the Jacobi-CG coarse solve.
+"""
+import numpy as np
+import pytest
+
+from helpers import density_field, manufactured_sin
+
+pytestmark = pytest.mark.gpu
+
+
+def relL2(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+def vertex_prolongation(c2n, nv):
+    """P_v: vertex values -> order-1 DG nodes (AoS cell*(d+1)+i); the order-1 nodes are
+    the cell vertices in cell2nodes order."""
+    E, d1 = c2n.shape
+    P = np.zeros((E * d1, nv))
+    P[np.arange(E * d1), c2n.ravel()] = 1.0
+    return P
+
+
+@pytest.mark.parametrize("dim,N,bc,factor", [(2, 2, "dirichlet", False), (2, 3, "mixed", True),
+                                             (3, 2, "dirichlet", True), (3, 3, "neumann", False)])
+def test_vertex_operator_is_galerkin(ip, oracle, dim, N, bc, factor):
+    mesh = ip.Mesh.square(4, jitter=0.1, seed=3) if dim == 2 else ip.Mesh.cube(2, 0.1, 3)
+    coords, c2n = mesh.arrays()
+    s = ip.PoissonSolver(mesh, N)
+    bt = np.zeros(s.n_bfaces, dtype=np.int32)
+    if bc == "neumann":
+        bt[:] = 1
+    elif bc == "mixed":
+        bt[1::3] = 1
+    s.set_bc_types(bt)
+    o1 = oracle.Oracle(coords, c2n, 1)
+    o1.set_penalty_order(N)
+    o1.set_bc_types(bt)
+    if factor:
+        k = 1.0 / density_field(s.node_coords(), 0.5, rho_in=10.0)
+        s.set_factor(k)
+        o1.set_factor(k @ ip.ref_interp(dim, 1, N).T)
+    s.set_pmg(list(range(N, 0, -1)))
+    s.set_coarse(ip.COARSE_AMG)
+    s.set_nullspace(bc == "neumann")
+    s.init()
+    ptr, col, val = s.coarse_matrix()
+    nv = coords.shape[0]
+    Av = np.zeros((nv, nv))
+    for r in range(nv):
+        Av[r, col[ptr[r]:ptr[r + 1]]] = val[ptr[r]:ptr[r + 1]]
+    P = vertex_prolongation(c2n, nv)
+    ref = P.T @ o1.dense() @ P
+    assert np.abs(Av - ref).max() < 1e-12 * np.abs(ref).max()
+    assert s.amg_levels()[0] == nv
+
+
+def amg_solver(ip, mesh, N, coarse, precision="fp64"):
+    s = ip.PoissonSolver(mesh, N)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    s.set_pmg(list(range(N, 0, -1)), coarse_iters=40)
+    s.set_coarse(coarse)
+    if precision == "mixed":
+        s.set_precision(ip.PRECISION_MIXED)
+    s.set_tol_and_iter(1e-12, 0.0, 500)
+    return s
+
+
+@pytest.mark.parametrize("dim,N", [(2, 3), (3, 2), (3, 3)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_amg_pmg_solve_matches_oracle(ip, oracle, dim, N, precision):
+    mesh = ip.Mesh.square(10, jitter=0.1, seed=5) if dim == 2 else ip.Mesh.cube(4, 0.1, 5)
+    s = amg_solver(ip, mesh, N, ip.COARSE_AMG, precision)
+    u, f = manufactured_sin(s.node_coords())
+    b = s.mass(f)
+    x = np.zeros_like(b)
+    assert s.solve(b, x)
+    o = oracle.Oracle(*mesh.arrays(), N)
+    xo, rc, _, _ = o.solve_jacobi(b, rtol=1e-13, max_iter=100000)
+    assert rc == 0
+    assert relL2(x, xo) < (1e-10 if precision == "fp64" else 1e-8), relL2(x, xo)
+
+
+def test_amg_iterations_flat_under_refinement(ip):
+    its = {}
+    for n in (4, 8, 12):
+        mesh = ip.Mesh.cube(n, 0.1, 9)
+        for coarse in (ip.COARSE_AMG, ip.COARSE_JACOBI_CG):
+            s = amg_solver(ip, mesh, 3, coarse)
+            s.set_tol_and_iter(1e-10, 0.0, 500)
+            _, f = manufactured_sin(s.node_coords())
+            b = s.mass(f)
+            x = np.zeros_like(b)
+            assert s.solve(b, x)
+            its[(n, coarse)] = s.last_stats["iterations"]
+    amg = [its[(n, ip.COARSE_AMG)] for n in (4, 8, 12)]
+    jcg = [its[(n, ip.COARSE_JACOBI_CG)] for n in (4, 8, 12)]
+    assert max(amg) - min(amg) <= 4, its
+    assert amg[-1] < jcg[-1], its
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_amg_variable_coefficient_neumann(ip, oracle, precision):
+    n, N = 4, 3
+    mesh = ip.Mesh.cube(n, 0.1, 21)
+    s = ip.PoissonSolver(mesh, N)
+    o = oracle.Oracle(*mesh.arrays(), N)
+    s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
+    o.set_bc_types(np.ones(o.n_bfaces, dtype=np.int32))
+    k = 1.0 / density_field(s.node_coords(), 1.0 / n)
+    s.set_factor(k)
+    o.set_factor(k)
+    s.set_nullspace(True)
+    s.set_pmg([3, 2, 1])
+    s.set_coarse(ip.COARSE_AMG)
+    if precision == "mixed":
+        s.set_precision(ip.PRECISION_MIXED)
+    s.set_tol_and_iter(1e-10, 0.0, 400)
+    f = np.random.default_rng(4).standard_normal((s.n_cells, s.Np))
+    b = s.mass(f)
+    b -= b.mean()
+    x = np.zeros_like(b)
+    assert s.solve(b, x)
+    xo, rc, _, _ = o.solve_jacobi(b, rtol=1e-12, max_iter=50000, nullspace=True)
+    assert rc == 0
+    assert relL2(x, xo) < (1e-8 if precision == "mixed" else 1e-9)
+
+
+def test_amg_repeated_cycles(ip):
+    mesh = ip.Mesh.cube(6, 0.1, 2)
+    res = {}
+    for cyc in (1, 3):
+        s = amg_solver(ip, mesh, 3, ip.COARSE_AMG)
+        s.set_coarse(ip.COARSE_AMG, amg_cycles=cyc)
+        s.set_tol_and_iter(1e-10, 0.0, 500)
+        _, f = manufactured_sin(s.node_coords())
+        b = s.mass(f)
+        x = np.zeros_like(b)
+        assert s.solve(b, x)
+        res[cyc] = (x, s.last_stats["iterations"])
+    assert res[3][1] <= res[1][1]
+    assert relL2(res[3][0], res[1][0]) < 1e-8
