@@ -172,6 +172,19 @@ int ins_poisson_inv_block_diag(ins_poisson_t h, double* out, int layout);
 int ins_poisson_apply_bc(ins_poisson_t h, double* rhs, int layout);
 /* out = J M in per cell (DGMesh::mass). */
 int ins_poisson_mass(ins_poisson_t h, const double* in, double* out, int layout);
+/* The caller's pressure rhs, fused (ins_pressure_rhs_2d.h:1-12, then op2_gemv(LIFT, dPdNOld) and
+ * mesh->mass, src/solvers/2d/ins_solver.cpp:310-319): dPdN_old <- b0 dPdN + b1 dPdN_old and
+ * div <- J M (-div/dt + LIFT dPdN_old) in place, ready for ins_poisson_solve. dPdN, dPdN_old are
+ * cells x (Nfaces*Nfp) in face-mask order; all three in `layout` (SoA shares the stride). */
+int ins_poisson_pressure_rhs(ins_poisson_t h, double b0, double b1, double dt, const double* dPdN,
+                             double* dPdN_old, double* div, int layout);
+/* Initial-guess extrapolation: add_to_pr_history (src/solvers/2d/ins_solver_base.cpp:1141-1155)
+ * stores (t, x); initial_guess_extrapolation (src/solvers/3d/ins_solver.cpp:342-343) writes the
+ * Lagrange extrapolation through the stored pairs at time t into ans (unchanged when the history
+ * is empty). The history keeps the latest n (1..8, default 3) solutions on the device. */
+int ins_poisson_set_history_size(ins_poisson_t h, int n);
+int ins_poisson_history_push(ins_poisson_t h, double t, const double* x, int layout);
+int ins_poisson_initial_guess(ins_poisson_t h, double t, double* ans, int layout);
 /* Power-iteration estimate of lambda_max(D^-1 A) on p-multigrid level `level`. */
 int ins_poisson_lambda_max(ins_poisson_t h, int level, double* out);
 

@@ -370,6 +370,35 @@ int ins_poisson_inv_block_diag(ins_poisson_t h, double* out, int layout) {
   });
 }
 
+int ins_poisson_pressure_rhs(ins_poisson_t h, double b0, double b1, double dt, const double* dPdN,
+                             double* dPdN_old, double* div, int layout) {
+  return guarded([&] {
+    S(h).pressure_rhs(b0, b1, dt, dPdN, dPdN_old, div, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_history_size(ins_poisson_t h, int n) {
+  return guarded([&] {
+    S(h).set_history_size(n);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_history_push(ins_poisson_t h, double t, const double* x, int layout) {
+  return guarded([&] {
+    S(h).history_push(t, x, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_initial_guess(ins_poisson_t h, double t, double* ans, int layout) {
+  return guarded([&] {
+    S(h).extrapolate(t, ans, to_layout(layout));
+    return INS_OK;
+  });
+}
+
 int ins_poisson_set_coarse(ins_poisson_t h, int kind, int amg_cycles) {
   return guarded([&] {
     auto& s = S(h);

@@ -64,4 +64,10 @@ void block_invert(int rows, int64_t E, int64_t stride, double* blk, hipStream_t 
 void block_apply(int rows, int64_t E, int64_t stride, const double* blk, const double* r, double* z,
                  hipStream_t s);
 
+// Pressure rhs of the caller (rhs_kernels.hip): div <- J M (-div/dt) + J EMAT g with
+// g = b0 dPdN + b1 dPdNOld written back to dPdNOld; M is Np x Np, EMAT Np x (Nf*Nfp).
+void launch_pressure_rhs(int dim, int N, int64_t E, int64_t stride, const double* J, const double* M,
+                         const double* EMAT, double b0, double b1, double dt, const double* dPdN,
+                         double* dPdNOld, double* div, hipStream_t s);
+
 }  // namespace insp

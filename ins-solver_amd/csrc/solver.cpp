@@ -986,6 +986,70 @@ void PoissonSolver::mass(const double* in, double* out, Layout layout) {
   INSP_HIP(hipStreamSynchronize(stream_));
 }
 
+void PoissonSolver::pressure_rhs(double b0, double b1, double dt, const double* dPdN, double* dPdNOld,
+                                 double* div, Layout layout) {
+  if (!(dt > 0) || !std::isfinite(b0) || !std::isfinite(b1)) throw InvalidArgument("invalid rhs coefficients");
+  if (!dPdN || !dPdNOld || !div) throw InvalidArgument("null vector");
+  const int Np = np(), NF = (mesh_.dim + 1) * nfp();
+  if (Mfine_.empty()) {
+    Mfine_.upload(fine_re_.M.data(), fine_re_.M.size());
+    Efine_.upload(fine_re_.EMAT.data(), fine_re_.EMAT.size());
+  }
+  if (layout == Layout::DEVICE_SOA) {
+    launch_pressure_rhs(mesh_.dim, N_, mesh_.n_cells, stride_, J_.p, Mfine_.p, Efine_.p, b0, b1, dt, dPdN,
+                        dPdNOld, div, stream_);
+    INSP_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  DBuf<double> g0, g1, v;
+  g0.alloc((size_t)NF * stride_);
+  g1.alloc((size_t)NF * stride_);
+  v.alloc((size_t)Np * stride_);
+  import_vec(dPdN, g0.p, layout, NF);
+  import_vec(dPdNOld, g1.p, layout, NF);
+  import_vec(div, v.p, layout, Np);
+  launch_pressure_rhs(mesh_.dim, N_, mesh_.n_cells, stride_, J_.p, Mfine_.p, Efine_.p, b0, b1, dt, g0.p,
+                      g1.p, v.p, stream_);
+  export_vec(g1.p, dPdNOld, layout, NF);
+  export_vec(v.p, div, layout, Np);
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::set_history_size(int n) {
+  if (n < 1 || n > 8) throw InvalidArgument("history size must be in 1..8");
+  hist_max_ = n;
+  while ((int)hist_.size() > hist_max_) hist_.erase(hist_.begin());
+}
+
+void PoissonSolver::history_push(double t, const double* x, Layout layout) {
+  if (!std::isfinite(t)) throw InvalidArgument("invalid time");
+  for (const Snapshot& h : hist_)
+    if (h.t == t) throw InvalidArgument("history already holds this time");
+  Snapshot s;
+  s.t = t;
+  s.x.alloc((size_t)np() * stride_);
+  import_vec(x, s.x.p, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+  hist_.push_back(std::move(s));
+  while ((int)hist_.size() > hist_max_) hist_.erase(hist_.begin());
+}
+
+void PoissonSolver::extrapolate(double t, double* ans, Layout layout) {
+  if (hist_.empty()) return;  // keep the caller's guess
+  const int n = (int)hist_.size();
+  const int64_t len = (int64_t)np() * stride_;
+  DBuf<double> acc;
+  acc.alloc((size_t)len);
+  for (int i = 0; i < n; ++i) {
+    double w = 1.0;  // Lagrange basis polynomial of node i at t
+    for (int j = 0; j < n; ++j)
+      if (j != i) w *= (t - hist_[j].t) / (hist_[i].t - hist_[j].t);
+    axpby<double>(len, w, hist_[i].x.p, 1.0, acc.p, stream_);
+  }
+  export_vec(acc.p, ans, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
 void PoissonSolver::node_coords(double* xyz) const { mesh_node_coords(mesh_, fine_re_, xyz); }
 
 const HostCsr& PoissonSolver::vertex_matrix() const {

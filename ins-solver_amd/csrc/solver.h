@@ -155,6 +155,18 @@ class PoissonSolver {
   void inv_block_diag(double* out);
   void apply_bc(double* rhs, Layout layout);
   void mass(const double* in, double* out, Layout layout);
+  // The caller's pressure rhs (ins_pressure_rhs_2d.h + op2_gemv(LIFT) + mass,
+  // src/solvers/2d/ins_solver.cpp:310-319): div <- J M (-div/dt + LIFT g),
+  // g = b0 dPdN + b1 dPdNOld stored back into dPdNOld. Face data: cells x Nf*Nfp.
+  void pressure_rhs(double b0, double b1, double dt, const double* dPdN, double* dPdNOld, double* div,
+                    Layout layout);
+  // Initial guess from earlier solutions (add_to_pr_history / initial_guess_extrapolation,
+  // src/solvers/2d/ins_solver_base.cpp:1141-1155, src/solvers/3d/ins_solver.cpp:342-350):
+  // Lagrange extrapolation in time through the stored (t, x) pairs.
+  void set_history_size(int n);
+  void history_push(double t, const double* x, Layout layout);
+  int history_count() const { return (int)hist_.size(); }
+  void extrapolate(double t, double* ans, Layout layout);
   void node_coords(double* xyz) const;  // host, dim x cells x Np (AoS per coordinate)
   double level_lambda_max(int l) const;
   int n_levels() const { return (int)levels_.size(); }
@@ -214,6 +226,13 @@ class PoissonSolver {
   std::vector<std::unique_ptr<Level>> levels_;
   std::unique_ptr<Amg> amg_;
   DBuf<double> blk_;  // inverse diagonal blocks (block-Jacobi)
+  DBuf<double> Mfine_, Efine_;  // reference mass and EMAT of the solver order (pressure rhs)
+  struct Snapshot {
+    double t;
+    DBuf<double> x;
+  };
+  std::vector<Snapshot> hist_;
+  int hist_max_ = 3;
   SolverType type_ = SolverType::PMULTIGRID;
   Precision prec_ = Precision::FP64;
   PMGConfig pmg_;

@@ -54,7 +54,8 @@ EXPORTED = [
     "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
     "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
-    "ins_poisson_mass", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
+    "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
+    "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
@@ -134,6 +135,10 @@ def _load():
         "ins_poisson_inv_block_diag": [vp, vp, C.c_int],
         "ins_poisson_apply_bc": [vp, vp, C.c_int],
         "ins_poisson_mass": [vp, vp, vp, C.c_int],
+        "ins_poisson_pressure_rhs": [vp, dbl, dbl, dbl, vp, vp, vp, C.c_int],
+        "ins_poisson_set_history_size": [vp, C.c_int],
+        "ins_poisson_history_push": [vp, dbl, vp, C.c_int],
+        "ins_poisson_initial_guess": [vp, dbl, vp, C.c_int],
         "ins_poisson_lambda_max": [vp, C.c_int, dp],
         "ins_poisson_set_profiling": [vp, C.c_int],
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
@@ -416,6 +421,25 @@ class PoissonSolver:
         r = _f64(rhs).copy()
         _check(lib.ins_poisson_apply_bc(self._h, _ptr(r), LAYOUT_HOST_AOS))
         return r
+
+    def pressure_rhs(self, b0, b1, dt, dPdN, dPdN_old, div):
+        """Host arrays: returns (rhs, dPdN_old_new); inputs are left unchanged."""
+        g0, g1, v = _f64(dPdN), _f64(dPdN_old).copy(), _f64(div).copy()
+        _check(lib.ins_poisson_pressure_rhs(self._h, b0, b1, dt, _ptr(g0), _ptr(g1), _ptr(v),
+                                            LAYOUT_HOST_AOS))
+        return v, g1
+
+    def set_history_size(self, n):
+        _check(lib.ins_poisson_set_history_size(self._h, n))
+
+    def history_push(self, t, x):
+        _check(lib.ins_poisson_history_push(self._h, t, _ptr(_f64(x)), LAYOUT_HOST_AOS))
+
+    def initial_guess(self, t, ans):
+        """Writes the extrapolated guess into ans (float64, C-contiguous) in place."""
+        if not (isinstance(ans, np.ndarray) and ans.dtype == np.float64 and ans.flags.c_contiguous):
+            raise TypeError("ans must be a C-contiguous float64 array (updated in place)")
+        _check(lib.ins_poisson_initial_guess(self._h, t, _ptr(ans), LAYOUT_HOST_AOS))
 
     def lambda_max(self, level):
         v = C.c_double()
