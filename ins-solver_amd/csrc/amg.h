@@ -4,7 +4,8 @@
 // The order-1 DG space contains the continuous P1 functions; for them every
 // interior-face term of the SIPG form vanishes, so the Galerkin operator
 // A_v = P_v^T A_1 P_v (P_v: vertex value -> the DG nodes at that vertex) is the
-// P1 stiffness form with the Nitsche boundary terms of A_1. A_v is assembled
+// P1 stiffness form with the Nitsche boundary terms of A_1 (symmetric for a
+// constant coefficient; the AMG is built from its symmetric part). A_v is assembled
 // exactly on the device by probing A_1 with vertex colour classes (a distance-2
 // colouring of the vertex graph keeps the columns of one class apart), so it
 // takes the coefficient, penalty and boundary types from the operator itself.

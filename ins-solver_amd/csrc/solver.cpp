@@ -634,11 +634,14 @@ void PoissonSolver::build_coarse_space() {
   INSP_HIP(hipMemcpyAsync(g.val.data(), val.p, g.val.size() * sizeof(double), hipMemcpyDeviceToHost,
                           stream_));
   INSP_HIP(hipStreamSynchronize(stream_));
-  {  // symmetrise against rounding in the probes (the pattern is symmetric)
+  // A_v is the exact Galerkin product. With a variable coefficient the SIPG operator takes
+  // k from the owner side in its symmetric face term and is not exactly symmetric; the AMG
+  // hierarchy is built from the symmetric part (the pattern is symmetric).
+  A->Av = g;
+  {
     HostCsr gt = csr_transpose(g);
     for (size_t k = 0; k < g.val.size(); ++k) g.val[k] = 0.5 * (g.val[k] + gt.val[k]);
   }
-  A->Av = g;
   AmgOptions opt;
   opt.cheb_order = A->cheb_order;
   bool singular = nullspace_;
