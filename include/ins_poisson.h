@@ -223,6 +223,8 @@ typedef struct {
   int max_iter_eig;       /* power iterations for lambda_max (max_iter_eig_approx) */
   int coarse_iters;       /* Jacobi-CG iterations on order 1 */
   int use_graph;          /* replay one captured HIP graph per lockstep iteration */
+  int coarse;             /* INS_COARSE_AMG (default): Chebyshev on order 1 + vertex-space AMG;
+                             INS_COARSE_JACOBI_CG: coarse_iters Jacobi-CG iterations on order 1 */
 } ins_batch_config;
 
 typedef struct {
@@ -237,11 +239,13 @@ typedef struct {
   int rhs_nodal;          /* 0: rhs is mass-weighted (M f); 1: rhs is nodal f, b = J M f */
 } ins_problem;
 
-/* Defaults: MIXED, Chebyshev 3, 1 pre / 1 post, 1.1, 0.1, 15, 40, graphs on. */
+/* Defaults: MIXED, Chebyshev 3, 1 pre / 1 post, 1.1, 0.1, 15, 40, graphs on, AMG coarse. */
 void ins_batch_config_default(ins_batch_config* cfg);
-/* Device bytes one problem holds while resident (admission control). */
+/* Device bytes one problem holds while resident (admission control); n_nodes > 0 adds the
+ * estimate for the AMG coarse solve of a mesh with that many vertices. */
 int ins_batch_problem_bytes(int dim, int64_t n_cells, int order, int has_factor, int precision,
                             int64_t* bytes);
+int ins_batch_amg_bytes(int dim, int64_t n_nodes, int64_t n_cells, int precision, int64_t* bytes);
 int ins_batch_create(int dim, const ins_batch_config* cfg, ins_batch_t* out);
 void ins_batch_destroy(ins_batch_t b);
 /* Uploads n problems, sets them up (penalties, diagonals, Chebyshev bounds, in

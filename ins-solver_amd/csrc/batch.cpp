@@ -1,5 +1,7 @@
 #include "batch.h"
 
+#include "amg.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,6 +34,9 @@ struct BatchEngine::Prob {
   ProbResult result;
   bool done = false;
   std::chrono::steady_clock::time_point t0;
+  void* amg_arena = nullptr;     // vertex maps + hierarchy (AMG coarse solve)
+  size_t amg_bytes = 0;
+  std::vector<int> amg_rows;     // rows per AMG level
 };
 
 struct BatchEngine::View {
@@ -41,6 +46,12 @@ struct BatchEngine::View {
   DBuf<int> counts;
   DBuf<double> partial;
   int tile_cap = 0, prob_cap = 0;
+  DBuf<int2> ltiles;
+  DBuf<int> lcount;
+  int ltile_cap = 0;
+  int2* h_ltiles = nullptr;
+  int* h_lcount = nullptr;
+  int levels = 0;                // deepest AMG hierarchy in the view
   int2* h_tiles = nullptr;
   int* h_probs = nullptr;
   int2* h_ranges = nullptr;
@@ -52,6 +63,8 @@ struct BatchEngine::View {
     if (h_probs) (void)hipHostFree(h_probs);
     if (h_ranges) (void)hipHostFree(h_ranges);
     if (h_counts) (void)hipHostFree(h_counts);
+    if (h_ltiles) (void)hipHostFree(h_ltiles);
+    if (h_lcount) (void)hipHostFree(h_lcount);
   }
 };
 
@@ -97,8 +110,10 @@ BatchEngine::BatchEngine(const BatchConfig& cfg, hipStream_t stream) : cfg_(cfg)
 BatchEngine::~BatchEngine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (graph_) (void)hipGraphExecDestroy(graph_);
-  for (auto& p : probs_)
+  for (auto& p : probs_) {
     if (p && p->arena) (void)hipFree(p->arena);
+    if (p && p->amg_arena) (void)hipFree(p->amg_arena);
+  }
   for (void* b : table_bufs_) (void)hipFree(b);
   if (d_desc_) (void)hipFree(d_desc_);
   if (d_scal_) (void)hipFree(d_scal_);
@@ -202,6 +217,43 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   v.bv.prob_cap = v.prob_cap;
   v.bv.partial = v.partial.p;
   v.bv.dim = cfg_.dim;
+  // AMG row tiles per level
+  std::vector<int> per_level(kAmgLevels, 0);
+  int levels = 0;
+  for (int sl : slots) {
+    const Prob& P = *probs_[sl];
+    levels = std::max(levels, (int)P.amg_rows.size());
+    for (size_t k = 0; k < P.amg_rows.size(); ++k) per_level[k] += (P.amg_rows[k] + kTile - 1) / kTile;
+  }
+  const int lneed = std::max(1, *std::max_element(per_level.begin(), per_level.end()));
+  const int lcap = grow(lneed, v.ltile_cap);
+  if (lcap != v.ltile_cap || !v.ltiles.p) {
+    v.ltiles.alloc((size_t)kAmgLevels * lcap);
+    if (v.h_ltiles) (void)hipHostFree(v.h_ltiles);
+    INSP_HIP(hipHostMalloc((void**)&v.h_ltiles, sizeof(int2) * kAmgLevels * lcap, hipHostMallocDefault));
+    v.ltile_cap = lcap;
+  }
+  if (!v.lcount.p) {
+    v.lcount.alloc(kAmgLevels);
+    INSP_HIP(hipHostMalloc((void**)&v.h_lcount, sizeof(int) * kAmgLevels, hipHostMallocDefault));
+  }
+  for (int k = 0; k < kAmgLevels; ++k) {
+    int t = 0;
+    for (int sl : slots) {
+      const Prob& P = *probs_[sl];
+      if (k >= (int)P.amg_rows.size()) continue;
+      for (int r0 = 0; r0 < P.amg_rows[k]; r0 += kTile) v.h_ltiles[(size_t)k * lcap + t++] = make_int2(sl, r0);
+    }
+    v.h_lcount[k] = t;
+  }
+  v.levels = levels;
+  INSP_HIP(hipMemcpyAsync(v.ltiles.p, v.h_ltiles, sizeof(int2) * kAmgLevels * lcap, hipMemcpyHostToDevice,
+                          stream_));
+  INSP_HIP(hipMemcpyAsync(v.lcount.p, v.h_lcount, sizeof(int) * kAmgLevels, hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  v.bv.ltiles = v.ltiles.p;
+  v.bv.lcount = v.lcount.p;
+  v.bv.ltile_cap = v.ltile_cap;
 }
 
 void BatchEngine::read_scal(const std::vector<int>& slots) {
@@ -220,35 +272,66 @@ void BatchEngine::vcycle(View& v) {
   const bool f32 = std::is_same<T, float>::value;
   batch::to_t(b, f32, BV_R, BV_VB, s);
   const int K = cfg_.cheb_order;
-  auto smooth = [&](bool first_zero) {
+  // Chebyshev on order N (level 0) or on order 1 (level 1)
+  auto smooth = [&](int level, bool first_zero) {
+    const int in = level ? BV_CX : BV_VX, ax = level ? BV_CAP : BV_VAX;
     for (int k = 0; k < K; ++k) {
       if (first_zero && k == 0) {
-        batch::cheb<T>(b, 0, true, s);
+        batch::cheb<T>(b, level, 0, true, s);
       } else {
-        batch::trace<T>(b, tables_, BV_VX, BV_TRT, s);
-        batch::apply<T>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, s);
-        batch::cheb<T>(b, k, false, s);
+        if (level == 0) batch::trace<T>(b, tables_, in, BV_TRT, s);
+        batch::apply<T>(b, tables_, level, in, BV_TRT, ax, false, s);
+        batch::cheb<T>(b, level, k, false, s);
       }
     }
   };
-  for (int it = 0; it < cfg_.pre_it; ++it) smooth(it == 0);
+  for (int it = 0; it < cfg_.pre_it; ++it) smooth(0, it == 0);
   batch::trace<T>(b, tables_, BV_VX, BV_TRT, s);
   batch::apply<T>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, s);
-  batch::sub_from_b<T>(b, s);
+  batch::sub_from_b<T>(b, 0, s);
   batch::restrict_<T>(b, tables_, s);
-  batch::cg_init<T>(b, s);
-  batch::finalize(b, 1, BS_CA, s);
-  int a = BS_CA, c = BS_CB;
-  for (int it = 0; it < cfg_.coarse_iters; ++it) {
-    batch::apply<T>(b, tables_, 1, BV_CP, BV_TRT, BV_CAP, true, s);
-    batch::finalize(b, 1, BS_CPAP, s);
-    batch::cg_xr<T>(b, a, s);
-    batch::finalize(b, 1, c, s);
-    batch::cg_p<T>(b, c, a, s);
-    std::swap(a, c);
+  if (cfg_.amg) {
+    // order 1: smooth, then correct from the vertex space by one AMG V-cycle
+    smooth(1, true);
+    batch::apply<T>(b, tables_, 1, BV_CX, BV_TRT, BV_CAP, false, s);
+    batch::sub_from_b<T>(b, 1, s);
+    batch::vertex_restrict<T>(b, s);
+    const int L = v.levels, D = cfg_.amg_cheb;
+    for (int k = 0; k < L; ++k) {
+      batch::amg_dense<T>(b, k, s);
+      if (k + 1 == L) break;
+      batch::amg_cheb<T>(b, k, 0, true, s);
+      for (int j = 1; j < D; ++j) {
+        batch::amg_spmv<T>(b, k, false, s);
+        batch::amg_cheb<T>(b, k, j, false, s);
+      }
+      batch::amg_spmv<T>(b, k, true, s);
+      batch::amg_restrict<T>(b, k, s);
+    }
+    for (int k = L - 2; k >= 0; --k) {
+      batch::amg_prolong<T>(b, k, s);
+      for (int j = 0; j < D; ++j) {
+        batch::amg_spmv<T>(b, k, false, s);
+        batch::amg_cheb<T>(b, k, j, false, s);
+      }
+    }
+    batch::vertex_prolong<T>(b, s);
+    smooth(1, false);
+  } else {
+    batch::cg_init<T>(b, s);
+    batch::finalize(b, 1, BS_CA, s);
+    int a = BS_CA, c = BS_CB;
+    for (int it = 0; it < cfg_.coarse_iters; ++it) {
+      batch::apply<T>(b, tables_, 1, BV_CP, BV_TRT, BV_CAP, true, s);
+      batch::finalize(b, 1, BS_CPAP, s);
+      batch::cg_xr<T>(b, a, s);
+      batch::finalize(b, 1, c, s);
+      batch::cg_p<T>(b, c, a, s);
+      std::swap(a, c);
+    }
   }
   batch::prolong<T>(b, tables_, s);
-  for (int it = 0; it < cfg_.post_it; ++it) smooth(false);
+  for (int it = 0; it < cfg_.post_it; ++it) smooth(0, false);
   batch::from_t(b, f32, BV_VX, BV_Z, s);
   batch::sum_d(b, BV_Z, s);
   batch::finalize(b, 1, BS_MEAN, s);
@@ -278,6 +361,7 @@ void BatchEngine::run_iteration(View& v) {
 std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
   const int d = cfg_.dim, Nf = d + 1, NK = d == 2 ? 3 : 6;
   std::vector<int> ids;
+  std::vector<HostMeshPtr> meshes;
   for (auto& sp : specs) {
     if (!sp.mesh || sp.mesh->dim != d) throw InvalidArgument("problem dimension does not match the batch");
     if (sp.order < 1 || sp.order > kMaxOrder) throw InvalidArgument("order must be in 1..4");
@@ -420,43 +504,42 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
     probs_[slot] = std::move(P);
     upload_desc(slot);
     ids.push_back(slot);
+    meshes.push_back(sp.mesh);
   }
   if (ids.empty()) return ids;
-  // Chebyshev bounds: power iteration on D^-1 A for all new problems at once.
+  // Chebyshev bounds: power iteration on D^-1 A for all new problems at once, on order N
+  // and (AMG coarse solve) on order 1.
   set_view(*work_, ids);
-  const BatchView& b = work_->bv;
-  hipStream_t s = stream_;
-  batch::rand_d(b, BV_R, s);
-  for (int it = 0; it < std::max(1, cfg_.max_iter_eig); ++it) {
-    batch::trace<double>(b, tables_, BV_R, BV_TRD, s);
-    batch::apply<double>(b, tables_, 0, BV_R, BV_TRD, BV_Z, false, s);
-    batch::inv_apply_d(b, BV_Z, s);
-    batch::dot<double>(b, 0, BV_R, BV_Z, BV_R, BV_R, 2, s);
-    batch::finalize(b, 2, BS_TOP, s);
-    batch::dot<double>(b, 0, BV_Z, BV_Z, BV_Z, BV_Z, 1, s);
-    batch::finalize(b, 1, BS_NRM, s);
-    batch::from_t(b, false, BV_Z, BV_R, s);
-    batch::scale_d(b, BV_R, BS_NRM, s);
-  }
-  read_scal(ids);
-  for (int slot : ids) {
-    Prob& P = *probs_[slot];
-    const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
-    const double lam = sc[BS_BOT] > 0 ? sc[BS_TOP] / sc[BS_BOT] : 1.0;
-    const double hi = cfg_.eig_safety * lam, lo = cfg_.cheb_min_factor * hi;
-    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-    double rho_old = 1.0 / sigma;
-    P.desc.c1[0] = 0.0;
-    P.desc.c2[0] = 1.0 / theta;
-    for (int k = 1; k < 8; ++k) {
-      const double rho = 1.0 / (2.0 * sigma - rho_old);
-      P.desc.c1[k] = rho * rho_old;
-      P.desc.c2[k] = 2.0 * rho / delta;
-      rho_old = rho;
+  for (int level = 0; level < (cfg_.amg ? 2 : 1); ++level) {
+    const BatchView& b = work_->bv;
+    hipStream_t s = stream_;
+    batch::rand_d(b, level, BV_R, s);
+    for (int it = 0; it < std::max(1, cfg_.max_iter_eig); ++it) {
+      if (level == 0) batch::trace<double>(b, tables_, BV_R, BV_TRD, s);
+      batch::apply<double>(b, tables_, level, BV_R, BV_TRD, BV_Z, false, s);
+      batch::inv_apply_d(b, level, BV_Z, s);
+      batch::dot<double>(b, level, BV_R, BV_Z, BV_R, BV_R, 2, s);
+      batch::finalize(b, 2, BS_TOP, s);
+      batch::dot<double>(b, level, BV_Z, BV_Z, BV_Z, BV_Z, 1, s);
+      batch::finalize(b, 1, BS_NRM, s);
+      batch::from_t(b, false, BV_Z, BV_R, s);
+      batch::scale_d(b, BV_R, BS_NRM, s);
     }
-    INSP_HIP(hipMemcpyAsync(d_desc_ + slot, &P.desc, sizeof(BatchProb), hipMemcpyHostToDevice, stream_));
+    read_scal(ids);
+    for (int slot : ids) {
+      Prob& P = *probs_[slot];
+      const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
+      const double lam = sc[BS_BOT] > 0 ? sc[BS_TOP] / sc[BS_BOT] : 1.0;
+      chebyshev_coefficients(cfg_.eig_safety * lam, cfg_.cheb_min_factor, 8,
+                             level ? P.desc.amg.oc1 : P.desc.c1, level ? P.desc.amg.oc2 : P.desc.c2);
+    }
   }
-  // clear the power-iteration scratch (x starts at zero)
+  if (cfg_.amg)
+    for (size_t q = 0; q < ids.size(); ++q) setup_amg(ids[q], *meshes[q]);
+  for (int slot : ids)
+    INSP_HIP(hipMemcpyAsync(d_desc_ + slot, &probs_[slot]->desc, sizeof(BatchProb), hipMemcpyHostToDevice,
+                            stream_));
+  // clear the power-iteration and probing scratch (x starts at zero)
   for (int slot : ids) {
     Prob& P = *probs_[slot];
     fill<double>((int64_t)P.Np * P.S, (double*)P.desc.v[BV_R], 0.0, stream_);
@@ -465,6 +548,158 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
   INSP_HIP(hipStreamSynchronize(stream_));
   start(*work_, ids, false);
   return ids;
+}
+
+void BatchEngine::chebyshev_coefficients(double hi, double min_factor, int n, double* c1, double* c2) {
+  const double lo = min_factor * hi;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  c1[0] = 0.0;
+  c2[0] = 1.0 / theta;
+  for (int k = 1; k < n; ++k) {
+    const double rho = 1.0 / (2.0 * sigma - rho_old);
+    c1[k] = rho * rho_old;
+    c2[k] = 2.0 * rho / delta;
+    rho_old = rho;
+  }
+}
+
+size_t BatchEngine::amg_bytes_estimate(int dim, int64_t n_nodes, int64_t cells, bool fp32) {
+  // vertex maps + operator (~(7|15) entries per row in 2D|3D) over a hierarchy of about
+  // 1.5x the vertex rows, with transfers and four work vectors per row, plus the dense
+  // last level.
+  const size_t t = fp32 ? 4 : 8, nnz_row = dim == 2 ? 7 : 15;
+  const size_t rows = (size_t)(1.5 * (double)n_nodes) + 1;
+  size_t b = (size_t)(dim + 1) * cells * 8 + (size_t)n_nodes * 4;   // cv, vlist, vptr
+  b += rows * nnz_row * (4 + t) * 3 / 2 + rows * (4 + 5 * t) + 2 * rows * 8 * (4 + t);
+  b += (size_t)kAmgMaxCoarse * kAmgMaxCoarse * t + 64 * 1024;
+  return b;
+}
+
+void BatchEngine::setup_amg(int slot, const HostMesh& m) {
+  Prob& P = *probs_[slot];
+  const int d1 = cfg_.dim + 1, nv = (int)m.n_nodes;
+  const int64_t E = P.E, S = P.S;
+  const size_t t = cfg_.fp32 ? 4 : 8;
+  // vertex maps: order-1 nodes are the cell vertices in cell2nodes order
+  std::vector<int> vptr((size_t)nv + 1, 0), vlist((size_t)E * d1), cv((size_t)d1 * S, 0);
+  for (int64_t e = 0; e < E; ++e)
+    for (int i = 0; i < d1; ++i) ++vptr[(size_t)m.cell2nodes[e * d1 + i] + 1];
+  for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+  {
+    std::vector<int> pos(vptr.begin(), vptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int i = 0; i < d1; ++i) {
+        const int v = m.cell2nodes[e * d1 + i];
+        vlist[pos[v]++] = (int)(i * S + e);
+        cv[(size_t)i * S + e] = v;
+      }
+  }
+  // probe A_v = P_v^T A_1 P_v with distance-2 colour classes (double, on this problem alone)
+  HostCsr g = vertex_graph(m);
+  int ncol = 0;
+  const std::vector<int> colour = distance2_colouring(g, &ncol);
+  DBuf<int> d_cv, d_vptr, d_vlist, gptr, gcol, dcol;
+  d_cv.upload(cv.data(), cv.size());
+  d_vptr.upload(vptr.data(), vptr.size());
+  d_vlist.upload(vlist.data(), vlist.size());
+  gptr.upload(g.ptr.data(), g.ptr.size());
+  gcol.upload(g.col.data(), g.col.size());
+  dcol.upload(colour.data(), colour.size());
+  DBuf<double> val, xv, yv;
+  val.alloc(g.col.size());
+  xv.alloc((size_t)nv);
+  yv.alloc((size_t)nv);
+  set_view(*work_, {slot});
+  double* u = (double*)P.desc.v[BV_R];
+  double* y = (double*)P.desc.v[BV_Z];
+  for (int c = 0; c < ncol; ++c) {
+    amg::probe_fill(nv, dcol.p, c, xv.p, stream_);
+    amg::vertex_prolong<double>(d1, E, S, d_cv.p, xv.p, u, false, stream_);
+    batch::apply<double>(work_->bv, tables_, 1, BV_R, BV_TRD, BV_Z, false, stream_);
+    amg::vertex_restrict<double>(nv, d_vptr.p, d_vlist.p, y, yv.p, stream_);
+    amg::probe_extract(nv, gptr.p, gcol.p, dcol.p, c, yv.p, val.p, stream_);
+  }
+  INSP_HIP(hipMemcpyAsync(g.val.data(), val.p, g.val.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+  INSP_HIP(hipStreamSynchronize(stream_));
+  {  // the hierarchy is built from the symmetric part
+    HostCsr gt = csr_transpose(g);
+    for (size_t k = 0; k < g.val.size(); ++k) g.val[k] = 0.5 * (g.val[k] + gt.val[k]);
+  }
+  AmgOptions opt;
+  opt.max_coarse = kAmgMaxCoarse;
+  opt.max_levels = kAmgLevels;
+  opt.cheb_order = cfg_.amg_cheb;
+  const AmgHierarchy H = amg_setup(std::move(g), opt, P.nullspace);
+  if (H.n_coarse > kAmgMaxCoarse) throw std::runtime_error("AMG coarsening stalled above the dense size");
+  // one device allocation for the maps and the hierarchy
+  auto a = [](size_t bytes) { return align256(bytes); };
+  size_t bytes = a(cv.size() * 4) + a(vptr.size() * 4) + a(vlist.size() * 4);
+  for (size_t k = 0; k < H.levels.size(); ++k) {
+    const AmgHostLevel& h = H.levels[k];
+    bytes += a((h.A.ptr.size()) * 4) + a(h.A.col.size() * 4) + a(h.A.val.size() * t) + 5 * a((size_t)h.A.n * t);
+    if (k + 1 < H.levels.size())
+      bytes += a(h.P.ptr.size() * 4) + a(h.P.col.size() * 4) + a(h.P.val.size() * t) + a(h.R.ptr.size() * 4) +
+               a(h.R.col.size() * 4) + a(h.R.val.size() * t);
+  }
+  bytes += a((size_t)H.n_coarse * H.n_coarse * t);
+  INSP_HIP(hipMalloc(&P.amg_arena, bytes));
+  INSP_HIP(hipMemset(P.amg_arena, 0, bytes));
+  INSP_HIP(hipDeviceSynchronize());
+  P.amg_bytes = bytes;
+  bytes_in_use_ += bytes;
+  char* base = (char*)P.amg_arena;
+  size_t off = 0;
+  auto put = [&](const void* h, size_t nbytes) -> void* {
+    if (off + a(nbytes) > bytes) throw std::runtime_error("AMG arena layout overflow");
+    void* d = base + off;
+    off += a(nbytes);
+    if (h && nbytes) INSP_HIP(hipMemcpy(d, h, nbytes, hipMemcpyHostToDevice));
+    return d;
+  };
+  auto put_t = [&](const std::vector<double>& v) -> void* {
+    if (cfg_.fp32) {
+      std::vector<float> f(v.begin(), v.end());
+      return put(f.data(), f.size() * 4);
+    }
+    return put(v.data(), v.size() * 8);
+  };
+  BatchAmg& D = P.desc.amg;
+  D.cv = (const int*)put(cv.data(), cv.size() * 4);
+  D.vptr = (const int*)put(vptr.data(), vptr.size() * 4);
+  D.vlist = (const int*)put(vlist.data(), vlist.size() * 4);
+  D.nv = nv;
+  D.nlev = (int)H.levels.size();
+  P.amg_rows.clear();
+  for (size_t k = 0; k < H.levels.size(); ++k) {
+    const AmgHostLevel& h = H.levels[k];
+    BatchAmgLevel& L = D.lev[k];
+    L.n = h.A.n;
+    L.m = k + 1 < H.levels.size() ? h.P.m : 0;
+    L.Aptr = (const int*)put(h.A.ptr.data(), h.A.ptr.size() * 4);
+    L.Acol = (const int*)put(h.A.col.data(), h.A.col.size() * 4);
+    L.Aval = put_t(h.A.val);
+    std::vector<double> inv((size_t)h.A.n);
+    for (int i = 0; i < h.A.n; ++i) inv[i] = h.diag[i] != 0 ? 1.0 / h.diag[i] : 0.0;
+    L.inv = put_t(inv);
+    L.x = put(nullptr, (size_t)h.A.n * t);
+    L.b = put(nullptr, (size_t)h.A.n * t);
+    L.r = put(nullptr, (size_t)h.A.n * t);
+    L.d = put(nullptr, (size_t)h.A.n * t);
+    if (k + 1 < H.levels.size()) {
+      L.Pptr = (const int*)put(h.P.ptr.data(), h.P.ptr.size() * 4);
+      L.Pcol = (const int*)put(h.P.col.data(), h.P.col.size() * 4);
+      L.Pval = put_t(h.P.val);
+      L.Rptr = (const int*)put(h.R.ptr.data(), h.R.ptr.size() * 4);
+      L.Rcol = (const int*)put(h.R.col.data(), h.R.col.size() * 4);
+      L.Rval = put_t(h.R.val);
+    }
+    chebyshev_coefficients(1.1 * h.lmax, 1.0 / 20.0, 4, L.ch1, L.ch2);
+    P.amg_rows.push_back(h.A.n);
+  }
+  D.nc = H.n_coarse;
+  D.cinv = put_t(H.coarse_inv);
 }
 
 // r = b - A x, convergence test, then z = M r, p = z for problems not yet converged.
@@ -543,9 +778,12 @@ void BatchEngine::iterate() {
   if (running_slots_.empty()) return;
   const bool refresh = graph_tile_cap_ == -2 || !run_->tiles.p;
   if (refresh) {
-    const int old_t = run_->tile_cap, old_p = run_->prob_cap;
+    const int old_t = run_->tile_cap, old_p = run_->prob_cap, old_l = run_->ltile_cap,
+              old_lev = run_->levels;
     set_view(*run_, running_slots_);
-    if (run_->tile_cap != old_t || run_->prob_cap != old_p || !graph_) {
+    // the graph bakes in the grid sizes and the number of AMG levels it walks
+    if (run_->tile_cap != old_t || run_->prob_cap != old_p || run_->ltile_cap != old_l ||
+        run_->levels != old_lev || !graph_) {
       if (graph_) {
         INSP_HIP(hipGraphExecDestroy(graph_));
         graph_ = nullptr;
@@ -610,7 +848,8 @@ void BatchEngine::release(int id) {
   if (!probs_[id]->done) throw InvalidArgument("problem is still running");
   INSP_HIP(hipStreamSynchronize(stream_));
   (void)hipFree(probs_[id]->arena);
-  bytes_in_use_ -= probs_[id]->bytes;
+  if (probs_[id]->amg_arena) (void)hipFree(probs_[id]->amg_arena);
+  bytes_in_use_ -= probs_[id]->bytes + probs_[id]->amg_bytes;
   probs_[id].reset();
   free_slots_.push_back(id);
 }

@@ -44,6 +44,29 @@ enum BatchScal : int {
   BS_COUNT = 16
 };
 
+constexpr int kAmgLevels = 8;   // deepest AMG hierarchy of one problem in a batch
+constexpr int kAmgMaxCoarse = 256;
+
+// One level of a problem's vertex-space AMG hierarchy (values in the engine's
+// V-cycle precision).
+struct BatchAmgLevel {
+  int n, m;                      // rows, rows of the next level (0 on the last)
+  const int *Aptr, *Acol, *Pptr, *Pcol, *Rptr, *Rcol;
+  const void *Aval, *Pval, *Rval, *inv;
+  void *x, *b, *r, *d;
+  double ch1[4], ch2[4];         // Chebyshev step coefficients
+};
+
+struct BatchAmg {
+  int nlev, nv, nc, pad;         // levels, vertices, size of the dense last level
+  const int* cv;                 // order-1 SoA entry -> vertex
+  const int* vptr;               // vertex -> SoA entries (gather list)
+  const int* vlist;
+  const void* cinv;              // dense symmetric inverse on the last level
+  double oc1[8], oc2[8];         // Chebyshev coefficients of the order-1 DG level
+  BatchAmgLevel lev[kAmgLevels];
+};
+
 struct BatchProb {
   int N, Np, Np1, var, nullspace, pad;
   int64_t E, S;                  // cells, stride (multiple of kTile)
@@ -62,6 +85,7 @@ struct BatchProb {
   void* v[BV_COUNT];
   double* scal;                  // BS_COUNT doubles
   double c1[8], c2[8];           // Chebyshev step coefficients (order N level)
+  BatchAmg amg;                  // vertex-space coarse solve (nlev = 0: Jacobi-CG on order 1)
 };
 
 struct BatchTables {
@@ -84,6 +108,11 @@ struct BatchView {
   int tile_cap, prob_cap;        // grid sizes
   double* partial;               // 4 * tile_cap doubles
   int dim;
+  // AMG row tiles: level k lists 256-row tiles (slot, first row) of every problem that
+  // has level k, at ltiles[k * ltile_cap + t], with lcount[k] entries.
+  const int2* ltiles;
+  const int* lcount;
+  int ltile_cap;
 };
 
 namespace batch {
@@ -103,10 +132,28 @@ void scal_copy(const BatchView& b, int dst, int src, hipStream_t s);
 void residual(const BatchView& b, hipStream_t s);           // r = b - Ap, partial |r|^2
 void to_t(const BatchView& b, bool fp32, int src_d, int dst, hipStream_t s);   // double -> T
 void from_t(const BatchView& b, bool fp32, int src, int dst_d, hipStream_t s); // T -> double
+// level 0: order-N fields (VB, VAX, VD, VX, invF, c1/c2); level 1: order-1 fields
+// (CB, CAP, CZ, CX, invC, amg.oc1/oc2).
 template <typename T>
-void cheb(const BatchView& b, int step, bool zero_init, hipStream_t s);
+void cheb(const BatchView& b, int level, int step, bool zero_init, hipStream_t s);
 template <typename T>
-void sub_from_b(const BatchView& b, hipStream_t s);         // vax = vb - vax
+void sub_from_b(const BatchView& b, int level, hipStream_t s);  // ax = b - ax on that level
+// Vertex-space AMG (problems with amg.nlev > 0).
+template <typename T>
+void vertex_restrict(const BatchView& b, hipStream_t s);   // lev[0].b = P_v^T CAP
+template <typename T>
+void vertex_prolong(const BatchView& b, hipStream_t s);    // CX += P_v lev[0].x
+template <typename T>
+void amg_spmv(const BatchView& b, int k, bool residual, hipStream_t s);  // r = A x or b - A x
+template <typename T>
+void amg_cheb(const BatchView& b, int k, int step, bool zero_init, hipStream_t s);
+template <typename T>
+void amg_restrict(const BatchView& b, int k, hipStream_t s);  // lev[k+1].b = R lev[k].r
+template <typename T>
+void amg_prolong(const BatchView& b, int k, hipStream_t s);   // lev[k].x += P lev[k+1].x
+template <typename T>
+void amg_dense(const BatchView& b, int k, hipStream_t s);     // last level k: x = Cinv b
+
 template <typename T>
 void restrict_(const BatchView& b, const BatchTables& t, hipStream_t s);
 template <typename T>
@@ -119,8 +166,8 @@ template <typename T>
 void cg_p(const BatchView& b, int num, int den, hipStream_t s);
 void sum_d(const BatchView& b, int field, hipStream_t s);   // partial sum (double field)
 void sub_mean_d(const BatchView& b, int field, hipStream_t s);
-void rand_d(const BatchView& b, int field, hipStream_t s);
-void inv_apply_d(const BatchView& b, int field, hipStream_t s);   // field *= invF (double)
+void rand_d(const BatchView& b, int level, int field, hipStream_t s);
+void inv_apply_d(const BatchView& b, int level, int field, hipStream_t s);  // field *= inv (double)
 void scale_d(const BatchView& b, int field, int slot, hipStream_t s); // field /= sqrt(scal)
 }  // namespace batch
 
@@ -132,6 +179,8 @@ struct BatchConfig {
   double eig_safety = 1.1, cheb_min_factor = 0.1;
   int max_iter_eig = 15;
   int coarse_iters = 40;     // Jacobi-CG iterations on order 1
+  bool amg = true;           // vertex-space AMG below order 1 instead of Jacobi-CG
+  int amg_cheb = 2;          // Chebyshev degree on the AMG levels
   bool use_graph = true;     // replay one captured iteration
 };
 
@@ -168,6 +217,8 @@ class BatchEngine {
 
   // Device bytes one problem occupies (admission by HBM budget).
   static size_t arena_bytes(int dim, int64_t cells, int order, bool var, bool fp32);
+  // Upper estimate of the extra device bytes of the AMG coarse solve of one problem.
+  static size_t amg_bytes_estimate(int dim, int64_t n_nodes, int64_t cells, bool fp32);
   // Uploads, sets up (penalty, diagonals, Chebyshev bounds) and starts a problem.
   // Returns its id. Set-up of several problems can be batched with add_many.
   std::vector<int> add_many(std::vector<ProblemSpec>&& specs);
@@ -175,6 +226,7 @@ class BatchEngine {
   void iterate();
   int running() const;
   int free_slots() const { return (int)free_slots_.size(); }
+  const BatchConfig& config() const { return cfg_; }
   // Ids finished since the last call (their results are ready).
   std::vector<int> take_finished();
   const ProbResult& result(int id) const;
@@ -192,6 +244,8 @@ class BatchEngine {
   void run_iteration(View& v);
   void start(View& v, const std::vector<int>& slots, bool restart);
   void finish(int slot, ProbStatus st);
+  void setup_amg(int slot, const HostMesh& m);
+  static void chebyshev_coefficients(double hi, double min_factor, int n, double* c1, double* c2);
   void read_scal(const std::vector<int>& slots);
   template <typename T>
   void vcycle(View& v);

@@ -232,16 +232,17 @@ __global__ void __launch_bounds__(kB) kb_convert(BatchView b, int src, int dst) 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int step, int zero_init) {
+__global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int level, int step, int zero_init) {
   const Cell c = locate(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const int prec = std::is_same<T, float>::value;
-  const T* inv = (const T*)P.invF[prec];
-  const T *f = (const T*)P.v[BV_VB], *ax = (const T*)P.v[BV_VAX];
-  T *d = (T*)P.v[BV_VD], *x = (T*)P.v[BV_VX];
-  const T c1 = (T)P.c1[step], c2 = (T)P.c2[step];
-  for (int i = 0; i < P.Np; ++i) {
+  const int rows = level ? P.Np1 : P.Np;
+  const T* inv = (const T*)(level ? P.invC[prec] : P.invF[prec]);
+  const T *f = (const T*)P.v[level ? BV_CB : BV_VB], *ax = (const T*)P.v[level ? BV_CAP : BV_VAX];
+  T *d = (T*)P.v[level ? BV_CZ : BV_VD], *x = (T*)P.v[level ? BV_CX : BV_VX];
+  const T c1 = (T)(level ? P.amg.oc1[step] : P.c1[step]), c2 = (T)(level ? P.amg.oc2[step] : P.c2[step]);
+  for (int i = 0; i < rows; ++i) {
     const int64_t q = i * P.S + c.e;
     const T res = inv[q] * (zero_init ? f[q] : f[q] - ax[q]);
     const T di = step == 0 ? c2 * res : fma(c1, d[q], c2 * res);
@@ -251,16 +252,140 @@ __global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int step, int zero_in
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kB) kb_sub_from_b(BatchView b) {
+__global__ void __launch_bounds__(kB) kb_sub_from_b(BatchView b, int level) {
   const Cell c = locate(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
-  const T* f = (const T*)P.v[BV_VB];
-  T* ax = (T*)P.v[BV_VAX];
-  for (int i = 0; i < P.Np; ++i) {
+  const int rows = level ? P.Np1 : P.Np;
+  const T* f = (const T*)P.v[level ? BV_CB : BV_VB];
+  T* ax = (T*)P.v[level ? BV_CAP : BV_VAX];
+  for (int i = 0; i < rows; ++i) {
     const int64_t q = i * P.S + c.e;
     ax[q] = f[q] - ax[q];
   }
+}
+
+// ---- vertex-space AMG ----------------------------------------------------
+
+struct Row {
+  int slot, r;
+  bool valid;
+};
+
+// Rows of AMG level k: one 256-row tile per block, from the level's tile table.
+__device__ __forceinline__ Row locate_row(const BatchView& b, int k) {
+  Row c;
+  const int t = (int)xcd_block();
+  c.slot = 0;
+  c.r = 0;
+  c.valid = false;
+  if (t < b.lcount[k]) {
+    const int2 tt = b.ltiles[(int64_t)k * b.ltile_cap + t];
+    c.slot = tt.x;
+    c.r = tt.y + threadIdx.x;
+    c.valid = c.r < b.P[c.slot].amg.lev[k].n;
+  }
+  return c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_vertex_restrict(BatchView b) {
+  const Row c = locate_row(b, 0);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const T* r = (const T*)P.v[BV_CAP];
+  T s = 0;
+  for (int q = P.amg.vptr[c.r]; q < P.amg.vptr[c.r + 1]; ++q) s += r[P.amg.vlist[q]];
+  ((T*)P.amg.lev[0].b)[c.r] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_vertex_prolong(BatchView b) {
+  const Cell c = locate(b);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  if (P.amg.nlev == 0) return;
+  const T* xv = (const T*)P.amg.lev[0].x;
+  T* x = (T*)P.v[BV_CX];
+  for (int i = 0; i < P.Np1; ++i) {
+    const int64_t q = i * P.S + c.e;
+    x[q] += xv[P.amg.cv[q]];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_amg_spmv(BatchView b, int k, int residual) {
+  const Row c = locate_row(b, k);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  if (k >= P.amg.nlev - 1) return;  // the last level is solved densely
+  const BatchAmgLevel& L = P.amg.lev[k];
+  const T* val = (const T*)L.Aval;
+  const T* x = (const T*)L.x;
+  T s = 0;
+  for (int q = L.Aptr[c.r]; q < L.Aptr[c.r + 1]; ++q) s = fma(val[q], x[L.Acol[q]], s);
+  ((T*)L.r)[c.r] = residual ? ((const T*)L.b)[c.r] - s : s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_amg_cheb(BatchView b, int k, int step, int zero_init) {
+  const Row c = locate_row(b, k);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  if (k >= P.amg.nlev - 1) return;
+  const BatchAmgLevel& L = P.amg.lev[k];
+  const T f = ((const T*)L.b)[c.r];
+  const T res = ((const T*)L.inv)[c.r] * (zero_init ? f : f - ((const T*)L.r)[c.r]);
+  T* d = (T*)L.d;
+  T* x = (T*)L.x;
+  const T di = step == 0 ? (T)L.ch2[0] * res : fma((T)L.ch1[step], d[c.r], (T)L.ch2[step] * res);
+  d[c.r] = di;
+  x[c.r] = zero_init ? di : x[c.r] + di;
+}
+
+// Rows of level k+1: b_{k+1} = R_k r_k.
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_amg_restrict(BatchView b, int k) {
+  const Row c = locate_row(b, k + 1);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  const BatchAmgLevel& L = P.amg.lev[k];
+  const T* val = (const T*)L.Rval;
+  const T* r = (const T*)L.r;
+  T s = 0;
+  for (int q = L.Rptr[c.r]; q < L.Rptr[c.r + 1]; ++q) s = fma(val[q], r[L.Rcol[q]], s);
+  ((T*)P.amg.lev[k + 1].b)[c.r] = s;
+}
+
+// Rows of level k: x_k += P_k x_{k+1}.
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_amg_prolong(BatchView b, int k) {
+  const Row c = locate_row(b, k);
+  if (!c.valid) return;
+  const BatchProb& P = b.P[c.slot];
+  if (k >= P.amg.nlev - 1) return;
+  const BatchAmgLevel& L = P.amg.lev[k];
+  const T* val = (const T*)L.Pval;
+  const T* xc = (const T*)P.amg.lev[k + 1].x;
+  T s = 0;
+  for (int q = L.Pptr[c.r]; q < L.Pptr[c.r + 1]; ++q) s = fma(val[q], xc[L.Pcol[q]], s);
+  ((T*)L.x)[c.r] += s;
+}
+
+// One block per problem whose last level is k: x = Cinv b (Cinv symmetric, n <= 256).
+template <typename T>
+__global__ void __launch_bounds__(kB) kb_amg_dense(BatchView b, int k) {
+  const int j = blockIdx.x;
+  if (j >= *b.nprobs) return;
+  const BatchProb& P = b.P[b.probs[j]];
+  if (P.amg.nlev == 0 || P.amg.nlev - 1 != k) return;
+  const int n = P.amg.nc, i = threadIdx.x;
+  if (i >= n) return;
+  const T* M = (const T*)P.amg.cinv;
+  const T* bb = (const T*)P.amg.lev[k].b;
+  T s = 0;
+  for (int q = 0; q < n; ++q) s = fma(M[(int64_t)q * n + i], bb[q], s);
+  ((T*)P.amg.lev[k].x)[i] = s;
 }
 
 template <typename T, int R, int K, bool ADD>
@@ -403,14 +528,16 @@ __global__ void __launch_bounds__(kB) kb_sub_mean_d(BatchView b, int field) {
   for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] -= m;
 }
 
-__global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int field) {
+__global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int level, int field) {
   const Cell c = locate(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   double* x = (double*)P.v[field];
-  for (int i = 0; i < P.Np; ++i) {
+  const int rows = level ? P.Np1 : P.Np;
+  const int N = level ? 1 : P.N;
+  for (int i = 0; i < rows; ++i) {
     const uint64_t idx = (uint64_t)(i * P.S + c.e);
-    uint64_t z = 0x5eedULL + 977ULL * P.N + 0x9E3779B97F4A7C15ull * (idx + 1);
+    uint64_t z = 0x5eedULL + 977ULL * N + 0x9E3779B97F4A7C15ull * (idx + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
@@ -418,13 +545,14 @@ __global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int field) {
   }
 }
 
-__global__ void __launch_bounds__(kB) kb_inv_apply_d(BatchView b, int field) {
+__global__ void __launch_bounds__(kB) kb_inv_apply_d(BatchView b, int level, int field) {
   const Cell c = locate(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
-  const double* inv = (const double*)P.invF[0];
+  const double* inv = (const double*)(level ? P.invC[0] : P.invF[0]);
   double* x = (double*)P.v[field];
-  for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] *= inv[i * P.S + c.e];
+  const int rows = level ? P.Np1 : P.Np;
+  for (int i = 0; i < rows; ++i) x[i * P.S + c.e] *= inv[i * P.S + c.e];
 }
 
 __global__ void __launch_bounds__(kB) kb_scale_d(BatchView b, int field, int slot) {
@@ -504,15 +632,58 @@ void from_t(const BatchView& b, bool fp32, int src, int dst_d, hipStream_t s) {
 }
 
 template <typename T>
-void cheb(const BatchView& b, int step, bool zero_init, hipStream_t s) {
-  hipLaunchKernelGGL((kb_cheb<T>), tiles_grid(b), dim3(kB), 0, s, b, step, (int)zero_init);
+void cheb(const BatchView& b, int level, int step, bool zero_init, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cheb<T>), tiles_grid(b), dim3(kB), 0, s, b, level, step, (int)zero_init);
   check("kb_cheb");
 }
 
 template <typename T>
-void sub_from_b(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL((kb_sub_from_b<T>), tiles_grid(b), dim3(kB), 0, s, b);
+void sub_from_b(const BatchView& b, int level, hipStream_t s) {
+  hipLaunchKernelGGL((kb_sub_from_b<T>), tiles_grid(b), dim3(kB), 0, s, b, level);
   check("kb_sub_from_b");
+}
+
+template <typename T>
+void vertex_restrict(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL((kb_vertex_restrict<T>), dim3((unsigned)b.ltile_cap), dim3(kB), 0, s, b);
+  check("kb_vertex_restrict");
+}
+
+template <typename T>
+void vertex_prolong(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL((kb_vertex_prolong<T>), tiles_grid(b), dim3(kB), 0, s, b);
+  check("kb_vertex_prolong");
+}
+
+template <typename T>
+void amg_spmv(const BatchView& b, int k, bool residual, hipStream_t s) {
+  hipLaunchKernelGGL((kb_amg_spmv<T>), dim3((unsigned)b.ltile_cap), dim3(kB), 0, s, b, k, (int)residual);
+  check("kb_amg_spmv");
+}
+
+template <typename T>
+void amg_cheb(const BatchView& b, int k, int step, bool zero_init, hipStream_t s) {
+  hipLaunchKernelGGL((kb_amg_cheb<T>), dim3((unsigned)b.ltile_cap), dim3(kB), 0, s, b, k, step,
+                     (int)zero_init);
+  check("kb_amg_cheb");
+}
+
+template <typename T>
+void amg_restrict(const BatchView& b, int k, hipStream_t s) {
+  hipLaunchKernelGGL((kb_amg_restrict<T>), dim3((unsigned)b.ltile_cap), dim3(kB), 0, s, b, k);
+  check("kb_amg_restrict");
+}
+
+template <typename T>
+void amg_prolong(const BatchView& b, int k, hipStream_t s) {
+  hipLaunchKernelGGL((kb_amg_prolong<T>), dim3((unsigned)b.ltile_cap), dim3(kB), 0, s, b, k);
+  check("kb_amg_prolong");
+}
+
+template <typename T>
+void amg_dense(const BatchView& b, int k, hipStream_t s) {
+  hipLaunchKernelGGL((kb_amg_dense<T>), dim3((unsigned)b.prob_cap), dim3(kB), 0, s, b, k);
+  check("kb_amg_dense");
 }
 
 template <typename T>
@@ -555,13 +726,13 @@ void sub_mean_d(const BatchView& b, int field, hipStream_t s) {
   check("kb_sub_mean_d");
 }
 
-void rand_d(const BatchView& b, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_rand_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+void rand_d(const BatchView& b, int level, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_rand_d, tiles_grid(b), dim3(kB), 0, s, b, level, field);
   check("kb_rand_d");
 }
 
-void inv_apply_d(const BatchView& b, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_inv_apply_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+void inv_apply_d(const BatchView& b, int level, int field, hipStream_t s) {
+  hipLaunchKernelGGL(kb_inv_apply_d, tiles_grid(b), dim3(kB), 0, s, b, level, field);
   check("kb_inv_apply_d");
 }
 
@@ -575,8 +746,15 @@ void scale_d(const BatchView& b, int field, int slot, hipStream_t s) {
   template void apply<T>(const BatchView&, const BatchTables&, int, int, int, int, bool,          \
                          hipStream_t);                                                            \
   template void dot<T>(const BatchView&, int, int, int, int, int, int, hipStream_t);              \
-  template void cheb<T>(const BatchView&, int, bool, hipStream_t);                                \
-  template void sub_from_b<T>(const BatchView&, hipStream_t);                                     \
+  template void cheb<T>(const BatchView&, int, int, bool, hipStream_t);                           \
+  template void sub_from_b<T>(const BatchView&, int, hipStream_t);                                \
+  template void vertex_restrict<T>(const BatchView&, hipStream_t);                                \
+  template void vertex_prolong<T>(const BatchView&, hipStream_t);                                 \
+  template void amg_spmv<T>(const BatchView&, int, bool, hipStream_t);                            \
+  template void amg_cheb<T>(const BatchView&, int, int, bool, hipStream_t);                       \
+  template void amg_restrict<T>(const BatchView&, int, hipStream_t);                              \
+  template void amg_prolong<T>(const BatchView&, int, hipStream_t);                               \
+  template void amg_dense<T>(const BatchView&, int, hipStream_t);                                 \
   template void restrict_<T>(const BatchView&, const BatchTables&, hipStream_t);                  \
   template void prolong<T>(const BatchView&, const BatchTables&, hipStream_t);                    \
   template void cg_init<T>(const BatchView&, hipStream_t);                                        \

@@ -83,6 +83,9 @@ insp::BatchConfig to_batch_config(int dim, const ins_batch_config* c) {
   b.max_iter_eig = c->max_iter_eig;
   b.coarse_iters = c->coarse_iters;
   b.use_graph = c->use_graph != 0;
+  if (c->coarse != INS_COARSE_JACOBI_CG && c->coarse != INS_COARSE_AMG)
+    throw insp::InvalidArgument("unknown coarse solve");
+  b.amg = c->coarse == INS_COARSE_AMG;
   return b;
 }
 
@@ -580,6 +583,17 @@ void ins_batch_config_default(ins_batch_config* c) {
   c->max_iter_eig = d.max_iter_eig;
   c->coarse_iters = d.coarse_iters;
   c->use_graph = d.use_graph ? 1 : 0;
+  c->coarse = d.amg ? INS_COARSE_AMG : INS_COARSE_JACOBI_CG;
+}
+
+int ins_batch_amg_bytes(int dim, int64_t n_nodes, int64_t n_cells, int precision, int64_t* bytes) {
+  return guarded([&] {
+    if ((dim != 2 && dim != 3) || n_nodes < 1 || n_cells < 1 || !bytes)
+      throw insp::InvalidArgument("invalid problem size");
+    *bytes = (int64_t)insp::BatchEngine::amg_bytes_estimate(dim, n_nodes, n_cells,
+                                                            precision == INS_PRECISION_MIXED);
+    return INS_OK;
+  });
 }
 
 int ins_batch_problem_bytes(int dim, int64_t n_cells, int order, int has_factor, int precision,

@@ -31,8 +31,9 @@ int64_t Server::submit(ProblemSpec&& spec) {
   p.id = next_id_++;
   p.dim = dim;
   p.arrival = std::chrono::steady_clock::now();
-  p.bytes = BatchEngine::arena_bytes(dim, spec.mesh->n_cells, spec.order, !spec.factor.empty(),
-                                     dim == 2 ? cfg_.cfg2d.fp32 : cfg_.cfg3d.fp32);
+  const BatchConfig& bc = dim == 2 ? cfg_.cfg2d : cfg_.cfg3d;
+  p.bytes = BatchEngine::arena_bytes(dim, spec.mesh->n_cells, spec.order, !spec.factor.empty(), bc.fp32);
+  if (bc.amg) p.bytes += BatchEngine::amg_bytes_estimate(dim, spec.mesh->n_nodes, spec.mesh->n_cells, bc.fp32);
   p.spec = std::move(spec);
   {
     std::lock_guard<std::mutex> g(mu_);
@@ -121,8 +122,7 @@ void Server::loop() {
         const auto now = std::chrono::steady_clock::now();
         const std::vector<int> ids = eng_[d]->add_many(std::move(specs));
         for (size_t k = 0; k < ids.size(); ++k) {
-          Active a{which[k]->id, d + 2, which[k]->arrival, now};
-          a.arrival = which[k]->arrival;
+          Active a{which[k]->id, d + 2, which[k]->arrival, now, which[k]->bytes};
           active_[d][ids[k]] = a;
         }
       }
@@ -142,9 +142,8 @@ void Server::loop() {
           rr.done = true;
           {
             std::lock_guard<std::mutex> g(mu_);
-            const size_t before = eng_[d]->bytes_in_use();
             eng_[d]->release(pid);
-            in_use_ -= before - eng_[d]->bytes_in_use();
+            in_use_ -= a.bytes;  // what admission charged
             results_[a.id] = std::move(rr);
             ++completed_;
           }

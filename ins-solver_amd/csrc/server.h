@@ -61,6 +61,7 @@ class Server {
     int64_t id;
     int dim;
     std::chrono::steady_clock::time_point arrival, admitted;
+    size_t bytes;            // charged against the budget at admission
   };
   void loop();
 

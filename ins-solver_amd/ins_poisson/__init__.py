@@ -57,7 +57,7 @@ EXPORTED = [
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
-    "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_create", "ins_batch_destroy",
+    "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
     "ins_batch_release", "ins_batch_stats", "ins_server_create", "ins_server_destroy",
     "ins_server_submit", "ins_server_poll", "ins_server_wait", "ins_server_wait_all",
@@ -88,7 +88,8 @@ class SolveStats(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("cheb_order", C.c_int), ("pre_it", C.c_int),
                 ("post_it", C.c_int), ("eig_safety", C.c_double), ("cheb_min_factor", C.c_double),
-                ("max_iter_eig", C.c_int), ("coarse_iters", C.c_int), ("use_graph", C.c_int)]
+                ("max_iter_eig", C.c_int), ("coarse_iters", C.c_int), ("use_graph", C.c_int),
+                ("coarse", C.c_int)]
 
 
 class Problem(C.Structure):
@@ -145,6 +146,7 @@ def _load():
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
         "ins_ref_interp": [C.c_int, C.c_int, C.c_int, vp],
         "ins_batch_problem_bytes": [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.POINTER(i64)],
+        "ins_batch_amg_bytes": [C.c_int, i64, i64, C.c_int, C.POINTER(i64)],
         "ins_batch_create": [C.c_int, C.POINTER(BatchConfig), C.POINTER(vp)],
         "ins_batch_add": [vp, C.POINTER(Problem), C.c_int, vp],
         "ins_batch_iterate": [vp, C.c_int, C.POINTER(C.c_int)],
@@ -474,6 +476,12 @@ def problem_bytes(dim, n_cells, order, has_factor=False, precision=PRECISION_MIX
     b = C.c_int64()
     _check(lib.ins_batch_problem_bytes(dim, n_cells, order, int(bool(has_factor)), precision,
                                        C.byref(b)))
+    return b.value
+
+
+def amg_bytes(dim, n_nodes, n_cells, precision=PRECISION_MIXED):
+    b = C.c_int64()
+    _check(lib.ins_batch_amg_bytes(dim, n_nodes, n_cells, precision, C.byref(b)))
     return b.value
 
 

@@ -56,11 +56,13 @@ MIX3 = [(2, 1), (3, 2), (2, 3), (2, 4)]
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_batch_fp64_matches_oracle(ip, oracle, dim):
+@pytest.mark.parametrize("coarse", ["jacobi_cg", "amg"])
+def test_batch_fp64_matches_oracle(ip, oracle, dim, coarse):
     mix = MIX2 if dim == 2 else MIX3
     meshes = [(sq if dim == 2 else cube)(ip, n, 30 + i) for i, (n, _) in enumerate(mix)]
     probs = [sin_problem(ip, m, N) for m, (_, N) in zip(meshes, mix)]
-    _, _, res = run_batch(ip, dim, probs, precision=ip.PRECISION_FP64)
+    kind = ip.COARSE_AMG if coarse == "amg" else ip.COARSE_JACOBI_CG
+    _, _, res = run_batch(ip, dim, probs, precision=ip.PRECISION_FP64, coarse=kind)
     for m, (_, N), p, (rc, x, st) in zip(meshes, mix, probs, res):
         assert rc == ip.OK, st
         assert st["converged"] == 1 and st["rel_residual"] <= 1e-12
@@ -69,12 +71,13 @@ def test_batch_fp64_matches_oracle(ip, oracle, dim):
         xo, orc, it_o, _, _ = o.solve_pmg(b, [N, 1] if N > 1 else [1], rtol=1e-12, max_iter=400)
         assert orc == 0
         assert relL2(x, xo) < 1e-9, (N, relL2(x, xo))
-        if N > 1:
+        if N > 1 and coarse == "jacobi_cg":  # the oracle's p-multigrid has the Jacobi-CG coarse solve
             assert abs(st["iterations"] - it_o) <= 3, (N, st["iterations"], it_o)
 
 
 def test_batch_mixed_variable_neumann(ip, oracle):
-    """rho contrast 10 circle, pure Neumann with the mean-zero nullspace, FP32 V-cycle."""
+    """rho contrast 10 circle, pure Neumann with the mean-zero nullspace, FP32 V-cycle with the
+    AMG coarse solve (the default)."""
     probs, meshes = [], []
     for i, (n, N) in enumerate([(6, 2), (8, 3), (5, 4)]):
         m = sq(ip, n, 50 + i)
@@ -153,7 +156,8 @@ def test_batch_not_converged_release_and_reuse(ip):
     rc, x, st = b.result(ids[0])
     assert rc == ip.NOT_CONVERGED and st["iterations"] == 2 and st["converged"] == 0
     used = b.stats()["bytes_in_use"]
-    assert used == ip.problem_bytes(2, m.n_cells, 2, precision=ip.PRECISION_FP64)
+    arena = ip.problem_bytes(2, m.n_cells, 2, precision=ip.PRECISION_FP64)
+    assert arena < used <= arena + ip.amg_bytes(2, m.n_nodes, m.n_cells, ip.PRECISION_FP64)
     b.release(ids[0])
     assert b.stats()["bytes_in_use"] == 0
     with pytest.raises(ip.InsPoissonError):
@@ -171,7 +175,8 @@ def test_server_matches_batch_and_budget(ip):
     reqs3 = [sin_problem(ip, cube(ip, n, 300 + i), N, rtol=1e-10) for i, (n, N) in enumerate(MIX3)]
     _, _, ref2 = run_batch(ip, 2, reqs2, **cfg)
     _, _, ref3 = run_batch(ip, 3, reqs3, **cfg)
-    sizes = [ip.problem_bytes(r["mesh"].dim, r["mesh"].n_cells, r["order"]) for r in reqs2 + reqs3]
+    sizes = [ip.problem_bytes(r["mesh"].dim, r["mesh"].n_cells, r["order"]) +
+             ip.amg_bytes(r["mesh"].dim, r["mesh"].n_nodes, r["mesh"].n_cells) for r in reqs2 + reqs3]
     budget = max(sizes) * 2
     srv = ip.Server(budget, cfg2d=cfg, cfg3d=cfg)
     ids = []
@@ -187,7 +192,7 @@ def test_server_matches_batch_and_budget(ip):
     t.start()
     t.join()
     big = ip.Mesh.square(40, jitter=0.1, seed=1)
-    assert ip.problem_bytes(2, big.n_cells, 4) > budget
+    assert ip.problem_bytes(2, big.n_cells, 4) + ip.amg_bytes(2, big.n_nodes, big.n_cells) > budget
     rej = srv.submit(**sin_problem(ip, big, 4))
     srv.wait_all()
     rc, _, _, _, _ = srv.wait(rej)
@@ -207,3 +212,32 @@ def test_server_matches_batch_and_budget(ip):
     assert s["completed"] == len(ids)
     assert 0 < s["peak_bytes"] <= budget
     srv.close()
+
+
+def test_batch_amg_high_contrast(ip, oracle):
+    """rho contrast 1000 circle/sphere, pure Neumann: the AMG coarse solve converges in far fewer
+    lockstep iterations than Jacobi-CG(40) and reaches the oracle's solution."""
+    cases = [(2, sq(ip, 12, 71), 2), (3, cube(ip, 4, 72), 3)]
+    for dim, m, N in cases:
+        xyz = m.node_coords(N)
+        n = 12 if dim == 2 else 4
+        k = 1.0 / density_field(xyz, 1.0 / n, rho_in=1000.0)
+        f = np.random.default_rng(dim).standard_normal(xyz[0].shape)
+        b = mass_rhs(ip, m, N, f)
+        b -= b.mean()
+        prob = dict(mesh=m, order=N, rhs=b, factor=k, nullspace=True,
+                    bc_types=np.ones(m.n_bfaces, dtype=np.int32), rtol=1e-10, max_iter=500)
+        its = {}
+        for coarse in (ip.COARSE_AMG, ip.COARSE_JACOBI_CG):
+            _, _, res = run_batch(ip, dim, [prob], precision=ip.PRECISION_MIXED, coarse=coarse)
+            rc, x, st = res[0]
+            assert rc == ip.OK, (dim, coarse, st)
+            its[coarse] = st["iterations"]
+            if coarse == ip.COARSE_AMG:
+                o = oracle.Oracle(*m.arrays(), N)
+                o.set_bc_types(prob["bc_types"])
+                o.set_factor(k)
+                xo, orc, _, _ = o.solve_jacobi(b, rtol=1e-12, max_iter=200000, nullspace=True)
+                assert orc == 0
+                assert relL2(x, xo) < 1e-7, relL2(x, xo)
+        assert its[ip.COARSE_AMG] < its[ip.COARSE_JACOBI_CG], (dim, its)
