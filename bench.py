@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-mixed-leg", action="store_true")
     ap.add_argument("--skip-amg", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 256 MiB device copy 3 times first (counter calibration)")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -337,6 +339,14 @@ def main():
     import torch
     import ins_poisson as ip
 
+    if a.copy_probe:
+        # a plain copy of known size: 256 MiB read + 256 MiB written per launch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mesh, s, orders = cube_solver(ip, a.n, a.order, a.precision, a.rtol, a.coarse_iters)
     xyz = s.node_coords()
     u, f = manufactured_sin(xyz)
