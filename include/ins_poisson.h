@@ -225,6 +225,9 @@ typedef struct {
   int use_graph;          /* replay one captured HIP graph per lockstep iteration */
   int coarse;             /* INS_COARSE_AMG (default): Chebyshev on order 1 + vertex-space AMG;
                              INS_COARSE_JACOBI_CG: coarse_iters Jacobi-CG iterations on order 1 */
+  int tile;               /* cells per tile and block: 64, 128, 256, or 0 (default) = autotuned: the
+                             first batch of the process times each size on its problems and the
+                             fastest is kept for the dimension and precision */
 } ins_batch_config;
 
 typedef struct {
@@ -262,6 +265,9 @@ int ins_batch_result(ins_batch_t b, int id, double* x, ins_solve_stats* stats);
 /* Frees a finished problem's device memory; its id may be reused by a later add. */
 int ins_batch_release(ins_batch_t b, int id);
 int ins_batch_stats(ins_batch_t b, int64_t* bytes_in_use, int64_t* iterations_run, int* running);
+/* Tile size in use and, when it was autotuned, the timed cost (ms) of one smoothing step for
+ * tiles of 64, 128 and 256 cells (ms may be NULL; zeros when the size was given). */
+int ins_batch_tuning(ins_batch_t b, int* tile, double* ms);
 
 /* ---- serving: continuous batching under an HBM budget ----
  * Requests are queued by ins_server_submit from any thread. A worker thread

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 #include "dg_layout.h"
@@ -21,6 +23,10 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 int np_of(int dim, int N) { return dim == 2 ? (N + 1) * (N + 2) / 2 : (N + 1) * (N + 2) * (N + 3) / 6; }
 int nfp_of(int dim, int N) { return dim == 2 ? N + 1 : (N + 1) * (N + 2) / 2; }
 }  // namespace
+
+// Autotuned tile size per (dimension, V-cycle precision), shared by the engines of a process.
+std::mutex g_tune_mu;
+std::map<std::pair<int, bool>, BatchEngine::TuneResult> g_tune;
 
 struct BatchEngine::Prob {
   int slot = -1, N = 0, Np = 0, Np1 = 0, Nfp = 0, dim = 0;
@@ -90,6 +96,19 @@ BatchEngine::BatchEngine(const BatchConfig& cfg, hipStream_t stream) : cfg_(cfg)
   if (cfg_.dim != 2 && cfg_.dim != 3) throw InvalidArgument("batch dimension must be 2 or 3");
   if (cfg_.cheb_order < 1 || cfg_.cheb_order > 8 || cfg_.pre_it < 1 || cfg_.post_it < 0)
     throw InvalidArgument("invalid batch smoother settings");
+  if (cfg_.tile != 0 && cfg_.tile != 64 && cfg_.tile != 128 && cfg_.tile != 256)
+    throw InvalidArgument("tile must be 64, 128, 256 or 0 (autotuned)");
+  if (cfg_.tile) {
+    tile_ = cfg_.tile;
+    tuned_ = true;
+  } else {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find({cfg_.dim, cfg_.fp32});
+    if (it != g_tune.end()) {
+      tile_ = it->second.tile;
+      tuned_ = true;
+    }
+  }
   if (!stream_) {
     INSP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     own_stream_ = true;
@@ -162,7 +181,7 @@ void BatchEngine::upload_desc(int slot) {
 
 void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   int ntiles = 0;
-  for (int s : slots) ntiles += (int)(probs_[s]->S / kTile);
+  for (int s : slots) ntiles += (int)(probs_[s]->S / tile_);
   const int nprobs = (int)slots.size();
   auto grow = [](int need, int cap) {
     int c = std::max(cap, 64);
@@ -194,8 +213,8 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   for (int j = 0; j < nprobs; ++j) {
     const Prob& P = *probs_[slots[j]];
     v.h_probs[j] = slots[j];
-    v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / kTile));
-    for (int64_t c = 0; c < P.S; c += kTile) v.h_tiles[t++] = make_int2(slots[j], (int)c);
+    v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / tile_));
+    for (int64_t c = 0; c < P.S; c += tile_) v.h_tiles[t++] = make_int2(slots[j], (int)c);
   }
   v.h_counts[0] = ntiles;
   v.h_counts[1] = nprobs;
@@ -217,7 +236,8 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   v.bv.prob_cap = v.prob_cap;
   v.bv.partial = v.partial.p;
   v.bv.dim = cfg_.dim;
-  // AMG row tiles per level
+  v.bv.tile = tile_;
+  // AMG row tiles per level (256 rows each, independent of the cell tile)
   std::vector<int> per_level(kAmgLevels, 0);
   int levels = 0;
   for (int sl : slots) {
@@ -507,6 +527,7 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
     meshes.push_back(sp.mesh);
   }
   if (ids.empty()) return ids;
+  if (!tuned_) autotune(ids);
   // Chebyshev bounds: power iteration on D^-1 A for all new problems at once, on order N
   // and (AMG coarse solve) on order 1.
   set_view(*work_, ids);
@@ -548,6 +569,68 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
   INSP_HIP(hipStreamSynchronize(stream_));
   start(*work_, ids, false);
   return ids;
+}
+
+void BatchEngine::autotune(const std::vector<int>& slots) {
+  // Time the order-N smoothing step (trace, apply, Chebyshev update) of the V-cycle precision
+  // over these problems for each tile size; the operator dominates a lockstep iteration.
+  const int cand[3] = {64, 128, 256};
+  TuneResult r{256, {0, 0, 0}};
+  hipEvent_t e0, e1;
+  INSP_HIP(hipEventCreate(&e0));
+  INSP_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  for (int q = 0; q < 3; ++q) {
+    tile_ = cand[q];
+    set_view(*work_, slots);
+    const BatchView& b = work_->bv;
+    auto step = [&] {
+      if (cfg_.fp32) {
+        batch::trace<float>(b, tables_, BV_VX, BV_TRT, stream_);
+        batch::apply<float>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, stream_);
+        batch::cheb<float>(b, 0, 1, false, stream_);
+      } else {
+        batch::trace<double>(b, tables_, BV_VX, BV_TRT, stream_);
+        batch::apply<double>(b, tables_, 0, BV_VX, BV_TRT, BV_VAX, false, stream_);
+        batch::cheb<double>(b, 0, 1, false, stream_);
+      }
+    };
+    for (int w = 0; w < 2; ++w) step();
+    INSP_HIP(hipEventRecord(e0, stream_));
+    for (int w = 0; w < 5; ++w) step();
+    INSP_HIP(hipEventRecord(e1, stream_));
+    INSP_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    INSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    r.ms[q] = ms / 5.0;
+    if (r.ms[q] < best) {
+      best = r.ms[q];
+      r.tile = cand[q];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the timing runs wrote the V-cycle scratch vectors: clear them
+  for (int slot : slots) {
+    const Prob& P = *probs_[slot];
+    const size_t t = cfg_.fp32 ? 4 : 8;
+    INSP_HIP(hipMemsetAsync(P.desc.v[BV_VX], 0, (size_t)P.Np * P.S * t, stream_));
+    INSP_HIP(hipMemsetAsync(P.desc.v[BV_VAX], 0, (size_t)P.Np * P.S * t, stream_));
+    INSP_HIP(hipMemsetAsync(P.desc.v[BV_VD], 0, (size_t)P.Np * P.S * t, stream_));
+    INSP_HIP(hipMemsetAsync(P.desc.v[BV_TRT], 0, (size_t)(cfg_.dim + 1) * P.Nfp * P.S * t, stream_));
+  }
+  INSP_HIP(hipStreamSynchronize(stream_));
+  tile_ = r.tile;
+  tuned_ = true;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune.emplace(std::make_pair(cfg_.dim, cfg_.fp32), r);
+  tile_ = g_tune[{cfg_.dim, cfg_.fp32}].tile;  // an engine tuned first elsewhere wins
+}
+
+BatchEngine::TuneResult BatchEngine::last_tune(int dim, bool fp32) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find({dim, fp32});
+  return it == g_tune.end() ? TuneResult{0, {0, 0, 0}} : it->second;
 }
 
 void BatchEngine::chebyshev_coefficients(double hi, double min_factor, int n, double* c1, double* c2) {

@@ -26,7 +26,7 @@ namespace insp {
 
 using HostMeshPtr = std::shared_ptr<HostMesh>;
 
-constexpr int kTile = 256;
+constexpr int kTile = 256;  // stride granularity; cell tiles are 64, 128 or 256 cells
 
 // Vector fields of a problem, indexed by id in the descriptor.
 enum BatchVec : int {
@@ -108,6 +108,7 @@ struct BatchView {
   int tile_cap, prob_cap;        // grid sizes
   double* partial;               // 4 * tile_cap doubles
   int dim;
+  int tile;                      // cells per tile = threads per block (64, 128, 256)
   // AMG row tiles: level k lists 256-row tiles (slot, first row) of every problem that
   // has level k, at ltiles[k * ltile_cap + t], with lcount[k] entries.
   const int2* ltiles;
@@ -182,6 +183,7 @@ struct BatchConfig {
   bool amg = true;           // vertex-space AMG below order 1 instead of Jacobi-CG
   int amg_cheb = 2;          // Chebyshev degree on the AMG levels
   bool use_graph = true;     // replay one captured iteration
+  int tile = 0;              // cells per tile: 64, 128, 256, or 0 = autotuned
 };
 
 struct ProblemSpec {
@@ -233,6 +235,14 @@ class BatchEngine {
   void release(int id);      // frees the problem's device memory
   size_t bytes_in_use() const { return bytes_in_use_; }
   int64_t iterations_run() const { return iters_run_; }
+  int tile() const { return tile_; }
+  // Times the batched operator for each tile size on these problems (already added) and
+  // returns the fastest; the choice is cached per (dim, precision) for the process.
+  struct TuneResult {
+    int tile;
+    double ms[3];              // per tile size 64, 128, 256
+  };
+  static TuneResult last_tune(int dim, bool fp32);
 
  private:
   struct Prob;
@@ -251,6 +261,9 @@ class BatchEngine {
   void vcycle(View& v);
 
   BatchConfig cfg_;
+  int tile_ = 256;
+  bool tuned_ = false;
+  void autotune(const std::vector<int>& slots);
   hipStream_t stream_;
   bool own_stream_ = false;
   BatchTables tables_{};

@@ -31,6 +31,7 @@ struct Cell {
   bool live, valid;
 };
 
+template <int TB>
 __device__ __forceinline__ Cell locate(const BatchView& b) {
   Cell c;
   c.t = (int)xcd_block();
@@ -47,12 +48,12 @@ __device__ __forceinline__ Cell locate(const BatchView& b) {
   return c;
 }
 
-template <int K>
+template <int TB, int K>
 __device__ __forceinline__ void put_partials(const BatchView& b, const Cell& c, const double (&v)[K]) {
   if (!c.live) return;  // uniform per block
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const double s = block_sum1<kB>(v[k]);
+    const double s = block_sum1<TB>(v[k]);
     if (threadIdx.x == 0) b.partial[(int64_t)k * b.tile_cap + c.t] = s;
   }
 }
@@ -63,9 +64,9 @@ __device__ __forceinline__ const T* ctab(const BatchTables& tb, int N) {
   else return tb.C64[N];
 }
 
-template <int DIM, typename T>
-__global__ void __launch_bounds__(kB) kb_trace(BatchView b, BatchTables tb, int in, int tr) {
-  const Cell c = locate(b);
+template <int TB, int DIM, typename T>
+__global__ void __launch_bounds__(TB) kb_trace(BatchView b, BatchTables tb, int in, int tr) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const int prec = std::is_same<T, float>::value;
@@ -91,10 +92,10 @@ __global__ void __launch_bounds__(kB) kb_trace(BatchView b, BatchTables tb, int 
   }
 }
 
-template <int DIM, typename T>
-__global__ void __launch_bounds__(kB)
+template <int TB, int DIM, typename T>
+__global__ void __launch_bounds__(TB)
 kb_apply(BatchView b, BatchTables tb, int level, int in, int tr, int out, int dot) {
-  const Cell c = locate(b);
+  const Cell c = locate<TB>(b);
   double dv[1] = {0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -123,7 +124,7 @@ kb_apply(BatchView b, BatchTables tb, int level, int in, int tr, int out, int do
 #undef INSP_BAPPLY
     }
   }
-  if (dot) put_partials<1>(b, c, dv);
+  if (dot) put_partials<TB, 1>(b, c, dv);
 }
 
 __global__ void __launch_bounds__(kB) kb_finalize(BatchView b, int nvals, int slot0) {
@@ -142,10 +143,10 @@ __global__ void __launch_bounds__(kB) kb_finalize(BatchView b, int nvals, int sl
 template <typename T>
 __device__ __forceinline__ int rows_of(const BatchProb& P, int level) { return level ? P.Np1 : P.Np; }
 
-template <typename T>
-__global__ void __launch_bounds__(kB)
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB)
 kb_dot(BatchView b, int level, int fa, int fb, int fc, int fd, int nvals) {
-  const Cell c = locate(b);
+  const Cell c = locate<TB>(b);
   double v[2] = {0.0, 0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -158,13 +159,14 @@ kb_dot(BatchView b, int level, int fa, int fb, int fc, int fd, int nvals) {
       if (nvals > 1) v[1] += (double)cc[q] * (double)dd[q];
     }
   }
-  put_partials<2>(b, c, v);
+  put_partials<TB, 2>(b, c, v);
 }
 
 __device__ __forceinline__ double sdiv(double a, double b) { return b != 0.0 ? a / b : 0.0; }
 
-__global__ void __launch_bounds__(kB) kb_fcg_update_xr(BatchView b) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_fcg_update_xr(BatchView b) {
+  const Cell c = locate<TB>(b);
   double v[2] = {0.0, 0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -180,11 +182,12 @@ __global__ void __launch_bounds__(kB) kb_fcg_update_xr(BatchView b) {
       v[1] += ri * z[q];
     }
   }
-  put_partials<2>(b, c, v);
+  put_partials<TB, 2>(b, c, v);
 }
 
-__global__ void __launch_bounds__(kB) kb_fcg_update_p(BatchView b) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_fcg_update_p(BatchView b) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   // flexible (Polak-Ribiere) beta = r.(z - z_old) / (r_old.z_old)  (cg_compute_beta_pre_flex.h)
@@ -204,8 +207,9 @@ __global__ void kb_scal_copy(BatchView b, int dst, int src) {
   s[dst] = s[src];
 }
 
-__global__ void __launch_bounds__(kB) kb_residual(BatchView b) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_residual(BatchView b) {
+  const Cell c = locate<TB>(b);
   double v[1] = {0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -218,12 +222,12 @@ __global__ void __launch_bounds__(kB) kb_residual(BatchView b) {
       v[0] += t * t;
     }
   }
-  put_partials<1>(b, c, v);
+  put_partials<TB, 1>(b, c, v);
 }
 
-template <typename Tin, typename Tout>
-__global__ void __launch_bounds__(kB) kb_convert(BatchView b, int src, int dst) {
-  const Cell c = locate(b);
+template <int TB, typename Tin, typename Tout>
+__global__ void __launch_bounds__(TB) kb_convert(BatchView b, int src, int dst) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const Tin* x = (const Tin*)P.v[src];
@@ -231,9 +235,9 @@ __global__ void __launch_bounds__(kB) kb_convert(BatchView b, int src, int dst) 
   for (int i = 0; i < P.Np; ++i) y[i * P.S + c.e] = (Tout)x[i * P.S + c.e];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int level, int step, int zero_init) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_cheb(BatchView b, int level, int step, int zero_init) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const int prec = std::is_same<T, float>::value;
@@ -251,9 +255,9 @@ __global__ void __launch_bounds__(kB) kb_cheb(BatchView b, int level, int step, 
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_sub_from_b(BatchView b, int level) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_sub_from_b(BatchView b, int level) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const int rows = level ? P.Np1 : P.Np;
@@ -299,9 +303,9 @@ __global__ void __launch_bounds__(kB) kb_vertex_restrict(BatchView b) {
   ((T*)P.amg.lev[0].b)[c.r] = s;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_vertex_prolong(BatchView b) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_vertex_prolong(BatchView b) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   if (P.amg.nlev == 0) return;
@@ -409,9 +413,9 @@ __device__ __forceinline__ const T* ptab(const BatchTables& tb, int N, bool rest
   else return restrict_op ? tb.Pt64[N] : tb.P64[N];
 }
 
-template <int DIM, typename T>
-__global__ void __launch_bounds__(kB) kb_restrict(BatchView b, BatchTables tb) {
-  const Cell c = locate(b);
+template <int TB, int DIM, typename T>
+__global__ void __launch_bounds__(TB) kb_restrict(BatchView b, BatchTables tb) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const T* r = (const T*)P.v[BV_VAX];
@@ -428,9 +432,9 @@ __global__ void __launch_bounds__(kB) kb_restrict(BatchView b, BatchTables tb) {
   }
 }
 
-template <int DIM, typename T>
-__global__ void __launch_bounds__(kB) kb_prolong(BatchView b, BatchTables tb) {
-  const Cell c = locate(b);
+template <int TB, int DIM, typename T>
+__global__ void __launch_bounds__(TB) kb_prolong(BatchView b, BatchTables tb) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const T* cx = (const T*)P.v[BV_CX];
@@ -447,9 +451,9 @@ __global__ void __launch_bounds__(kB) kb_prolong(BatchView b, BatchTables tb) {
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_cg_init(BatchView b) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_cg_init(BatchView b) {
+  const Cell c = locate<TB>(b);
   double v[1] = {0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -467,12 +471,12 @@ __global__ void __launch_bounds__(kB) kb_cg_init(BatchView b) {
       v[0] += (double)ri * (double)zi;
     }
   }
-  put_partials<1>(b, c, v);
+  put_partials<TB, 1>(b, c, v);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_cg_xr(BatchView b, int num) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_cg_xr(BatchView b, int num) {
+  const Cell c = locate<TB>(b);
   double v[1] = {0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
@@ -490,12 +494,12 @@ __global__ void __launch_bounds__(kB) kb_cg_xr(BatchView b, int num) {
       v[0] += (double)ri * (double)zi;
     }
   }
-  put_partials<1>(b, c, v);
+  put_partials<TB, 1>(b, c, v);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kB) kb_cg_p(BatchView b, int num, int den) {
-  const Cell c = locate(b);
+template <int TB, typename T>
+__global__ void __launch_bounds__(TB) kb_cg_p(BatchView b, int num, int den) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const T beta = (T)sdiv(P.scal[num], P.scal[den]);
@@ -507,19 +511,21 @@ __global__ void __launch_bounds__(kB) kb_cg_p(BatchView b, int num, int den) {
   }
 }
 
-__global__ void __launch_bounds__(kB) kb_sum_d(BatchView b, int field) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_sum_d(BatchView b, int field) {
+  const Cell c = locate<TB>(b);
   double v[1] = {0.0};
   if (c.valid) {
     const BatchProb& P = b.P[c.slot];
     const double* x = (const double*)P.v[field];
     for (int i = 0; i < P.Np; ++i) v[0] += x[i * P.S + c.e];
   }
-  put_partials<1>(b, c, v);
+  put_partials<TB, 1>(b, c, v);
 }
 
-__global__ void __launch_bounds__(kB) kb_sub_mean_d(BatchView b, int field) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_sub_mean_d(BatchView b, int field) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   if (!P.nullspace) return;
@@ -528,8 +534,9 @@ __global__ void __launch_bounds__(kB) kb_sub_mean_d(BatchView b, int field) {
   for (int i = 0; i < P.Np; ++i) x[i * P.S + c.e] -= m;
 }
 
-__global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int level, int field) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_rand_d(BatchView b, int level, int field) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   double* x = (double*)P.v[field];
@@ -545,8 +552,9 @@ __global__ void __launch_bounds__(kB) kb_rand_d(BatchView b, int level, int fiel
   }
 }
 
-__global__ void __launch_bounds__(kB) kb_inv_apply_d(BatchView b, int level, int field) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_inv_apply_d(BatchView b, int level, int field) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const double* inv = (const double*)(level ? P.invC[0] : P.invF[0]);
@@ -555,8 +563,9 @@ __global__ void __launch_bounds__(kB) kb_inv_apply_d(BatchView b, int level, int
   for (int i = 0; i < rows; ++i) x[i * P.S + c.e] *= inv[i * P.S + c.e];
 }
 
-__global__ void __launch_bounds__(kB) kb_scale_d(BatchView b, int field, int slot) {
-  const Cell c = locate(b);
+template <int TB>
+__global__ void __launch_bounds__(TB) kb_scale_d(BatchView b, int field, int slot) {
+  const Cell c = locate<TB>(b);
   if (!c.valid) return;
   const BatchProb& P = b.P[c.slot];
   const double s = P.scal[slot] > 0 ? 1.0 / sqrt(P.scal[slot]) : 0.0;
@@ -568,6 +577,15 @@ dim3 tiles_grid(const BatchView& b) { return dim3((unsigned)b.tile_cap); }
 
 }  // namespace
 
+#define INSP_TB(...)                                          \
+  do {                                                        \
+    switch (b.tile) {                                         \
+      case 64: { constexpr int TB_ = 64; __VA_ARGS__; } break;   \
+      case 128: { constexpr int TB_ = 128; __VA_ARGS__; } break; \
+      default: { constexpr int TB_ = 256; __VA_ARGS__; } break;  \
+    }                                                         \
+  } while (0)
+
 #define INSP_DIMT(CALL)                     \
   do {                                      \
     if (b.dim == 2) { constexpr int D_ = 2; CALL; } \
@@ -576,15 +594,15 @@ dim3 tiles_grid(const BatchView& b) { return dim3((unsigned)b.tile_cap); }
 
 template <typename T>
 void trace(const BatchView& b, const BatchTables& t, int in, int tr, hipStream_t s) {
-  INSP_DIMT(hipLaunchKernelGGL((kb_trace<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t, in, tr));
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_trace<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t, in, tr)));
   check("kb_trace");
 }
 
 template <typename T>
 void apply(const BatchView& b, const BatchTables& t, int level, int in, int tr, int out, bool dot,
            hipStream_t s) {
-  INSP_DIMT(hipLaunchKernelGGL((kb_apply<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t, level, in, tr,
-                               out, (int)dot));
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_apply<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t, level, in, tr,
+                               out, (int)dot)));
   check("kb_apply");
 }
 
@@ -595,17 +613,17 @@ void finalize(const BatchView& b, int nvals, int slot, hipStream_t s) {
 
 template <typename T>
 void dot(const BatchView& b, int level, int a, int bb, int c, int d, int nvals, hipStream_t s) {
-  hipLaunchKernelGGL((kb_dot<T>), tiles_grid(b), dim3(kB), 0, s, b, level, a, bb, c, d, nvals);
+  INSP_TB(hipLaunchKernelGGL((kb_dot<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b, level, a, bb, c, d, nvals));
   check("kb_dot");
 }
 
 void fcg_update_xr(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL(kb_fcg_update_xr, tiles_grid(b), dim3(kB), 0, s, b);
+  INSP_TB(hipLaunchKernelGGL((kb_fcg_update_xr<TB_>), tiles_grid(b), dim3(TB_), 0, s, b));
   check("kb_fcg_update_xr");
 }
 
 void fcg_update_p(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL(kb_fcg_update_p, tiles_grid(b), dim3(kB), 0, s, b);
+  INSP_TB(hipLaunchKernelGGL((kb_fcg_update_p<TB_>), tiles_grid(b), dim3(TB_), 0, s, b));
   check("kb_fcg_update_p");
 }
 
@@ -615,31 +633,31 @@ void scal_copy(const BatchView& b, int dst, int src, hipStream_t s) {
 }
 
 void residual(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL(kb_residual, tiles_grid(b), dim3(kB), 0, s, b);
+  INSP_TB(hipLaunchKernelGGL((kb_residual<TB_>), tiles_grid(b), dim3(TB_), 0, s, b));
   check("kb_residual");
 }
 
 void to_t(const BatchView& b, bool fp32, int src_d, int dst, hipStream_t s) {
-  if (fp32) hipLaunchKernelGGL((kb_convert<double, float>), tiles_grid(b), dim3(kB), 0, s, b, src_d, dst);
-  else hipLaunchKernelGGL((kb_convert<double, double>), tiles_grid(b), dim3(kB), 0, s, b, src_d, dst);
+  if (fp32) INSP_TB(hipLaunchKernelGGL((kb_convert<TB_, double, float>), tiles_grid(b), dim3(TB_), 0, s, b, src_d, dst));
+  else INSP_TB(hipLaunchKernelGGL((kb_convert<TB_, double, double>), tiles_grid(b), dim3(TB_), 0, s, b, src_d, dst));
   check("kb_convert");
 }
 
 void from_t(const BatchView& b, bool fp32, int src, int dst_d, hipStream_t s) {
-  if (fp32) hipLaunchKernelGGL((kb_convert<float, double>), tiles_grid(b), dim3(kB), 0, s, b, src, dst_d);
-  else hipLaunchKernelGGL((kb_convert<double, double>), tiles_grid(b), dim3(kB), 0, s, b, src, dst_d);
+  if (fp32) INSP_TB(hipLaunchKernelGGL((kb_convert<TB_, float, double>), tiles_grid(b), dim3(TB_), 0, s, b, src, dst_d));
+  else INSP_TB(hipLaunchKernelGGL((kb_convert<TB_, double, double>), tiles_grid(b), dim3(TB_), 0, s, b, src, dst_d));
   check("kb_convert");
 }
 
 template <typename T>
 void cheb(const BatchView& b, int level, int step, bool zero_init, hipStream_t s) {
-  hipLaunchKernelGGL((kb_cheb<T>), tiles_grid(b), dim3(kB), 0, s, b, level, step, (int)zero_init);
+  INSP_TB(hipLaunchKernelGGL((kb_cheb<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b, level, step, (int)zero_init));
   check("kb_cheb");
 }
 
 template <typename T>
 void sub_from_b(const BatchView& b, int level, hipStream_t s) {
-  hipLaunchKernelGGL((kb_sub_from_b<T>), tiles_grid(b), dim3(kB), 0, s, b, level);
+  INSP_TB(hipLaunchKernelGGL((kb_sub_from_b<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b, level));
   check("kb_sub_from_b");
 }
 
@@ -651,7 +669,7 @@ void vertex_restrict(const BatchView& b, hipStream_t s) {
 
 template <typename T>
 void vertex_prolong(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL((kb_vertex_prolong<T>), tiles_grid(b), dim3(kB), 0, s, b);
+  INSP_TB(hipLaunchKernelGGL((kb_vertex_prolong<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b));
   check("kb_vertex_prolong");
 }
 
@@ -688,56 +706,56 @@ void amg_dense(const BatchView& b, int k, hipStream_t s) {
 
 template <typename T>
 void restrict_(const BatchView& b, const BatchTables& t, hipStream_t s) {
-  INSP_DIMT(hipLaunchKernelGGL((kb_restrict<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t));
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_restrict<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t)));
   check("kb_restrict");
 }
 
 template <typename T>
 void prolong(const BatchView& b, const BatchTables& t, hipStream_t s) {
-  INSP_DIMT(hipLaunchKernelGGL((kb_prolong<D_, T>), tiles_grid(b), dim3(kB), 0, s, b, t));
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_prolong<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t)));
   check("kb_prolong");
 }
 
 template <typename T>
 void cg_init(const BatchView& b, hipStream_t s) {
-  hipLaunchKernelGGL((kb_cg_init<T>), tiles_grid(b), dim3(kB), 0, s, b);
+  INSP_TB(hipLaunchKernelGGL((kb_cg_init<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b));
   check("kb_cg_init");
 }
 
 template <typename T>
 void cg_xr(const BatchView& b, int num, hipStream_t s) {
-  hipLaunchKernelGGL((kb_cg_xr<T>), tiles_grid(b), dim3(kB), 0, s, b, num);
+  INSP_TB(hipLaunchKernelGGL((kb_cg_xr<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b, num));
   check("kb_cg_xr");
 }
 
 template <typename T>
 void cg_p(const BatchView& b, int num, int den, hipStream_t s) {
-  hipLaunchKernelGGL((kb_cg_p<T>), tiles_grid(b), dim3(kB), 0, s, b, num, den);
+  INSP_TB(hipLaunchKernelGGL((kb_cg_p<TB_, T>), tiles_grid(b), dim3(TB_), 0, s, b, num, den));
   check("kb_cg_p");
 }
 
 void sum_d(const BatchView& b, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_sum_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  INSP_TB(hipLaunchKernelGGL((kb_sum_d<TB_>), tiles_grid(b), dim3(TB_), 0, s, b, field));
   check("kb_sum_d");
 }
 
 void sub_mean_d(const BatchView& b, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_sub_mean_d, tiles_grid(b), dim3(kB), 0, s, b, field);
+  INSP_TB(hipLaunchKernelGGL((kb_sub_mean_d<TB_>), tiles_grid(b), dim3(TB_), 0, s, b, field));
   check("kb_sub_mean_d");
 }
 
 void rand_d(const BatchView& b, int level, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_rand_d, tiles_grid(b), dim3(kB), 0, s, b, level, field);
+  INSP_TB(hipLaunchKernelGGL((kb_rand_d<TB_>), tiles_grid(b), dim3(TB_), 0, s, b, level, field));
   check("kb_rand_d");
 }
 
 void inv_apply_d(const BatchView& b, int level, int field, hipStream_t s) {
-  hipLaunchKernelGGL(kb_inv_apply_d, tiles_grid(b), dim3(kB), 0, s, b, level, field);
+  INSP_TB(hipLaunchKernelGGL((kb_inv_apply_d<TB_>), tiles_grid(b), dim3(TB_), 0, s, b, level, field));
   check("kb_inv_apply_d");
 }
 
 void scale_d(const BatchView& b, int field, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(kb_scale_d, tiles_grid(b), dim3(kB), 0, s, b, field, slot);
+  INSP_TB(hipLaunchKernelGGL((kb_scale_d<TB_>), tiles_grid(b), dim3(TB_), 0, s, b, field, slot));
   check("kb_scale_d");
 }
 

@@ -86,6 +86,9 @@ insp::BatchConfig to_batch_config(int dim, const ins_batch_config* c) {
   if (c->coarse != INS_COARSE_JACOBI_CG && c->coarse != INS_COARSE_AMG)
     throw insp::InvalidArgument("unknown coarse solve");
   b.amg = c->coarse == INS_COARSE_AMG;
+  if (c->tile != 0 && c->tile != 64 && c->tile != 128 && c->tile != 256)
+    throw insp::InvalidArgument("tile must be 64, 128, 256 or 0");
+  b.tile = c->tile;
   return b;
 }
 
@@ -584,6 +587,20 @@ void ins_batch_config_default(ins_batch_config* c) {
   c->coarse_iters = d.coarse_iters;
   c->use_graph = d.use_graph ? 1 : 0;
   c->coarse = d.amg ? INS_COARSE_AMG : INS_COARSE_JACOBI_CG;
+  c->tile = d.tile;
+}
+
+int ins_batch_tuning(ins_batch_t b, int* tile, double* ms) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (tile) *tile = e.tile();
+    if (ms) {
+      const insp::BatchEngine::TuneResult r = insp::BatchEngine::last_tune(b->cfg.dim, b->cfg.fp32);
+      const bool tuned = b->cfg.tile == 0 && r.tile != 0;
+      for (int i = 0; i < 3; ++i) ms[i] = tuned ? r.ms[i] : 0.0;
+    }
+    return INS_OK;
+  });
 }
 
 int ins_batch_amg_bytes(int dim, int64_t n_nodes, int64_t n_cells, int precision, int64_t* bytes) {

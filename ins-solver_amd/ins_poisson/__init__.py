@@ -59,7 +59,7 @@ EXPORTED = [
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
-    "ins_batch_release", "ins_batch_stats", "ins_server_create", "ins_server_destroy",
+    "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_server_create", "ins_server_destroy",
     "ins_server_submit", "ins_server_poll", "ins_server_wait", "ins_server_wait_all",
     "ins_server_stats", "ins_poisson_last_error",
 ]
@@ -89,7 +89,7 @@ class BatchConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("cheb_order", C.c_int), ("pre_it", C.c_int),
                 ("post_it", C.c_int), ("eig_safety", C.c_double), ("cheb_min_factor", C.c_double),
                 ("max_iter_eig", C.c_int), ("coarse_iters", C.c_int), ("use_graph", C.c_int),
-                ("coarse", C.c_int)]
+                ("coarse", C.c_int), ("tile", C.c_int)]
 
 
 class Problem(C.Structure):
@@ -154,6 +154,7 @@ def _load():
         "ins_batch_result": [vp, C.c_int, vp, C.POINTER(SolveStats)],
         "ins_batch_release": [vp, C.c_int],
         "ins_batch_stats": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int)],
+        "ins_batch_tuning": [vp, C.POINTER(C.c_int), dp],
         "ins_server_create": [i64, C.POINTER(BatchConfig), C.POINTER(BatchConfig), C.POINTER(vp)],
         "ins_server_submit": [vp, C.POINTER(Problem), C.POINTER(i64)],
         "ins_server_poll": [vp, i64, C.POINTER(C.c_int)],
@@ -565,6 +566,13 @@ class Batch:
         b, it, run = C.c_int64(), C.c_int64(), C.c_int()
         _check(lib.ins_batch_stats(self._h, C.byref(b), C.byref(it), C.byref(run)))
         return {"bytes_in_use": b.value, "iterations_run": it.value, "running": run.value}
+
+    def tuning(self):
+        """(tile, {tile: ms per smoothing step}) of the autotuner (empty dict when fixed)."""
+        t, ms = C.c_int(), (C.c_double * 3)()
+        _check(lib.ins_batch_tuning(self._h, C.byref(t), ms))
+        times = {sz: ms[i] for i, sz in enumerate((64, 128, 256)) if ms[i] > 0}
+        return t.value, times
 
 
 class Server:

@@ -98,7 +98,7 @@ def test_mesh_node_coords_match_oracle(ip, oracle, dim, N):
 def test_batch_host_side_entry_points(ip):
     c = ip.batch_config()
     assert (c.precision, c.cheb_order, c.pre_it, c.post_it, c.max_iter_eig, c.coarse_iters,
-            c.use_graph, c.coarse) == (ip.PRECISION_MIXED, 3, 1, 1, 15, 40, 1, ip.COARSE_AMG)
+            c.use_graph, c.coarse, c.tile) == (ip.PRECISION_MIXED, 3, 1, 1, 15, 40, 1, ip.COARSE_AMG, 0)
     assert abs(c.eig_safety - 1.1) < 1e-15 and abs(c.cheb_min_factor - 0.1) < 1e-15
     # arena bytes grow with cells, order, a coefficient field and FP64 V-cycle storage
     b = lambda *a, **k: ip.problem_bytes(*a, **k)
@@ -123,6 +123,8 @@ def test_batch_host_side_entry_points(ip):
     assert ip.lib.ins_server_poll(None, 0, ctypes.byref(n)) == ip.ERR_INVALID
     with pytest.raises(TypeError):
         ip.batch_config(no_such_option=1)
+    bad = ip.batch_config(tile=96)
+    assert ip.lib.ins_batch_create(2, ctypes.byref(bad), ctypes.byref(h)) == ip.ERR_INVALID
     bad = ip.batch_config(coarse=5)
     assert ip.lib.ins_batch_create(2, ctypes.byref(bad), ctypes.byref(h)) == ip.ERR_INVALID
     assert ip.amg_bytes(3, 1000, 5000) > ip.amg_bytes(3, 500, 2500) > 0

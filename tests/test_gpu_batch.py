@@ -241,3 +241,21 @@ def test_batch_amg_high_contrast(ip, oracle):
                 assert orc == 0
                 assert relL2(x, xo) < 1e-7, relL2(x, xo)
         assert its[ip.COARSE_AMG] < its[ip.COARSE_JACOBI_CG], (dim, its)
+
+
+def test_batch_tile_sizes_and_autotuner(ip):
+    """Every tile size gives the same solution to rounding; the autotuner times all three."""
+    probs = [sin_problem(ip, sq(ip, n, 120 + n), N, rtol=1e-11) for n, N in MIX2]
+    xs = {}
+    for tile in (64, 128, 256):
+        b, ids, res = run_batch(ip, 2, probs, precision=ip.PRECISION_FP64, tile=tile)
+        assert b.tuning() == (tile, {})
+        assert all(r[0] == ip.OK for r in res)
+        xs[tile] = [r[1] for r in res]
+    for q in range(len(probs)):
+        for tile in (64, 128):
+            assert relL2(xs[tile][q], xs[256][q]) < 1e-9
+    b, _, res = run_batch(ip, 3, [sin_problem(ip, cube(ip, 3, 5), 3, rtol=1e-8)])
+    tile, times = b.tuning()
+    assert tile in (64, 128, 256) and sorted(times) == [64, 128, 256]
+    assert times[tile] == min(times.values())
