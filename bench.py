@@ -228,6 +228,8 @@ def batch_leg(ip, count):
     bt.run()
     t2 = time.perf_counter()
     res = [bt.result(i) for i in ids]
+    setup_parts = bt.setup_seconds()
+    tile, tile_ms = bt.tuning()
     its = [r[2]["iterations"] for r in res]
     ok = all(r[0] == ip.OK for r in res)
     worst = max(r[2]["rel_residual"] for r in res)
@@ -238,7 +240,8 @@ def batch_leg(ip, count):
                        "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
                        "converged": bool(plain_ok)},
         "batched_mixed": {"gdofs": dofs / (t2 - t1) * 1e-9, "seconds": t2 - t1,
-                          "setup_seconds": t1 - t0, "solves_per_s": count / (t2 - t1),
+                          "setup_seconds": t1 - t0, "setup_parts_s": setup_parts,
+                          "tile": tile, "tile_ms": tile_ms, "solves_per_s": count / (t2 - t1),
                           "gdofs_incl_setup": dofs / (t2 - t0) * 1e-9,
                           "lockstep_iterations": st["iterations_run"],
                           "mean_iterations": float(np.mean(its)), "max_rel_residual": worst,

@@ -268,6 +268,10 @@ int ins_batch_stats(ins_batch_t b, int64_t* bytes_in_use, int64_t* iterations_ru
 /* Tile size in use and, when it was autotuned, the timed cost (ms) of one smoothing step for
  * tiles of 64, 128 and 256 cells (ms may be NULL; zeros when the size was given). */
 int ins_batch_tuning(ins_batch_t b, int* tile, double* ms);
+/* Wall seconds spent setting problems up so far (5 values): upload with penalties and
+ * diagonals, Chebyshev bounds, AMG coarse space, first residual and preconditioner, and the
+ * host hierarchy construction inside the AMG part. */
+int ins_batch_setup_seconds(ins_batch_t b, double* t5);
 
 /* ---- serving: continuous batching under an HBM budget ----
  * Requests are queued by ins_server_submit from any thread. A worker thread

@@ -5,8 +5,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <exception>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 
 #include "dg_layout.h"
@@ -378,8 +381,15 @@ void BatchEngine::run_iteration(View& v) {
   batch::scal_copy(b, BS_RZ, BS_RZN, s);
 }
 
+namespace {
+double since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
 std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
   const int d = cfg_.dim, Nf = d + 1, NK = d == 2 ? 3 : 6;
+  auto tp = std::chrono::steady_clock::now();
   std::vector<int> ids;
   std::vector<HostMeshPtr> meshes;
   for (auto& sp : specs) {
@@ -527,6 +537,8 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
     meshes.push_back(sp.mesh);
   }
   if (ids.empty()) return ids;
+  setup_s_[0] += since(tp);
+  tp = std::chrono::steady_clock::now();
   if (!tuned_) autotune(ids);
   // Chebyshev bounds: power iteration on D^-1 A for all new problems at once, on order N
   // and (AMG coarse solve) on order 1.
@@ -555,8 +567,11 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
                              level ? P.desc.amg.oc1 : P.desc.c1, level ? P.desc.amg.oc2 : P.desc.c2);
     }
   }
-  if (cfg_.amg)
-    for (size_t q = 0; q < ids.size(); ++q) setup_amg(ids[q], *meshes[q]);
+  setup_s_[1] += since(tp);
+  tp = std::chrono::steady_clock::now();
+  if (cfg_.amg) setup_amg_all(ids, meshes);
+  setup_s_[2] += since(tp);
+  tp = std::chrono::steady_clock::now();
   for (int slot : ids)
     INSP_HIP(hipMemcpyAsync(d_desc_ + slot, &probs_[slot]->desc, sizeof(BatchProb), hipMemcpyHostToDevice,
                             stream_));
@@ -568,6 +583,7 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
   }
   INSP_HIP(hipStreamSynchronize(stream_));
   start(*work_, ids, false);
+  setup_s_[3] += since(tp);
   return ids;
 }
 
@@ -659,63 +675,161 @@ size_t BatchEngine::amg_bytes_estimate(int dim, int64_t n_nodes, int64_t cells, 
   return b;
 }
 
-void BatchEngine::setup_amg(int slot, const HostMesh& m) {
-  Prob& P = *probs_[slot];
+namespace {
+// Runs fn(i) for i < n on up to 16 host threads.
+template <typename F>
+void parallel_for(size_t n, F&& fn) {
+  const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({n, hw, 16});
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (size_t i; (i = next++) < n;) {
+        try {
+          fn(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(mu);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+}  // namespace
+
+struct BatchEngine::ProbeScratch {
+  DBuf<int> cv, vptr, vlist, gptr, gcol, colour;
+  DBuf<double> val, xv, yv;
+};
+
+struct BatchEngine::AmgPrep {
+  std::vector<int> vptr, vlist, cv, colour;
+  HostCsr g;
+  int ncol = 0;
+  AmgHierarchy H;
+};
+
+void BatchEngine::setup_amg_all(const std::vector<int>& slots, const std::vector<HostMeshPtr>& meshes) {
+  std::vector<AmgPrep> prep(slots.size());
+  // host: vertex maps, vertex graph, distance-2 colouring
+  auto t0 = std::chrono::steady_clock::now();
+  parallel_for(slots.size(), [&](size_t q) { amg_prepare(slots[q], *meshes[q], prep[q]); });
+  // device: probe each problem's vertex operator, scratch sized for the largest
+  ProbeScratch ps;
+  size_t ncv = 1, nvp = 1, nvl = 1, ngp = 1, ngc = 1;
+  for (const AmgPrep& A : prep) {
+    ncv = std::max(ncv, A.cv.size());
+    nvp = std::max(nvp, A.vptr.size());
+    nvl = std::max(nvl, A.vlist.size());
+    ngp = std::max(ngp, A.g.ptr.size());
+    ngc = std::max(ngc, A.g.col.size());
+  }
+  ps.cv.alloc(ncv, false);
+  ps.vptr.alloc(nvp, false);
+  ps.vlist.alloc(nvl, false);
+  ps.gptr.alloc(ngp, false);
+  ps.gcol.alloc(ngc, false);
+  ps.colour.alloc(nvp, false);
+  ps.val.alloc(ngc, false);
+  ps.xv.alloc(nvp, false);
+  ps.yv.alloc(nvp, false);
+  for (size_t q = 0; q < slots.size(); ++q) amg_probe(slots[q], prep[q], ps);
+  // host: hierarchies
+  const auto th = std::chrono::steady_clock::now();
+  parallel_for(slots.size(), [&](size_t q) { amg_build(slots[q], prep[q]); });
+  setup_s_[4] += since(th);
+  for (size_t q = 0; q < slots.size(); ++q) amg_upload(slots[q], prep[q]);
+  (void)t0;
+}
+
+void BatchEngine::amg_prepare(int slot, const HostMesh& m, AmgPrep& A) {
+  const Prob& P = *probs_[slot];
   const int d1 = cfg_.dim + 1, nv = (int)m.n_nodes;
   const int64_t E = P.E, S = P.S;
-  const size_t t = cfg_.fp32 ? 4 : 8;
   // vertex maps: order-1 nodes are the cell vertices in cell2nodes order
-  std::vector<int> vptr((size_t)nv + 1, 0), vlist((size_t)E * d1), cv((size_t)d1 * S, 0);
+  A.vptr.assign((size_t)nv + 1, 0);
+  A.vlist.resize((size_t)E * d1);
+  A.cv.assign((size_t)d1 * S, 0);
   for (int64_t e = 0; e < E; ++e)
-    for (int i = 0; i < d1; ++i) ++vptr[(size_t)m.cell2nodes[e * d1 + i] + 1];
-  for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
-  {
-    std::vector<int> pos(vptr.begin(), vptr.end() - 1);
-    for (int64_t e = 0; e < E; ++e)
-      for (int i = 0; i < d1; ++i) {
-        const int v = m.cell2nodes[e * d1 + i];
-        vlist[pos[v]++] = (int)(i * S + e);
-        cv[(size_t)i * S + e] = v;
-      }
-  }
-  // probe A_v = P_v^T A_1 P_v with distance-2 colour classes (double, on this problem alone)
-  HostCsr g = vertex_graph(m);
-  int ncol = 0;
-  const std::vector<int> colour = distance2_colouring(g, &ncol);
-  DBuf<int> d_cv, d_vptr, d_vlist, gptr, gcol, dcol;
-  d_cv.upload(cv.data(), cv.size());
-  d_vptr.upload(vptr.data(), vptr.size());
-  d_vlist.upload(vlist.data(), vlist.size());
-  gptr.upload(g.ptr.data(), g.ptr.size());
-  gcol.upload(g.col.data(), g.col.size());
-  dcol.upload(colour.data(), colour.size());
-  DBuf<double> val, xv, yv;
-  val.alloc(g.col.size());
-  xv.alloc((size_t)nv);
-  yv.alloc((size_t)nv);
+    for (int i = 0; i < d1; ++i) ++A.vptr[(size_t)m.cell2nodes[e * d1 + i] + 1];
+  for (int v = 0; v < nv; ++v) A.vptr[v + 1] += A.vptr[v];
+  std::vector<int> pos(A.vptr.begin(), A.vptr.end() - 1);
+  for (int64_t e = 0; e < E; ++e)
+    for (int i = 0; i < d1; ++i) {
+      const int v = m.cell2nodes[e * d1 + i];
+      A.vlist[pos[v]++] = (int)(i * S + e);
+      A.cv[(size_t)i * S + e] = v;
+    }
+  A.g = vertex_graph(m);
+  A.colour = distance2_colouring(A.g, &A.ncol);
+}
+
+void BatchEngine::amg_probe(int slot, AmgPrep& A, ProbeScratch& ps) {
+  // A_v = P_v^T A_1 P_v by distance-2 colour classes (double, this problem alone)
+  Prob& P = *probs_[slot];
+  const int d1 = cfg_.dim + 1, nv = (int)A.vptr.size() - 1;
+  const int64_t E = P.E, S = P.S;
+  auto h2d = [&](void* d, const void* h, size_t bytes) {
+    INSP_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
+  };
+  h2d(ps.cv.p, A.cv.data(), A.cv.size() * 4);
+  h2d(ps.vptr.p, A.vptr.data(), A.vptr.size() * 4);
+  h2d(ps.vlist.p, A.vlist.data(), A.vlist.size() * 4);
+  h2d(ps.gptr.p, A.g.ptr.data(), A.g.ptr.size() * 4);
+  h2d(ps.gcol.p, A.g.col.data(), A.g.col.size() * 4);
+  h2d(ps.colour.p, A.colour.data(), A.colour.size() * 4);
+  struct {
+    int* p;
+  } d_cv{ps.cv.p}, d_vptr{ps.vptr.p}, d_vlist{ps.vlist.p}, gptr{ps.gptr.p}, gcol{ps.gcol.p},
+      dcol{ps.colour.p};
+  DBuf<double>& val = ps.val;
+  DBuf<double>& xv = ps.xv;
+  DBuf<double>& yv = ps.yv;
   set_view(*work_, {slot});
   double* u = (double*)P.desc.v[BV_R];
   double* y = (double*)P.desc.v[BV_Z];
-  for (int c = 0; c < ncol; ++c) {
+  for (int c = 0; c < A.ncol; ++c) {
     amg::probe_fill(nv, dcol.p, c, xv.p, stream_);
     amg::vertex_prolong<double>(d1, E, S, d_cv.p, xv.p, u, false, stream_);
     batch::apply<double>(work_->bv, tables_, 1, BV_R, BV_TRD, BV_Z, false, stream_);
     amg::vertex_restrict<double>(nv, d_vptr.p, d_vlist.p, y, yv.p, stream_);
     amg::probe_extract(nv, gptr.p, gcol.p, dcol.p, c, yv.p, val.p, stream_);
   }
-  INSP_HIP(hipMemcpyAsync(g.val.data(), val.p, g.val.size() * sizeof(double), hipMemcpyDeviceToHost,
+  INSP_HIP(hipMemcpyAsync(A.g.val.data(), val.p, A.g.val.size() * sizeof(double), hipMemcpyDeviceToHost,
                           stream_));
   INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchEngine::amg_build(int slot, AmgPrep& A) {
+  const Prob& P = *probs_[slot];
   {  // the hierarchy is built from the symmetric part
-    HostCsr gt = csr_transpose(g);
-    for (size_t k = 0; k < g.val.size(); ++k) g.val[k] = 0.5 * (g.val[k] + gt.val[k]);
+    HostCsr gt = csr_transpose(A.g);
+    for (size_t k = 0; k < A.g.val.size(); ++k) A.g.val[k] = 0.5 * (A.g.val[k] + gt.val[k]);
   }
   AmgOptions opt;
   opt.max_coarse = kAmgMaxCoarse;
   opt.max_levels = kAmgLevels;
   opt.cheb_order = cfg_.amg_cheb;
-  const AmgHierarchy H = amg_setup(std::move(g), opt, P.nullspace);
-  if (H.n_coarse > kAmgMaxCoarse) throw std::runtime_error("AMG coarsening stalled above the dense size");
+  A.H = amg_setup(std::move(A.g), opt, P.nullspace);
+  if (A.H.n_coarse > kAmgMaxCoarse) throw std::runtime_error("AMG coarsening stalled above the dense size");
+}
+
+void BatchEngine::amg_upload(int slot, AmgPrep& A) {
+  Prob& P = *probs_[slot];
+  const size_t t = cfg_.fp32 ? 4 : 8;
+  const int nv = (int)A.vptr.size() - 1;
+  const AmgHierarchy& H = A.H;
+  const std::vector<int>& cv = A.cv;
+  const std::vector<int>& vptr = A.vptr;
+  const std::vector<int>& vlist = A.vlist;
   // one device allocation for the maps and the hierarchy
   auto a = [](size_t bytes) { return align256(bytes); };
   size_t bytes = a(cv.size() * 4) + a(vptr.size() * 4) + a(vlist.size() * 4);

@@ -236,6 +236,9 @@ class BatchEngine {
   size_t bytes_in_use() const { return bytes_in_use_; }
   int64_t iterations_run() const { return iters_run_; }
   int tile() const { return tile_; }
+  // Wall seconds of problem set-up so far: upload + penalties/diagonals, Chebyshev bounds,
+  // AMG (probing + host hierarchy), start; and the host-hierarchy share of the AMG part.
+  const double* setup_seconds() const { return setup_s_; }
   // Times the batched operator for each tile size on these problems (already added) and
   // returns the fastest; the choice is cached per (dim, precision) for the process.
   struct TuneResult {
@@ -254,7 +257,13 @@ class BatchEngine {
   void run_iteration(View& v);
   void start(View& v, const std::vector<int>& slots, bool restart);
   void finish(int slot, ProbStatus st);
-  void setup_amg(int slot, const HostMesh& m);
+  struct AmgPrep;
+  struct ProbeScratch;
+  void setup_amg_all(const std::vector<int>& slots, const std::vector<HostMeshPtr>& meshes);
+  void amg_prepare(int slot, const HostMesh& m, AmgPrep& A);  // host, thread-safe
+  void amg_probe(int slot, AmgPrep& A, ProbeScratch& ps);     // device
+  void amg_build(int slot, AmgPrep& A);                       // host, thread-safe
+  void amg_upload(int slot, AmgPrep& A);                      // device
   static void chebyshev_coefficients(double hi, double min_factor, int n, double* c1, double* c2);
   void read_scal(const std::vector<int>& slots);
   template <typename T>
@@ -262,6 +271,7 @@ class BatchEngine {
 
   BatchConfig cfg_;
   int tile_ = 256;
+  double setup_s_[5] = {0, 0, 0, 0, 0};
   bool tuned_ = false;
   void autotune(const std::vector<int>& slots);
   hipStream_t stream_;

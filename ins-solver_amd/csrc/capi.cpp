@@ -590,6 +590,15 @@ void ins_batch_config_default(ins_batch_config* c) {
   c->tile = d.tile;
 }
 
+int ins_batch_setup_seconds(ins_batch_t b, double* t5) {
+  return guarded([&] {
+    auto& e = B(b);
+    if (!t5) throw insp::InvalidArgument("null output");
+    std::copy(e.setup_seconds(), e.setup_seconds() + 5, t5);
+    return INS_OK;
+  });
+}
+
 int ins_batch_tuning(ins_batch_t b, int* tile, double* ms) {
   return guarded([&] {
     auto& e = B(b);

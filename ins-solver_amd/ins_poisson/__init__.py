@@ -59,7 +59,7 @@ EXPORTED = [
     "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
-    "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_server_create", "ins_server_destroy",
+    "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
     "ins_server_submit", "ins_server_poll", "ins_server_wait", "ins_server_wait_all",
     "ins_server_stats", "ins_poisson_last_error",
 ]
@@ -155,6 +155,7 @@ def _load():
         "ins_batch_release": [vp, C.c_int],
         "ins_batch_stats": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int)],
         "ins_batch_tuning": [vp, C.POINTER(C.c_int), dp],
+        "ins_batch_setup_seconds": [vp, dp],
         "ins_server_create": [i64, C.POINTER(BatchConfig), C.POINTER(BatchConfig), C.POINTER(vp)],
         "ins_server_submit": [vp, C.POINTER(Problem), C.POINTER(i64)],
         "ins_server_poll": [vp, i64, C.POINTER(C.c_int)],
@@ -566,6 +567,11 @@ class Batch:
         b, it, run = C.c_int64(), C.c_int64(), C.c_int()
         _check(lib.ins_batch_stats(self._h, C.byref(b), C.byref(it), C.byref(run)))
         return {"bytes_in_use": b.value, "iterations_run": it.value, "running": run.value}
+
+    def setup_seconds(self):
+        t = (C.c_double * 5)()
+        _check(lib.ins_batch_setup_seconds(self._h, t))
+        return dict(zip(("upload", "eigen", "amg", "start", "amg_host"), list(t)))
 
     def tuning(self):
         """(tile, {tile: ms per smoothing step}) of the autotuner (empty dict when fixed)."""
