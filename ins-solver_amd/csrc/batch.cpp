@@ -61,6 +61,7 @@ struct BatchEngine::View {
   int2* h_ltiles = nullptr;
   int* h_lcount = nullptr;
   int levels = 0;                // deepest AMG hierarchy in the view
+  int ocap[5] = {0, 0, 0, 0, 0};  // grid per order
   int2* h_tiles = nullptr;
   int* h_probs = nullptr;
   int2* h_ranges = nullptr;
@@ -207,17 +208,24 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   }
   if (!v.ranges.p) {
     v.ranges.alloc(max_slots_);
-    v.counts.alloc(2);
+    v.counts.alloc(12);
     INSP_HIP(hipHostMalloc((void**)&v.h_ranges, sizeof(int2) * max_slots_, hipHostMallocDefault));
-    INSP_HIP(hipHostMalloc((void**)&v.h_counts, sizeof(int) * 2, hipHostMallocDefault));
+    INSP_HIP(hipHostMalloc((void**)&v.h_counts, sizeof(int) * 12, hipHostMallocDefault));
     std::memset(v.h_ranges, 0, sizeof(int2) * max_slots_);
   }
+  // tiles grouped by order, so each order's kernels run over one contiguous range
   int t = 0;
-  for (int j = 0; j < nprobs; ++j) {
-    const Prob& P = *probs_[slots[j]];
-    v.h_probs[j] = slots[j];
-    v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / tile_));
-    for (int64_t c = 0; c < P.S; c += tile_) v.h_tiles[t++] = make_int2(slots[j], (int)c);
+  for (int j = 0; j < nprobs; ++j) v.h_probs[j] = slots[j];
+  for (int o = 1; o <= kMaxOrder; ++o) {
+    v.h_counts[7 + o] = t;  // offset of order o
+    for (int j = 0; j < nprobs; ++j) {
+      const Prob& P = *probs_[slots[j]];
+      if (P.N != o) continue;
+      v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / tile_));
+      for (int64_t c = 0; c < P.S; c += tile_) v.h_tiles[t++] = make_int2(slots[j], (int)c);
+    }
+    v.h_counts[2 + o] = t - v.h_counts[7 + o];
+    v.ocap[o] = grow(v.h_counts[2 + o], v.ocap[o]);
   }
   v.h_counts[0] = ntiles;
   v.h_counts[1] = nprobs;
@@ -227,7 +235,7 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
                           hipMemcpyHostToDevice, stream_));
   INSP_HIP(hipMemcpyAsync(v.ranges.p, v.h_ranges, sizeof(int2) * max_slots_, hipMemcpyHostToDevice,
                           stream_));
-  INSP_HIP(hipMemcpyAsync(v.counts.p, v.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipMemcpyAsync(v.counts.p, v.h_counts, sizeof(int) * 12, hipMemcpyHostToDevice, stream_));
   INSP_HIP(hipStreamSynchronize(stream_));
   v.bv.P = d_desc_;
   v.bv.tiles = v.tiles.p;
@@ -240,6 +248,9 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   v.bv.partial = v.partial.p;
   v.bv.dim = cfg_.dim;
   v.bv.tile = tile_;
+  v.bv.ocount = v.counts.p + 2;
+  v.bv.ooff = v.counts.p + 7;
+  for (int o = 0; o <= kMaxOrder; ++o) v.bv.ocap[o] = v.ocap[o];
   // AMG row tiles per level (256 rows each, independent of the cell tile)
   std::vector<int> per_level(kAmgLevels, 0);
   int levels = 0;
@@ -977,10 +988,12 @@ void BatchEngine::iterate() {
   if (refresh) {
     const int old_t = run_->tile_cap, old_p = run_->prob_cap, old_l = run_->ltile_cap,
               old_lev = run_->levels;
+    int old_o[5];
+    std::copy(run_->ocap, run_->ocap + 5, old_o);
     set_view(*run_, running_slots_);
     // the graph bakes in the grid sizes and the number of AMG levels it walks
     if (run_->tile_cap != old_t || run_->prob_cap != old_p || run_->ltile_cap != old_l ||
-        run_->levels != old_lev || !graph_) {
+        run_->levels != old_lev || !std::equal(old_o, old_o + 5, run_->ocap) || !graph_) {
       if (graph_) {
         INSP_HIP(hipGraphExecDestroy(graph_));
         graph_ = nullptr;

@@ -109,6 +109,11 @@ struct BatchView {
   double* partial;               // 4 * tile_cap doubles
   int dim;
   int tile;                      // cells per tile = threads per block (64, 128, 256)
+  // Tiles are grouped by problem order: order o owns tiles [ooff[o], ooff[o] + ocount[o]) of
+  // the table (device values), launched with a grid of ocap[o] blocks.
+  const int* ocount;
+  const int* ooff;
+  int ocap[5];
   // AMG row tiles: level k lists 256-row tiles (slot, first row) of every problem that
   // has level k, at ltiles[k * ltile_cap + t], with lcount[k] entries.
   const int2* ltiles;

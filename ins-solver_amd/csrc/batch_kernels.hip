@@ -31,6 +31,20 @@ struct Cell {
   bool live, valid;
 };
 
+// Wave-uniform copies: per-tile values (problem slot, its pointers and sizes) are the same in
+// every lane; reading them through readfirstlane keeps them in scalar registers, so the
+// operator constants behind them are fetched with scalar loads rather than per lane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+template <typename X>
+__device__ __forceinline__ X* uni(X* p) {
+  return reinterpret_cast<X*>(uni((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+
 template <int TB>
 __device__ __forceinline__ Cell locate(const BatchView& b) {
   Cell c;
@@ -41,9 +55,9 @@ __device__ __forceinline__ Cell locate(const BatchView& b) {
   c.valid = false;
   if (c.live) {
     const int2 tt = b.tiles[c.t];
-    c.slot = tt.x;
-    c.e = tt.y + threadIdx.x;
-    c.valid = c.e < b.P[c.slot].E;
+    c.slot = uni(tt.x);
+    c.e = uni(tt.y) + threadIdx.x;
+    c.valid = c.e < uni(b.P[c.slot].E);
   }
   return c;
 }
@@ -64,64 +78,71 @@ __device__ __forceinline__ const T* ctab(const BatchTables& tb, int N) {
   else return tb.C64[N];
 }
 
-template <int TB, int DIM, typename T>
-__global__ void __launch_bounds__(TB) kb_trace(BatchView b, BatchTables tb, int in, int tr) {
-  const Cell c = locate<TB>(b);
-  if (!c.valid) return;
-  const BatchProb& P = b.P[c.slot];
-  const int prec = std::is_same<T, float>::value;
-  const T* u = (const T*)P.v[in];
-  T* trp = (T*)P.v[tr];
-  const T* k = (const T*)P.kF[prec];
-  const T* C = ctab<T>(tb, P.N);
-  switch (P.N) {
-    case 2:
-      if (k) trace_cell<DIM, 2, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
-      else trace_cell<DIM, 2, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
-      break;
-    case 3:
-      if (k) trace_cell<DIM, 3, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
-      else trace_cell<DIM, 3, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
-      break;
-    case 4:
-      if (k) trace_cell<DIM, 4, T, true>(c.e, P.S, u, P.Kt, k, C, trp);
-      else trace_cell<DIM, 4, T, false>(c.e, P.S, u, P.Kt, k, C, trp);
-      break;
-    default:
-      break;  // order 1 needs no trace pass
+// Tiles of one order: the block's tile is ooff[NN] + its (XCD-remapped) index.
+template <int TB, int NN>
+__device__ __forceinline__ Cell locate_order(const BatchView& b) {
+  Cell c;
+  const int local = (int)xcd_block();
+  c.live = local < uni(b.ocount[NN]);
+  c.t = uni(b.ooff[NN]) + local;
+  c.slot = 0;
+  c.e = 0;
+  c.valid = false;
+  if (c.live) {
+    const int2 tt = b.tiles[c.t];
+    c.slot = uni(tt.x);
+    c.e = uni(tt.y) + threadIdx.x;
+    c.valid = c.e < uni(b.P[c.slot].E);
   }
+  return c;
 }
 
-template <int TB, int DIM, typename T>
+// Face traces of order-NN problems (order 1 needs none).
+template <int TB, int DIM, typename T, int NN>
+__global__ void __launch_bounds__(TB) kb_trace(BatchView b, BatchTables tb, int in, int tr) {
+  const Cell c = locate_order<TB, NN>(b);
+  if (!c.live) return;  // uniform per block
+  const BatchProb& P = b.P[c.slot];
+  const int prec = std::is_same<T, float>::value;
+  const int64_t S = uni(P.S);
+  const T* u = uni((const T*)P.v[in]);
+  T* trp = uni((T*)P.v[tr]);
+  const T* k = uni((const T*)P.kF[prec]);
+  const T* C = uni(ctab<T>(tb, NN));
+  const double* Kt = uni(P.Kt);
+  if (!c.valid) return;
+  trace_cell<DIM, NN, T, 2>(c.e, S, u, Kt, k, as_const(C), trp);
+}
+
+// Operator of order-NN problems on their own order (level 0); with LEVEL1 the order-1
+// operator of every problem (all tiles), one order per kernel so each is compiled lean.
+template <int TB, int DIM, typename T, int NN, bool LEVEL1>
 __global__ void __launch_bounds__(TB)
-kb_apply(BatchView b, BatchTables tb, int level, int in, int tr, int out, int dot) {
-  const Cell c = locate<TB>(b);
+kb_apply(BatchView b, BatchTables tb, int in, int tr, int out, int dot) {
+  const Cell c = LEVEL1 ? locate<TB>(b) : locate_order<TB, NN>(b);
   double dv[1] = {0.0};
-  if (c.valid) {
+  if (c.live) {  // uniform per block: the problem's data in scalar registers
     const BatchProb& P = b.P[c.slot];
     const int prec = std::is_same<T, float>::value;
-    const int N = level ? 1 : P.N;
-    const T* u = (const T*)P.v[in];
-    T* o = (T*)P.v[out];
-    const double* tau = level ? P.tauC : P.tauF;
-    const T* k = (const T*)(level ? P.kC[prec] : P.kF[prec]);
-    const T* C = ctab<T>(tb, N);
-    const int* tab = tb.tab[N];
-    if (N == 1) {
-      dv[0] = k ? apply_p1_cell<DIM, T, true>(c.e, P.S, u, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o)
-                : apply_p1_cell<DIM, T, false>(c.e, P.S, u, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o);
-    } else {
-      const T* trp = (const T*)P.v[tr];
-#define INSP_BAPPLY(NN)                                                                            \
-  dv[0] = k ? apply_cell<DIM, NN, T, true>(c.e, P.S, u, trp, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o) \
-            : apply_cell<DIM, NN, T, false>(c.e, P.S, u, trp, P.Kt, tau, P.nbr, P.code, P.bctype, k, C, tab, o)
-      switch (N) {
-        case 2: INSP_BAPPLY(2); break;
-        case 3: INSP_BAPPLY(3); break;
-        case 4: INSP_BAPPLY(4); break;
-        default: break;
+    const int64_t S = uni(P.S);
+    const T* u = uni((const T*)P.v[in]);
+    T* o = uni((T*)P.v[out]);
+    const double* tau = uni(LEVEL1 ? P.tauC : P.tauF);
+    const T* k = uni((const T*)(LEVEL1 ? P.kC[prec] : P.kF[prec]));
+    const T* C = uni(ctab<T>(tb, NN));
+    const int* tab = uni(tb.tab[NN]);
+    const double* Kt = uni(P.Kt);
+    const int* nbr = uni(P.nbr);
+    const int* code = uni(P.code);
+    const int* bct = uni(P.bctype);
+    if (c.valid) {
+      if constexpr (NN == 1) {
+        dv[0] = apply_p1_cell<DIM, T, 2>(c.e, S, u, Kt, tau, nbr, code, bct, k, as_const(C), as_const(tab), o);
+      } else {
+        const T* trp = uni((const T*)P.v[tr]);
+        dv[0] = apply_cell<DIM, NN, T, 2>(c.e, S, u, trp, Kt, tau, nbr, code, bct, k, as_const(C), as_const(tab),
+                                          o);
       }
-#undef INSP_BAPPLY
     }
   }
   if (dot) put_partials<TB, 1>(b, c, dv);
@@ -594,15 +615,32 @@ dim3 tiles_grid(const BatchView& b) { return dim3((unsigned)b.tile_cap); }
 
 template <typename T>
 void trace(const BatchView& b, const BatchTables& t, int in, int tr, hipStream_t s) {
-  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_trace<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t, in, tr)));
+#define INSP_TRACE(NN)                                                                              \
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_trace<TB_, D_, T, NN>), dim3((unsigned)b.ocap[NN]), dim3(TB_), \
+                                       0, s, b, t, in, tr)))
+  INSP_TRACE(2);
+  INSP_TRACE(3);
+  INSP_TRACE(4);
+#undef INSP_TRACE
   check("kb_trace");
 }
 
 template <typename T>
 void apply(const BatchView& b, const BatchTables& t, int level, int in, int tr, int out, bool dot,
            hipStream_t s) {
-  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_apply<TB_, D_, T>), tiles_grid(b), dim3(TB_), 0, s, b, t, level, in, tr,
-                               out, (int)dot)));
+  if (level) {
+    INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_apply<TB_, D_, T, 1, true>), tiles_grid(b), dim3(TB_), 0, s, b,
+                                         t, in, tr, out, (int)dot)));
+  } else {
+#define INSP_APPLY(NN)                                                                              \
+  INSP_DIMT(INSP_TB(hipLaunchKernelGGL((kb_apply<TB_, D_, T, NN, false>), dim3((unsigned)b.ocap[NN]),      \
+                                       dim3(TB_), 0, s, b, t, in, tr, out, (int)dot)))
+    INSP_APPLY(1);
+    INSP_APPLY(2);
+    INSP_APPLY(3);
+    INSP_APPLY(4);
+#undef INSP_APPLY
+  }
   check("kb_apply");
 }
 

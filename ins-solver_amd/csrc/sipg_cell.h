@@ -18,11 +18,31 @@ __device__ __forceinline__ int64_t xcd_block() {
   return (int64_t)(x * q + (x < rem ? x : rem) + k);
 }
 
+// Operator tables (matrices and index tables) are read-only for a whole launch and the
+// same for every lane: reading them through the constant address space lets the compiler
+// fetch them with scalar loads into SGPRs whether the pointer came from a kernel argument
+// or from a problem descriptor.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
+// Coefficient at an index: VAR 0 = constant 1, 1 = from kfac, 2 = from kfac when it is
+// not null (decided per problem at run time; one code path for both).
+template <int VAR, typename T>
+__device__ __forceinline__ T kval(const T* __restrict__ kfac, int64_t idx) {
+  if constexpr (VAR == 0) return (T)1;
+  else if constexpr (VAR == 1) return kfac[idx];
+  else return kfac ? kfac[idx] : (T)1;
+}
+
 // Trace pass for one cell: T = sJ n.(k grad u) at every face node.
-template <int DIM, int N, typename T, bool VAR>
+template <int DIM, int N, typename T, int VAR>
 __device__ __forceinline__ void
 trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
-           const T* __restrict__ kfac, const T* __restrict__ C, T* __restrict__ tr) {
+           const T* __restrict__ kfac, cptr<T> __restrict__ C, T* __restrict__ tr) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, N);
@@ -53,7 +73,7 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
       g[s] = a;
     }
     T kp = 1;
-    if constexpr (VAR) kp = kfac[p * stride + e];
+    kp = kval<VAR>(kfac, p * stride + e);
 #pragma unroll
     for (int f = 0; f < Nf; ++f)
 #pragma unroll
@@ -68,13 +88,13 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
 }
 
 // One cell of the element pass; returns u.(A u) over the cell.
-template <int DIM, int N, typename T, bool VAR>
+template <int DIM, int N, typename T, int VAR>
 __device__ __forceinline__ double
 apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
            const double* __restrict__ Kt, const double* __restrict__ tauS,
            const int* __restrict__ nbr, const int* __restrict__ code,
-           const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
-           const int* __restrict__ tab, T* __restrict__ out) {
+           const int* __restrict__ bctype, const T* __restrict__ kfac, cptr<T> __restrict__ C,
+           cptr<int> __restrict__ tab, T* __restrict__ out) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, N);
@@ -99,7 +119,7 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
       g[s] = a;
     }
     T kj = 1;
-    if constexpr (VAR) kj = kfac[j * stride + e];
+    kj = kval<VAR>(kfac, j * stride + e);
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
       T h = 0;
@@ -148,7 +168,7 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
       const T jump = um - up;
       const T F = (T)0.5 * (ts * jump - (Tm - Tp));
       T sig = (T)-0.5 * jump;
-      if constexpr (VAR) sig *= kfac[p * stride + e];
+      sig *= kval<VAR>(kfac, p * stride + e);
 #pragma unroll
       for (int i = 0; i < Nfp; ++i)
         acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
@@ -189,12 +209,12 @@ __device__ __forceinline__ double face_alpha_rt(int dim, int f, int r) {
   }
 }
 
-template <int DIM, typename T, bool VAR>
+template <int DIM, typename T, int VAR>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
               const int* __restrict__ code, const int* __restrict__ bctype,
-              const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
+              const T* __restrict__ kfac, cptr<T> __restrict__ C, cptr<int> __restrict__ tab,
               T* __restrict__ out) {
   constexpr OpLayout L = op_layout(DIM, 1);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
@@ -206,7 +226,7 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     for (int m = 0; m < Np; ++m) {
       uu[m] = u[m * stride + e];
       acc[m] = 0;
-      kk[m] = VAR ? kfac[m * stride + e] : (T)1;
+      kk[m] = kval<VAR>(kfac, m * stride + e);
     }
     T K[L.NK];
 #pragma unroll
@@ -285,7 +305,7 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
         if (mode == 0) {
           const int node = tab[tbase + j] >> 6;
           up = u[(int64_t)node * stride + nb];
-          Tp = (VAR ? kfac[(int64_t)node * stride + nb] : (T)1) * tn0;
+          Tp = kval<VAR>(kfac, (int64_t)node * stride + nb) * tn0;
         } else {
           up = -um;
           Tp = -Tm;

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(256)
 k_trace(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
         const T* __restrict__ kfac, const T* __restrict__ C, T* __restrict__ tr) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e < E) trace_cell<DIM, N, T, VAR>(e, stride, u, Kt, kfac, C, tr);
+  if (e < E) trace_cell<DIM, N, T, VAR>(e, stride, u, Kt, kfac, as_const(C), tr);
 }
 
 template <int DIM, int N, typename T, bool VAR>
@@ -59,7 +59,9 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
         const int* __restrict__ tab, T* __restrict__ out, double* __restrict__ partial) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
-  if (e < E) dotv = apply_cell<DIM, N, T, VAR>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, C, tab, out);
+  if (e < E)
+    dotv = apply_cell<DIM, N, T, VAR>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
+                                      as_const(tab), out);
   // optional fused u.(A u) partial sum for CG step lengths
   if (partial) {
     const double t = block_sum1<256>(dotv);
@@ -78,7 +80,8 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_p1_cell<DIM, T, VAR>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac, C, tab, out);
+    dotv = apply_p1_cell<DIM, T, VAR>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
+                                      as_const(tab), out);
   if (partial) {
     const double t = block_sum1<256>(dotv);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
