@@ -39,6 +39,10 @@ METRIC = "Poisson GDoF/s & solves/s to 1e-8 rel res, mixed vs FP64; p99 solve la
 # flops 6*d*Np^2 + 2(d+1)*Np*Nf*Nfp + 4d^2*Np + Nf*Nfp*(4d+6), bytes 2*Np*8 + 10*8 + Nf*5*8 + Nf*8.
 FLOPS_PER_CELL = {(3, 3): 15040, (3, 2): 4512, (3, 1): 1032, (3, 4): 41190}
 BYTES_PER_CELL_F64 = {(3, 3): 592, (3, 2): 432, (3, 1): 336, (3, 4): 832}
+# HBM bytes of one trace + element pass on the 64^3x6 cube at N=3, FP64, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--copy-probe), calibrated by
+# a plain 256 MiB device copy: traffic = 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE.
+MEASURED_TRAFFIC = {(64, 3, "fp64"): 4.760e9}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -392,7 +396,7 @@ def main():
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved_tf / PEAK_FP64_TFLOPS if achieved_tf else None,
-        "traffic": None,
+        "traffic": MEASURED_TRAFFIC.get((a.n, a.order, a.precision)),
         "kernel": f"SIPG apply order {a.order} FP64 (k_trace + k_apply), {cells} cells",
         "avg_launch_ms": t_apply * 1e3,
         "launches": timing["apply_launches"],
@@ -401,6 +405,10 @@ def main():
         "hbm_achieved_gbs": achieved_gbs,
         "hbm_frac": achieved_gbs / PEAK_HBM_GBS if achieved_gbs else None,
     }
+    tr = roofline["traffic"]
+    if tr and t_apply > 0:
+        roofline["traffic_gbs"] = tr / t_apply * 1e-9
+        roofline["traffic_over_algorithmic"] = tr / bytes_ if bytes_ else None
     out = {
         "metric": METRIC,
         "value": gdofs,
