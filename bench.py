@@ -12,6 +12,10 @@ Prints one JSON line. `value` is GDoF/s = DoF x solves / wall time.
 
 Beside it the line carries the other forms of the same solve:
   mixed_vs_fp64  the same cube at rel res 1e-8, FP64 V-cycle vs FP32 V-cycle;
+  coarse_amg     the same cube with the vertex-space AMG coarse solve;
+  precision_form 48^3x6 tets, N=4, div(1/rho grad p) = f with rho = 1000 inside a
+                 sphere, pure Neumann, rel res 1e-8: plain FP64 path, FP64 with the
+                 AMG coarse solve, FP32 V-cycle inside the FP64 flexible CG;
   batch          256 independent 2D problems (log-uniform 500..50k triangles,
                  N in {2,3,4}, u = sin(k pi x) sin(m pi y), k,m in 1..4) solved
                  to 1e-8 one at a time on the plain FP64 path, then all at once
@@ -44,6 +48,7 @@ BYTES_PER_CELL_F64 = {(3, 3): 592, (3, 2): 432, (3, 1): 336, (3, 4): 832}
 # a plain 256 MiB device copy: traffic = 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE.
 MEASURED_TRAFFIC = {(64, 3, "fp64"): 4.760e9}
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_MATRIX_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -73,6 +78,9 @@ def parse():
     ap.add_argument("--skip-amg", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB device copy 3 times first (counter calibration)")
+    ap.add_argument("--precision-n", type=int, default=48, help="cube size of the precision form")
+    ap.add_argument("--precision-order", type=int, default=4)
+    ap.add_argument("--skip-precision", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -117,12 +125,15 @@ def cube_solver(ip, n, order, precision, rtol, coarse_iters):
     return mesh, s, orders
 
 
-def timed_solves(s, ip, torch, b_d, x_d, steps, warmup, profile=False):
+def timed_solves(s, ip, torch, b_d, x_d, steps, warmup, profile=False, tag=None):
     """steps solves from a zero guess, rhs/solution resident in HBM; returns wall, latencies, stats."""
     def one():
         x_d.zero_()
         s.solve(b_d.data_ptr(), x_d.data_ptr(), layout=ip.LAYOUT_DEVICE_SOA)
-        return dict(s.last_stats)
+        st = dict(s.last_stats)
+        if tag:
+            log(f"{tag}: solve done, {st['iterations']} its, rel res {st['rel_residual']:.2e}")
+        return st
 
     for _ in range(warmup):
         one()
@@ -182,6 +193,117 @@ def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
                      "amg_levels": s.amg_levels()}
     s.set_coarse(ip.COARSE_JACOBI_CG)
     return out
+
+
+def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206):
+    """Variable-coefficient pure-Neumann cube: div(1/rho grad p) = f, rho = contrast inside a
+    sphere of radius 0.25 at the centre (smoothed over 2h), f Gaussian projected to zero mean,
+    rel res 1e-8. Runs the plain FP64 path (Jacobi-CG coarse), FP64 with the AMG coarse solve and
+    the FP32 V-cycle with the AMG coarse solve inside the FP64 flexible CG; reports GDoF/s of each
+    and the relative L2 (mass norm, mean removed) of each solution against the FP64 AMG one."""
+    mesh = ip.Mesh.cube(n, 0.1, seed)
+    s = ip.PoissonSolver(mesh, order)
+    s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
+    s.set_nullspace(True)
+    xyz = s.node_coords()
+    k = 1.0 / density(xyz, 1.0 / n, contrast)
+    s.set_factor(k)
+    s.set_solver(ip.SOLVER_PMULTIGRID)
+    orders = list(range(order, 0, -1))
+    s.set_pmg(orders, pre_it=[1] * len(orders), post_it=[1] * len(orders),
+              cheb_orders=[3] * len(orders), coarse_iters=150)
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((s.n_cells, s.Np))
+    b = s.mass(f)
+    b -= b.sum() / b.size
+    dof = s.n_cells * s.Np
+
+    def to_soa(x):
+        soa = np.zeros((s.Np, s.stride))
+        soa[:, :s.n_cells] = x.T
+        return torch.from_numpy(soa).cuda()
+
+    b_d = to_soa(b)
+    x_d = torch.zeros_like(b_d)
+    out = {"workload": f"cube {n}^3x6 Kuhn tets, N={order}, rho {contrast:g} in a sphere r=0.25 "
+                       f"(band 2h), pure Neumann, mean-zero, rel res 1e-8", "dof": dof}
+    sols = {}
+    for name, prec, coarse in (("plain_fp64", ip.PRECISION_FP64, ip.COARSE_JACOBI_CG),
+                               ("fp64_amg", ip.PRECISION_FP64, ip.COARSE_AMG),
+                               ("mixed_amg", ip.PRECISION_MIXED, ip.COARSE_AMG)):
+        log(f"precision form: {name}")
+        s.set_precision(prec)
+        s.set_coarse(coarse)
+        s.set_tol_and_iter(1e-8, 0.0, 1000)
+        t0 = time.perf_counter()
+        s.init()
+        setup = time.perf_counter() - t0
+        wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1, tag=name)
+        x = x_d.cpu().numpy()[:, :s.n_cells].T.copy()
+        x -= np.sum(x * s.mass(np.ones_like(x))) / np.sum(s.mass(np.ones_like(x)))
+        sols[name] = x
+        out[name] = {"gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+                     "setup_s": setup, "iterations": [st["iterations"] for st in stats],
+                     "rel_residual": max(st["rel_residual"] for st in stats),
+                     "converged": all(st["converged"] for st in stats)}
+    ref = sols["fp64_amg"]
+    nref = float(np.sqrt(np.sum(ref * s.mass(ref))))
+    for name in ("plain_fp64", "mixed_amg"):
+        d = sols[name] - ref
+        out[name]["rel_l2_vs_fp64_amg"] = float(np.sqrt(np.sum(d * s.mass(d)))) / nref
+    out["fp32_apply"] = fp32_apply_kernels(ip, torch, s, order)
+    out["speedup_mixed_over_plain_fp64"] = out["mixed_amg"]["gdofs"] / out["plain_fp64"]["gdofs"]
+    out["speedup_mixed_over_fp64_amg"] = out["mixed_amg"]["gdofs"] / out["fp64_amg"]["gdofs"]
+    return out
+
+
+def mfma_issued_flops_per_cell(order):
+    """Flops issued on the matrix cores per cell by the 3D element pass (sipg_mfma.hip):
+    per 16-cell tile Q^2 16x16x4 products for G = D4 U and, per k-step of the second
+    product, one 16x16x4 per full 16-row output tile and one 4x4x1 (16 blocks) per 4-row group."""
+    np_ = (order + 1) * (order + 2) * (order + 3) // 6
+    nfp = (order + 1) * (order + 2) // 2
+    q, mtf, r4, j4 = (np_ + 3) // 4, np_ // 16, (np_ % 16 + 3) // 4, (nfp + 3) // 4
+    ks2 = 3 * q + 4 * 4 * j4
+    per_tile = q * q * 2048 + ks2 * (mtf * 2048 + r4 * 512)
+    return per_tile / 16.0
+
+
+def fp32_apply_kernels(ip, torch, s, order, reps=20):
+    """FP32 operator apply of the solver's fine level with each element-pass kernel, timed
+    with HIP events on the solver's stream (trace and element pass apart)."""
+    res = {}
+    cells = s.n_cells
+    x = torch.rand((s.Np, s.stride), dtype=torch.float64, device="cuda")
+    x[:, cells:] = 0
+    y = torch.zeros_like(x)
+    for name, kind in (("matrix_cores", ip.ELEMENT_PASS_MATRIX), ("vector", ip.ELEMENT_PASS_VECTOR)):
+        s.set_element_pass(kind)
+        s.init()
+        for _ in range(3):
+            s.mult_fp32((x.data_ptr(), y.data_ptr()), layout=ip.LAYOUT_DEVICE_SOA)
+        torch.cuda.synchronize()
+        s.take_timing()
+        s.set_profiling(True)
+        for _ in range(reps):
+            s.mult_fp32((x.data_ptr(), y.data_ptr()), layout=ip.LAYOUT_DEVICE_SOA)
+        torch.cuda.synchronize()
+        t = s.take_timing()
+        s.set_profiling(False)
+        apply_ms = t["apply_ms"] / max(t["apply_launches"], 1)
+        trace_ms = t["trace_ms"] / max(t["trace_launches"], 1)
+        useful = FLOPS_PER_CELL.get((3, order), 0) * cells
+        r = {"element_pass_ms": apply_ms, "trace_ms": trace_ms,
+             "useful_tflops": useful / (apply_ms + trace_ms) * 1e-9 if apply_ms > 0 else None}
+        if name == "matrix_cores":
+            issued = mfma_issued_flops_per_cell(order) * cells
+            r["mfma_issued_tflops"] = issued / apply_ms * 1e-9
+            r["mfma_frac_of_fp32_matrix_peak"] = issued / apply_ms * 1e-9 / PEAK_FP32_MATRIX_TFLOPS
+        res[name] = r
+    s.set_element_pass(ip.ELEMENT_PASS_AUTO)
+    res["speedup_matrix_over_vector"] = res["vector"]["element_pass_ms"] / res["matrix_cores"]["element_pass_ms"]
+    del x, y
+    return res
 
 
 def batch_problems(ip, count, seed=20261204):
@@ -450,6 +572,10 @@ def main():
         out["coarse_amg"] = coarse_amg_leg(ip, torch, s, b_d, x_d, dof, 2)
     del s, b_d, x_d, mesh
     torch.cuda.empty_cache()
+    if not a.skip_precision:
+        log("precision form leg")
+        out["precision_form"] = precision_leg(ip, torch, a.precision_n, a.precision_order, 2)
+        torch.cuda.empty_cache()
     if not a.skip_batch:
         log("batch leg")
         out["batch"] = batch_leg(ip, a.batch_problems)

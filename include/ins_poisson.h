@@ -129,6 +129,15 @@ int ins_poisson_set_tol_and_iter(ins_poisson_t h, double rtol, double atol, int 
 int ins_poisson_set_solver(ins_poisson_t h, int type);
 /* INS_PRECISION_FP64, or INS_PRECISION_MIXED: FP32 V-cycle inside the FP64 flexible CG. */
 int ins_poisson_set_precision(ins_poisson_t h, int precision);
+/* Kernel of the element pass of the FP32 operator (the V-cycle of INS_PRECISION_MIXED and
+ * ins_poisson_mult_fp32). No counterpart in the reference, whose operator has one code path:
+ * a tuning choice of this library, taking effect at the next init.
+ *   INS_ELEMENT_PASS_AUTO    matrix cores where they are the faster kernel (3D, order 4),
+ *                            vector FMA elsewhere (default);
+ *   INS_ELEMENT_PASS_VECTOR  vector FMA on every level;
+ *   INS_ELEMENT_PASS_MATRIX  matrix cores on every level that has them (3D, orders 2-4). */
+enum { INS_ELEMENT_PASS_AUTO = 0, INS_ELEMENT_PASS_VECTOR = 1, INS_ELEMENT_PASS_MATRIX = 2 };
+int ins_poisson_set_element_pass(ins_poisson_t h, int kind);
 /* p-multigrid options (the [p-multigrid] INI section: orders, pre_it, post_it, cheb_orders,
  * eigen_val_saftey_factor, cheb_min_lamda_factor, max_iter_eig_approx); per-level arrays
  * have n_levels entries and may be NULL for defaults. coarse_iters: Jacobi-CG iterations on

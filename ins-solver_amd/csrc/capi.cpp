@@ -341,6 +341,15 @@ int ins_poisson_set_precision(ins_poisson_t h, int precision) {
   });
 }
 
+int ins_poisson_set_element_pass(ins_poisson_t h, int kind) {
+  return guarded([&] {
+    if (kind != INS_ELEMENT_PASS_AUTO && kind != INS_ELEMENT_PASS_VECTOR && kind != INS_ELEMENT_PASS_MATRIX)
+      throw insp::InvalidArgument("unsupported element pass");
+    S(h).set_element_pass(static_cast<insp::ElementPass>(kind));
+    return INS_OK;
+  });
+}
+
 int ins_poisson_set_pmg(ins_poisson_t h, const int* orders, int n_levels, const int* pre_it,
                         const int* post_it, const int* cheb_orders, double eig_safety,
                         double cheb_min_factor, int max_iter_eig, int coarse_iters) {

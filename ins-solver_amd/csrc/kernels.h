@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace insp {
 
@@ -25,6 +26,7 @@ struct LevelView {
   const double* C64;   // operator constants in double (set-up kernels)
   const int* tab;
   const double* tauS;  // [Nf][stride] penalty * sJ
+  const void* frag = nullptr;  // matrix-core operator fragments in the kernel precision, or null
 };
 
 // SIPG operator: trace pass + element pass (order 1: element pass only). With
@@ -37,6 +39,16 @@ template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
                   T* out, hipStream_t s, double* partial = nullptr);
 int apply_blocks(int64_t E);
+// Element pass on the matrix cores (sipg_mfma.hip): 3D, orders 2-4. launch_apply
+// takes it whenever the level view carries fragments.
+bool mfma_apply_supported(int dim, int N);
+// A-operand fragments of one order built from the operator constants and index
+// table (make_op_tables); empty when the order has no matrix-core pass.
+std::vector<double> build_mfma_fragments(int dim, int N, const std::vector<double>& C,
+                                         const std::vector<int>& tab);
+template <typename T>
+void launch_apply_mfma(const MeshView& m, const LevelView& l, const T* u, const T* tr,
+                       const T* kfac, T* out, hipStream_t s, double* partial);
 // Exact diagonal of the operator (double).
 void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
                  hipStream_t s);

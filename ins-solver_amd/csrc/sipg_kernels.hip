@@ -347,6 +347,10 @@ template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
                   T* out, hipStream_t s, double* partial) {
   if (m.E == 0) return;
+  if (l.frag && mfma_apply_supported(m.dim, l.N)) {
+    launch_apply_mfma<T>(m, l, u, tr, kfac, out, s, partial);
+    return;
+  }
   const T* C = static_cast<const T*>(l.C);
   if (l.N == 1) {
     if (m.dim == 2) {

@@ -29,6 +29,7 @@ struct PoissonSolver::Work {
   DBuf<T> inv;             // inverse diagonal
   DBuf<T> k;               // factor (empty = constant)
   DBuf<T> C;               // operator constants
+  DBuf<T> frag;            // matrix-core fragments of the element pass (empty: vector pass)
   DBuf<T> P, Pt;           // transfer to/from the next coarser level
 };
 
@@ -149,6 +150,10 @@ MeshView PoissonSolver::mesh_view() const {
   return m;
 }
 
+bool PoissonSolver::matrix_pass(int l) const {
+  return l >= 0 && l < (int)levels_.size() && !levels_[l]->w32.frag.empty();
+}
+
 LevelView PoissonSolver::level_view(int l, bool fp32) const {
   const Level& lv = *levels_[l];
   LevelView v;
@@ -157,6 +162,7 @@ LevelView PoissonSolver::level_view(int l, bool fp32) const {
   v.C64 = lv.C64.p;
   v.tab = lv.tab.p;
   v.tauS = lv.tauS.p;
+  v.frag = fp32 ? (const void*)lv.w32.frag.p : (const void*)lv.w64.frag.p;
   return v;
 }
 
@@ -289,6 +295,12 @@ void PoissonSolver::build_levels() {
     lv->w64.C.upload(C.data(), C.size());
     std::vector<float> Cf(C.begin(), C.end());
     lv->w32.C.upload(Cf.data(), Cf.size());
+    const bool matrix = epass_ == ElementPass::MATRIX || (epass_ == ElementPass::AUTO && d == 3 && N == 4);
+    const std::vector<double> fr = matrix ? build_mfma_fragments(d, N, C, tab) : std::vector<double>();
+    if (!fr.empty()) {
+      std::vector<float> frf(fr.begin(), fr.end());
+      lv->w32.frag.upload(frf.data(), frf.size());
+    }
     levels_.push_back(std::move(lv));
   }
   for (size_t l = 0; l + 1 < levels_.size(); ++l) {

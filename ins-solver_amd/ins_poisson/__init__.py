@@ -33,6 +33,7 @@ SOLVER_BLOCK_JACOBI = 2
 SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+ELEMENT_PASS_AUTO, ELEMENT_PASS_VECTOR, ELEMENT_PASS_MATRIX = 0, 1, 2
 COARSE_JACOBI_CG = 0
 COARSE_AMG = 1
 BC_DIRICHLET = 0
@@ -51,7 +52,7 @@ EXPORTED = [
     "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords",
     "ins_poisson_set_bc_types", "ins_poisson_set_factor", "ins_poisson_set_bcs",
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
-    "ins_poisson_set_precision", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
+    "ins_poisson_set_precision", "ins_poisson_set_element_pass", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
     "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
@@ -124,6 +125,7 @@ def _load():
         "ins_poisson_set_tol_and_iter": [vp, dbl, dbl, C.c_int],
         "ins_poisson_set_solver": [vp, C.c_int],
         "ins_poisson_set_precision": [vp, C.c_int],
+        "ins_poisson_set_element_pass": [vp, C.c_int],
         "ins_poisson_set_pmg": [vp, vp, C.c_int, vp, vp, vp, dbl, dbl, C.c_int, C.c_int],
         "ins_poisson_set_coarse": [vp, C.c_int, C.c_int],
         "ins_poisson_coarse_matrix": [vp, C.POINTER(i64), C.POINTER(i64), vp, vp, vp],
@@ -340,6 +342,10 @@ class PoissonSolver:
 
     def set_precision(self, prec):
         _check(lib.ins_poisson_set_precision(self._h, prec))
+
+    def set_element_pass(self, kind):
+        """ELEMENT_PASS_AUTO (matrix cores at 3D order 4), _VECTOR or _MATRIX; next init."""
+        _check(lib.ins_poisson_set_element_pass(self._h, kind))
 
     def set_pmg(self, orders, pre_it=None, post_it=None, cheb_orders=None, eig_safety=1.1,
                 cheb_min_factor=0.1, max_iter_eig=15, coarse_iters=40):
