@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--precision-n", type=int, default=48, help="cube size of the precision form")
     ap.add_argument("--precision-order", type=int, default=4)
     ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the precision-form leg and print its JSON (a profiling aid)")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -470,6 +472,10 @@ def main():
     import torch
     import ins_poisson as ip
 
+    if a.precision_only:
+        res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps)
+        print(json.dumps({"precision_form": res}), flush=True)
+        return
     if a.copy_probe:
         # a plain copy of known size: 256 MiB read + 256 MiB written per launch
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
