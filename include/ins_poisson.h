@@ -201,6 +201,10 @@ int ins_poisson_lambda_max(ins_poisson_t h, int level, double* out);
 int ins_poisson_set_profiling(ins_poisson_t h, int on);
 int ins_poisson_take_timing(ins_poisson_t h, double* trace_ms, int64_t* trace_launches,
                             double* apply_ms, int64_t* apply_launches);
+/* Device time of the operator applies on the coarsest p-multigrid level (order 1 with the
+ * default levels: the applies of the coarse Jacobi-CG), one launch in four timed while
+ * profiling is on; launches counts the timed ones. */
+int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* launches);
 
 /* Reference-element tables of order `order` (the toolkit's DGConstants: Dr/Ds/Dt, MASS,
  * FMASK, face mass), row-major; any output may be NULL. rst: dim x Np, D: dim x Np x Np,

@@ -541,6 +541,15 @@ int ins_poisson_take_timing(ins_poisson_t h, double* trace_ms, int64_t* trace_la
   });
 }
 
+int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* launches) {
+  return guarded([&] {
+    const insp::CoarseTiming t = S(h).take_coarse_timing();
+    if (apply_ms) *apply_ms = t.apply_ms;
+    if (launches) *launches = t.launches;
+    return INS_OK;
+  });
+}
+
 int ins_ref_element(int dim, int order, double* rst, double* D, double* M, int* fmask,
                     double* Mface) {
   return guarded([&] {

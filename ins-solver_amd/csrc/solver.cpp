@@ -130,6 +130,7 @@ PoissonSolver::~PoissonSolver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& pr : ev_trace_) { ev_pool_.push_back(pr.first); ev_pool_.push_back(pr.second); }
   for (auto& pr : ev_apply_) { ev_pool_.push_back(pr.first); ev_pool_.push_back(pr.second); }
+  for (auto& pr : ev_coarse_) { ev_pool_.push_back(pr.first); ev_pool_.push_back(pr.second); }
   std::sort(ev_pool_.begin(), ev_pool_.end());
   ev_pool_.erase(std::unique(ev_pool_.begin(), ev_pool_.end()), ev_pool_.end());
   for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
@@ -457,9 +458,14 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
   const LevelView v = level_view(l, std::is_same<T, float>::value);
   const T* k = var_factor_ ? w.k.p : nullptr;
   const bool prof = profiling_ && l == 0;
+  const bool prof_coarse = profiling_ && l > 0 && l + 1 == (int)levels_.size() &&
+                           (coarse_sample_++ & 3) == 0;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (prof) {
     e0 = ev_take(); e1 = ev_take(); e2 = ev_take();
+    INSP_HIP(hipEventRecord(e0, stream_));
+  } else if (prof_coarse) {
+    e0 = ev_take(); e2 = ev_take();
     INSP_HIP(hipEventRecord(e0, stream_));
   }
   launch_trace<T>(mv, v, u, k, w.tr.p, stream_);
@@ -469,7 +475,25 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
     INSP_HIP(hipEventRecord(e2, stream_));
     ev_trace_.push_back({e0, e1});
     ev_apply_.push_back({e1, e2});
+  } else if (prof_coarse) {
+    INSP_HIP(hipEventRecord(e2, stream_));
+    ev_coarse_.push_back({e0, e2});
   }
+}
+
+CoarseTiming PoissonSolver::take_coarse_timing() {
+  INSP_HIP(hipStreamSynchronize(stream_));
+  CoarseTiming t;
+  for (auto& pr : ev_coarse_) {
+    float ms = 0;
+    INSP_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    t.apply_ms += ms;
+    t.launches++;
+    ev_pool_.push_back(pr.first);
+    ev_pool_.push_back(pr.second);
+  }
+  ev_coarse_.clear();
+  return t;
 }
 
 KernelTiming PoissonSolver::take_timing() {

@@ -118,6 +118,12 @@ struct KernelTiming {
   double trace_ms = 0, apply_ms = 0;
   int64_t trace_launches = 0, apply_launches = 0;
 };
+// Device time of the operator applies on the coarsest p-multigrid level (below the
+// finest), one launch in four timed, while profiling is enabled.
+struct CoarseTiming {
+  double apply_ms = 0;
+  int64_t launches = 0;
+};
 
 class PoissonSolver {
  public:
@@ -183,6 +189,7 @@ class PoissonSolver {
 
   void set_profiling(bool on) { profiling_ = on; }
   KernelTiming take_timing();
+  CoarseTiming take_coarse_timing();
 
  private:
   struct Level;
@@ -257,7 +264,8 @@ class PoissonSolver {
   int64_t smoother_applies_ = 0;
 
   bool profiling_ = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_trace_, ev_apply_;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_trace_, ev_apply_, ev_coarse_;
+  int64_t coarse_sample_ = 0;
   std::vector<hipEvent_t> ev_pool_;
   KernelTiming timing_;
   hipEvent_t ev_take();

@@ -57,7 +57,7 @@ EXPORTED = [
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
-    "ins_poisson_take_timing", "ins_ref_element", "ins_ref_interp",
+    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
     "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
@@ -145,6 +145,7 @@ def _load():
         "ins_poisson_lambda_max": [vp, C.c_int, dp],
         "ins_poisson_set_profiling": [vp, C.c_int],
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
+        "ins_poisson_take_coarse_timing": [vp, dp, C.POINTER(i64)],
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
         "ins_ref_interp": [C.c_int, C.c_int, C.c_int, vp],
         "ins_batch_problem_bytes": [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.POINTER(i64)],
@@ -460,11 +461,15 @@ class PoissonSolver:
         _check(lib.ins_poisson_set_profiling(self._h, int(bool(on))))
 
     def take_timing(self):
+        """Fine-level trace/apply device time and the sampled coarsest-level apply time."""
         tm, tl, am, al = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
         _check(lib.ins_poisson_take_timing(self._h, C.byref(tm), C.byref(tl), C.byref(am),
                                            C.byref(al)))
+        cm, cl = C.c_double(), C.c_int64()
+        _check(lib.ins_poisson_take_coarse_timing(self._h, C.byref(cm), C.byref(cl)))
         return {"trace_ms": tm.value, "trace_launches": tl.value, "apply_ms": am.value,
-                "apply_launches": al.value}
+                "apply_launches": al.value, "coarse_apply_ms": cm.value,
+                "coarse_apply_launches": cl.value}
 
 
 # --- batched solves and serving ----------------------------------------------
