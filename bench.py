@@ -93,6 +93,58 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def config0_leg(ip, reps=5):
+    """The CPU-runnable configuration: unit square 32x32 cells x 2 triangles, jitter 0.1h, N=3,
+    u = sin(pi x) sin(pi y), Dirichlet, Jacobi-PCG to 1e-10; GPU (FP64) and oracle on the host."""
+    import oracle as O
+    mesh = ip.Mesh.square(32, jitter=0.1, seed=20261202)
+    s = ip.PoissonSolver(mesh, 3)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    s.set_solver(ip.SOLVER_JACOBI)
+    s.set_tol_and_iter(1e-10, 0.0, 5000)
+    s.init()
+    xyz = s.node_coords()
+    u = np.sin(np.pi * xyz[0]) * np.sin(np.pi * xyz[1])
+    b = s.mass(2 * np.pi ** 2 * u)
+    x = np.zeros_like(b)
+    s.solve(b, x)
+    t = []
+    for _ in range(reps):
+        x = np.zeros_like(b)
+        t0 = time.perf_counter()
+        s.solve(b, x)
+        t.append(time.perf_counter() - t0)
+    gpu_its = s.last_stats["iterations"]
+    e = x - u
+    err_gpu = float(np.sqrt(np.sum(e * s.mass(e))))
+    coords, c2n = mesh.arrays()
+    o = O.Oracle(coords, c2n, 3)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    o.set_bc_types(np.zeros(o.n_bfaces, dtype=np.int32))
+    t0 = time.perf_counter()
+    xo, rc, its, rr = o.solve_jacobi(b, rtol=1e-10, max_iter=5000)
+    t_cpu = time.perf_counter() - t0
+    eo = xo - u
+    err_cpu = float(np.sqrt(np.sum(eo * s.mass(eo))))
+    dof = s.n_cells * s.Np
+    return {"workload": "unit square 32x32x2 tris (jitter 0.1h), N=3, Dirichlet, Jacobi-PCG to 1e-10",
+            "dof": dof, "gpu_ms": min(t) * 1e3, "gpu_iterations": gpu_its,
+            "cpu_ms": t_cpu * 1e3, "cpu_iterations": its, "cpu_threads": threads,
+            "l2_error_gpu": err_gpu, "l2_error_cpu": err_cpu,
+            "rel_l2_gpu_vs_oracle": float(np.linalg.norm(x - xo) / np.linalg.norm(xo))}
+
+
 def cpu_baseline(n, order, rtol, coarse_iters):
     import oracle as O
     import ins_poisson as ip
@@ -110,6 +162,7 @@ def cpu_baseline(n, order, rtol, coarse_iters):
     dt = time.perf_counter() - t0
     dof = o.n_cells * o.Np
     return {"value": dof / dt * 1e-9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"oracle p-MG FP64 solve (Jacobi-CG({coarse_iters}) coarse), cube {n}^3x6 tets "
                       f"N={order} ({dof} DoF), "
                       f"rtol {rtol:g}, {it} its, {dt:.1f} s, converged={rc == 0}"}
@@ -199,7 +252,7 @@ def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
     return out
 
 
-def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206):
+def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, peaks=None):
     """Variable-coefficient pure-Neumann cube: div(1/rho grad p) = f, rho = contrast inside a
     sphere of radius 0.25 at the centre (smoothed over 2h), f Gaussian projected to zero mean,
     rel res 1e-8. Runs the plain FP64 path (Jacobi-CG coarse), FP64 with the AMG coarse solve and
@@ -255,7 +308,8 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206):
     for name in ("plain_fp64", "mixed_amg"):
         d = sols[name] - ref
         out[name]["rel_l2_vs_fp64_amg"] = float(np.sqrt(np.sum(d * s.mass(d)))) / nref
-    out["fp32_apply"] = fp32_apply_kernels(ip, torch, s, order)
+    out["fp32_apply"] = fp32_apply_kernels(ip, torch, s, order,
+                                           peak_f32=peaks["mfma_f32_tflops"] if peaks else None)
     out["speedup_mixed_over_plain_fp64"] = out["mixed_amg"]["gdofs"] / out["plain_fp64"]["gdofs"]
     out["speedup_mixed_over_fp64_amg"] = out["mixed_amg"]["gdofs"] / out["fp64_amg"]["gdofs"]
     return out
@@ -273,7 +327,7 @@ def mfma_issued_flops_per_cell(order):
     return per_tile / 16.0
 
 
-def fp32_apply_kernels(ip, torch, s, order, reps=20):
+def fp32_apply_kernels(ip, torch, s, order, reps=20, peak_f32=None):
     """FP32 operator apply of the solver's fine level with each element-pass kernel, timed
     with HIP events on the solver's stream (trace and element pass apart)."""
     res = {}
@@ -303,6 +357,8 @@ def fp32_apply_kernels(ip, torch, s, order, reps=20):
             issued = mfma_issued_flops_per_cell(order) * cells
             r["mfma_issued_tflops"] = issued / apply_ms * 1e-9
             r["mfma_frac_of_fp32_matrix_peak"] = issued / apply_ms * 1e-9 / PEAK_FP32_MATRIX_TFLOPS
+            if peak_f32:
+                r["mfma_frac_of_measured_fp32_matrix_peak"] = issued / apply_ms * 1e-9 / peak_f32
         res[name] = r
     s.set_element_pass(ip.ELEMENT_PASS_AUTO)
     res["speedup_matrix_over_vector"] = res["vector"]["element_pass_ms"] / res["matrix_cores"]["element_pass_ms"]
@@ -474,8 +530,10 @@ def main():
     import torch
     import ins_poisson as ip
 
+    peaks = ip.device_peaks()
+    log(f"device peaks: {peaks}")
     if a.precision_only:
-        res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps)
+        res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps, peaks=peaks)
         print(json.dumps({"precision_form": res}), flush=True)
         return
     if a.copy_probe:
@@ -556,6 +614,10 @@ def main():
         "launches_timed": timing["coarse_apply_launches"],
         "algorithmic_bytes_per_launch": bytes_p1,
     }
+    roofline["measured_peak"] = peaks["copy_gbs"]
+    roofline["frac_of_measured_peak"] = ach_p1 / peaks["copy_gbs"] if ach_p1 else None
+    fine["measured_peak"] = peaks["mfma_f64_tflops"]
+    fine["frac_of_measured_peak"] = achieved_tf / peaks["mfma_f64_tflops"] if achieved_tf else None
     if roofline["traffic"] and t_p1 > 0:
         roofline["traffic_gbs"] = roofline["traffic"] / t_p1 * 1e-9
         roofline["traffic_over_algorithmic"] = roofline["traffic"] / bytes_p1
@@ -591,6 +653,7 @@ def main():
         "smoother_applies_per_solve": stats[-1]["smoother_applies"],
         "roofline": roofline,
         "roofline_fine_apply": fine,
+        "device_peaks_measured": peaks,
     }
     log(f"plain leg: {gdofs:.4f} GDoF/s, its {out['iterations']}")
     if not a.no_mixed_leg:
@@ -603,7 +666,8 @@ def main():
     torch.cuda.empty_cache()
     if not a.skip_precision:
         log("precision form leg")
-        out["precision_form"] = precision_leg(ip, torch, a.precision_n, a.precision_order, 2)
+        out["precision_form"] = precision_leg(ip, torch, a.precision_n, a.precision_order, 2,
+                                              peaks=peaks)
         torch.cuda.empty_cache()
     if not a.skip_batch:
         log("batch leg")
@@ -613,6 +677,8 @@ def main():
         log("serving leg")
         out["serving"] = serve_leg(ip, a.serve_requests, rates)
     if not a.skip_cpu:
+        log("configuration 0 (2D, GPU and oracle)")
+        out["config0"] = config0_leg(ip)
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.order, a.rtol, a.coarse_iters)
     print(json.dumps(out), flush=True)

@@ -206,6 +206,12 @@ int ins_poisson_take_timing(ins_poisson_t h, double* trace_ms, int64_t* trace_la
  * profiling is on; launches counts the timed ones. */
 int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* launches);
 
+/* Achievable peaks of the current device, measured in about a second: streaming copy
+ * bandwidth (GB/s, bytes read + written) and 16x16x4 matrix-core throughput in FP32 and
+ * FP64 (TFLOP/s). No counterpart in the reference: the denominators of the measured
+ * rooflines beside the published figures. Any output may be NULL. */
+int ins_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);
+
 /* Reference-element tables of order `order` (the toolkit's DGConstants: Dr/Ds/Dt, MASS,
  * FMASK, face mass), row-major; any output may be NULL. rst: dim x Np, D: dim x Np x Np,
  * M: Np x Np, fmask: Nfaces x Nfp, Mface: Nfaces x Nfp x Nfp. Host only, no device use. */

@@ -550,6 +550,13 @@ int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* l
   });
 }
 
+int ins_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops) {
+  return guarded([&] {
+    insp::measure_device_peaks(copy_gbs, mfma_f32_tflops, mfma_f64_tflops);
+    return INS_OK;
+  });
+}
+
 int ins_ref_element(int dim, int order, double* rst, double* D, double* M, int* fmask,
                     double* Mface) {
   return guarded([&] {

@@ -76,6 +76,10 @@ void block_invert(int rows, int64_t E, int64_t stride, double* blk, hipStream_t 
 void block_apply(int rows, int64_t E, int64_t stride, const double* blk, const double* r, double* z,
                  hipStream_t s);
 
+// Achievable peaks of the current device (probe_kernels.hip): streaming copy bandwidth
+// (GB/s, read + write) and FP32 / FP64 16x16x4 matrix-core throughput (TFLOP/s).
+void measure_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);
+
 // Pressure rhs of the caller (rhs_kernels.hip): div <- J M (-div/dt) + J EMAT g with
 // g = b0 dPdN + b1 dPdNOld written back to dPdNOld; M is Np x Np, EMAT Np x (Nf*Nfp).
 void launch_pressure_rhs(int dim, int N, int64_t E, int64_t stride, const double* J, const double* M,

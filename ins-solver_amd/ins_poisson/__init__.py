@@ -57,7 +57,7 @@ EXPORTED = [
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
-    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_ref_element", "ins_ref_interp",
+    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
     "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
@@ -146,6 +146,7 @@ def _load():
         "ins_poisson_set_profiling": [vp, C.c_int],
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
         "ins_poisson_take_coarse_timing": [vp, dp, C.POINTER(i64)],
+        "ins_device_peaks": [dp, dp, dp],
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
         "ins_ref_interp": [C.c_int, C.c_int, C.c_int, vp],
         "ins_batch_problem_bytes": [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.POINTER(i64)],
@@ -473,6 +474,13 @@ class PoissonSolver:
 
 
 # --- batched solves and serving ----------------------------------------------
+
+def device_peaks():
+    """Measured copy bandwidth (GB/s) and FP32 / FP64 matrix-core throughput (TFLOP/s)."""
+    c, f, d = C.c_double(), C.c_double(), C.c_double()
+    _check(lib.ins_device_peaks(C.byref(c), C.byref(f), C.byref(d)))
+    return {"copy_gbs": c.value, "mfma_f32_tflops": f.value, "mfma_f64_tflops": d.value}
+
 
 def batch_config(**kw):
     """BatchConfig with library defaults, overridden by keyword (precision, cheb_order, ...)."""
