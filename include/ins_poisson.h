@@ -52,7 +52,8 @@ enum {
 enum { INS_LAYOUT_HOST_AOS = 0, INS_LAYOUT_DEVICE_SOA = 1, INS_LAYOUT_DEVICE_AOS = 2 };
 
 /* Numbering follows LinearSolver::Solvers (src/solvers/2d/ins_solver_base.cpp:1294-1309). */
-enum { INS_SOLVER_JACOBI = 1, INS_SOLVER_BLOCK_JACOBI = 2, INS_SOLVER_PMULTIGRID = 4 };
+enum { INS_SOLVER_JACOBI = 1, INS_SOLVER_BLOCK_JACOBI = 2, INS_SOLVER_INV_MASS = 3,
+       INS_SOLVER_PMULTIGRID = 4 };
 
 enum { INS_PRECISION_FP64 = 0, INS_PRECISION_MIXED = 1 };
 
@@ -116,6 +117,20 @@ int ins_poisson_set_bc_types(ins_poisson_t h, const int* bc_types, int64_t n_bfa
 /* FactorPoissonMatrixFreeDiag::set_factor(op_dat cells x Np) - src/solvers/2d/mp_ins_solver.cpp:577.
  * NULL restores the constant coefficient. */
 int ins_poisson_set_factor(ins_poisson_t h, const double* factor, int layout);
+/* Mass term of the viscous (Helmholtz) operator, A u + J M (mm o u):
+ * FactorMMPoissonMatrixFree{Diag,BlockDiag}2D::set_mm_factor(op_dat cells x Np)
+ * (src/solvers/2d/ins_solver.cpp:506-513, kernel fvmf_2d_mult_mm_geof.h). NULL clears. */
+int ins_poisson_set_mm_factor(ins_poisson_t h, const double* mm, int layout);
+/* Constant mass factor, A u + mm J M u: MMPoissonMatrixFree2D::set_factor(DG_FP)
+ * (src/solvers/2d/ins_solver.cpp:518-520, kernel vmf_2d_mult_mm_geof.h). 0 = none. */
+int ins_poisson_set_mm_const(ins_poisson_t h, double mm);
+/* Scaling of the inverse-mass preconditioner (INS_SOLVER_INV_MASS), z = f (J M)^-1 r:
+ * PETScInvMassSolver::setFactor / ViscousSolver2D::set_inv_mass_factor
+ * (src/solvers/2d/ins_solver.cpp:521, slip_matrix/viscous_solver.cpp:49-51,347-357). */
+int ins_poisson_set_inv_mass_factor(ins_poisson_t h, double f);
+/* Per-node divisor instead, z_i = ((J M)^-1 r)_i / f_i: ViscousSolver2D::set_inv_mass_recp_factor
+ * (slip_matrix/viscous_solver.cpp:53-55,359-366). NULL returns to the scalar factor. */
+int ins_poisson_set_inv_mass_recp_factor(ins_poisson_t h, const double* f, int layout);
 /* LinearSolver::set_bcs(op_dat bfaces x Nfp) - src/solvers/2d/ins_solver.cpp:324. Values are
  * given at the face nodes of each bface in the owner's face-mask order. Layout HOST_AOS or
  * DEVICE_AOS (row per bface). NULL clears. */

@@ -2,6 +2,7 @@
 // status code and records a message for ins_poisson_last_error().
 #include "../../include/ins_poisson.h"
 
+#include <cmath>
 #include <new>
 #include <string>
 
@@ -300,6 +301,35 @@ int ins_poisson_set_factor(ins_poisson_t h, const double* factor, int layout) {
   });
 }
 
+int ins_poisson_set_mm_factor(ins_poisson_t h, const double* mm, int layout) {
+  return guarded([&] {
+    S(h).set_mm_factor(mm, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_mm_const(ins_poisson_t h, double mm) {
+  return guarded([&] {
+    S(h).set_mm_const(mm);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_inv_mass_factor(ins_poisson_t h, double f) {
+  return guarded([&] {
+    if (!std::isfinite(f) || f == 0.0) throw insp::InvalidArgument("inverse-mass factor must be finite and nonzero");
+    S(h).set_inv_mass_factor(f);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_inv_mass_recp_factor(ins_poisson_t h, const double* f, int layout) {
+  return guarded([&] {
+    S(h).set_inv_mass_recp_factor(f, to_layout(layout));
+    return INS_OK;
+  });
+}
+
 int ins_poisson_set_bcs(ins_poisson_t h, const double* bc_data, int layout) {
   return guarded([&] {
     const auto l = to_layout(layout);
@@ -325,7 +355,8 @@ int ins_poisson_set_tol_and_iter(ins_poisson_t h, double rtol, double atol, int 
 
 int ins_poisson_set_solver(ins_poisson_t h, int type) {
   return guarded([&] {
-    if (type != INS_SOLVER_JACOBI && type != INS_SOLVER_BLOCK_JACOBI && type != INS_SOLVER_PMULTIGRID)
+    if (type != INS_SOLVER_JACOBI && type != INS_SOLVER_BLOCK_JACOBI && type != INS_SOLVER_INV_MASS &&
+        type != INS_SOLVER_PMULTIGRID)
       throw insp::InvalidArgument("unsupported solver type");
     S(h).set_solver(static_cast<insp::SolverType>(type));
     return INS_OK;

@@ -76,6 +76,20 @@ void block_invert(int rows, int64_t E, int64_t stride, double* blk, hipStream_t 
 void block_apply(int rows, int64_t E, int64_t stride, const double* blk, const double* r, double* z,
                  hipStream_t s);
 
+// Mass term of the viscous (Helmholtz) operator (mass_kernels.hip): out += J M (mm o u),
+// or mmc J M u when mm is null; adds its share of u.(A u) to `partial` (same blocks as
+// launch_apply). np is the level's node count.
+template <typename T>
+void launch_mass_add(const MeshView& m, const LevelView& l, int np, const T* mm, T mmc, const T* u,
+                     T* out, hipStream_t s, double* partial);
+// diag += mm_i J M_ii (or mmc J M_ii).
+void launch_mass_diag(const MeshView& m, const LevelView& l, int np, const double* mm, double mmc,
+                      double* diag, hipStream_t s);
+// Inverse-mass preconditioner: z = scale (J M)^-1 r, or ((J M)^-1 r)_i / recp_i when recp
+// is not null; Minv is the reference element's inverse mass matrix (np x np, row-major).
+void launch_inv_mass(int np, int64_t E, int64_t stride, const double* J, const double* Minv,
+                     double scale, const double* recp, const double* r, double* z, hipStream_t s);
+
 // Achievable peaks of the current device (probe_kernels.hip): streaming copy bandwidth
 // (GB/s, read + write) and FP32 / FP64 16x16x4 matrix-core throughput (TFLOP/s).
 void measure_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);

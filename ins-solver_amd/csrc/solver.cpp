@@ -28,6 +28,7 @@ struct PoissonSolver::Work {
   DBuf<T> tr;              // face traces
   DBuf<T> inv;             // inverse diagonal
   DBuf<T> k;               // factor (empty = constant)
+  DBuf<T> mm;              // per-node mass-term factor (empty = constant or none)
   DBuf<T> C;               // operator constants
   DBuf<T> frag;            // matrix-core fragments of the element pass (empty: vector pass)
   DBuf<T> P, Pt;           // transfer to/from the next coarser level
@@ -187,6 +188,33 @@ void PoissonSolver::set_factor(const double* k, Layout layout) {
   initialised_ = false;
 }
 
+void PoissonSolver::set_mm_factor(const double* mm, Layout layout) {
+  if (!mm) {
+    var_mm_ = false;
+    mm_.release();
+  } else {
+    mm_.alloc((size_t)np() * stride_);
+    import_vec(mm, mm_.p, layout, np());
+    var_mm_ = true;
+  }
+  initialised_ = false;
+}
+
+void PoissonSolver::set_mm_const(double mm) {
+  if (!std::isfinite(mm)) throw InvalidArgument("mass factor must be finite");
+  mm_const_ = mm;
+  initialised_ = false;
+}
+
+void PoissonSolver::set_inv_mass_recp_factor(const double* f, Layout layout) {
+  if (!f) {
+    inv_mass_recp_.release();
+    return;
+  }
+  inv_mass_recp_.alloc((size_t)np() * stride_);
+  import_vec(f, inv_mass_recp_.p, layout, np());
+}
+
 void PoissonSolver::set_bcs(const double* bc, Layout layout) {
   if (!bc) {
     have_bc_ = false;
@@ -322,6 +350,26 @@ void PoissonSolver::build_levels() {
 
 void PoissonSolver::ensure_factor_levels() {
   const int64_t E = mesh_.n_cells;
+  // per-node mass-term factor on every level (interpolated like the coefficient)
+  for (size_t l = 0; l < levels_.size(); ++l) {
+    Level& lv = *levels_[l];
+    if (!var_mm_) {
+      lv.w64.mm.release();
+      lv.w32.mm.release();
+      continue;
+    }
+    const size_t n = (size_t)lv.L.Np * stride_;
+    lv.w64.mm.alloc(n);
+    if (l == 0) {
+      copy<double>((int64_t)n, mm_.p, lv.w64.mm.p, stream_);
+    } else {
+      Level& F = *levels_[l - 1];
+      launch_elem_matvec<double>(mesh_.dim, lv.N, F.N, E, stride_, F.Ik.p, F.w64.mm.p, lv.w64.mm.p,
+                                 false, stream_);
+    }
+    lv.w32.mm.alloc(n);
+    convert<double, float>((int64_t)n, lv.w64.mm.p, lv.w32.mm.p, stream_);
+  }
   for (size_t l = 0; l < levels_.size(); ++l) {
     Level& lv = *levels_[l];
     if (!var_factor_) {
@@ -351,6 +399,9 @@ void PoissonSolver::compute_diagonals() {
     Level& lv = *levels_[l];
     const int Np = lv.L.Np;
     launch_diag(mv, level_view((int)l, false), var_factor_ ? lv.w64.k.p : nullptr, lv.diag.p, stream_);
+    if (has_mass_term())
+      launch_mass_diag(mv, level_view((int)l, false), Np, var_mm_ ? lv.w64.mm.p : nullptr, mm_const_,
+                       lv.diag.p, stream_);
     lv.w64.inv.alloc((size_t)Np * stride_);
     reciprocal<double>(Np, mesh_.n_cells, stride_, lv.diag.p, lv.w64.inv.p, stream_);
     lv.w32.inv.alloc((size_t)Np * stride_);
@@ -434,6 +485,13 @@ void PoissonSolver::init() {
   if (type_ == SolverType::PMULTIGRID && pmg_.coarse == CoarseType::AMG) build_coarse_space();
   if (type_ == SolverType::BLOCK_JACOBI_PCG) build_block_jacobi();
   else blk_.release();
+  if (type_ == SolverType::INV_MASS_PCG) {
+    // M^-1 = V V^T for the nodal basis; computed here from M itself
+    const std::vector<double> Mi = mat_inverse(fine_re_.M, np());
+    minv_.upload(Mi.data(), Mi.size());
+  } else {
+    minv_.release();
+  }
   INSP_HIP(hipStreamSynchronize(stream_));
   initialised_ = true;
 }
@@ -471,6 +529,9 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
   launch_trace<T>(mv, v, u, k, w.tr.p, stream_);
   if (prof) INSP_HIP(hipEventRecord(e1, stream_));
   launch_apply<T>(mv, v, u, w.tr.p, k, out, stream_, partial);
+  if (has_mass_term())
+    launch_mass_add<T>(mv, v, lv.L.Np, var_mm_ ? w.mm.p : nullptr, (T)mm_const_, u, out, stream_,
+                       partial);
   if (prof) {
     INSP_HIP(hipEventRecord(e2, stream_));
     ev_trace_.push_back({e0, e1});
@@ -842,6 +903,9 @@ void PoissonSolver::precondition(const double* r, double* z) {
     jacobi<double>(n, r, l0.w64.inv.p, z, stream_);
   } else if (type_ == SolverType::BLOCK_JACOBI_PCG) {
     block_apply(np(), mesh_.n_cells, stride_, blk_.p, r, z, stream_);
+  } else if (type_ == SolverType::INV_MASS_PCG) {
+    launch_inv_mass(np(), mesh_.n_cells, stride_, J_.p, minv_.p, inv_mass_factor_,
+                    inv_mass_recp_.empty() ? nullptr : inv_mass_recp_.p, r, z, stream_);
   } else if (prec_ == Precision::FP64) {
     copy<double>(n, r, l0.w64.b.p, stream_);
     vcycle<double>(0);

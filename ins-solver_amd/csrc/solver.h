@@ -79,7 +79,7 @@ struct DBuf {
 // op_layout(); values in double.
 void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<int>& tab);
 
-enum class SolverType { JACOBI_PCG = 1, BLOCK_JACOBI_PCG = 2, PMULTIGRID = 4 };
+enum class SolverType { JACOBI_PCG = 1, BLOCK_JACOBI_PCG = 2, INV_MASS_PCG = 3, PMULTIGRID = 4 };
 // Solve on the coarsest p-multigrid order: Jacobi-CG there, or (AMG) Chebyshev
 // smoothing there and a correction from the continuous vertex space solved by
 // smoothed-aggregation AMG.
@@ -141,6 +141,15 @@ class PoissonSolver {
 
   void set_bc_types(const int* types, int64_t n);
   void set_factor(const double* k, Layout layout);  // nullptr = constant coefficient 1
+  // Mass term of the viscous operator (FactorMMPoissonMatrixFree::set_mm_factor, per node;
+  // MMPoissonMatrixFree::set_factor, constant). nullptr / 0 = none.
+  void set_mm_factor(const double* mm, Layout layout);
+  void set_mm_const(double mm);
+  // Inverse-mass preconditioner scaling: times f (PETScInvMassSolver::setFactor,
+  // ViscousSolver::set_inv_mass_factor) or divided by a per-node factor
+  // (set_inv_mass_recp_factor; nullptr = use the scalar).
+  void set_inv_mass_factor(double f) { inv_mass_factor_ = f; }
+  void set_inv_mass_recp_factor(const double* f, Layout layout);
   void set_bcs(const double* bc, Layout layout);    // bfaces x Nfp, nullptr = none
   void set_nullspace(bool ns) {
     if (ns != nullspace_ && pmg_.coarse == CoarseType::AMG) initialised_ = false;  // coarsest inverse
@@ -235,6 +244,13 @@ class PoissonSolver {
   bool have_bc_ = false;
   bool var_factor_ = false;
   DBuf<double> factor_;  // fine-level factor, SoA
+  bool var_mm_ = false;  // per-node mass-term factor (fine level in mm_, per level in Work::mm)
+  DBuf<double> mm_;
+  double mm_const_ = 0;  // constant mass-term factor, 0 = none
+  double inv_mass_factor_ = 1;
+  DBuf<double> inv_mass_recp_;  // per-node divisor of the inverse-mass preconditioner, or empty
+  DBuf<double> minv_;           // inverse reference mass matrix of the finest order
+  bool has_mass_term() const { return var_mm_ || mm_const_ != 0; }
 
   std::vector<std::unique_ptr<Level>> levels_;
   std::unique_ptr<Amg> amg_;

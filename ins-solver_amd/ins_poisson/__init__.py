@@ -30,6 +30,7 @@ LAYOUT_DEVICE_SOA = 1
 LAYOUT_DEVICE_AOS = 2
 SOLVER_JACOBI = 1
 SOLVER_BLOCK_JACOBI = 2
+SOLVER_INV_MASS = 3
 SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -52,7 +53,8 @@ EXPORTED = [
     "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords",
     "ins_poisson_set_bc_types", "ins_poisson_set_factor", "ins_poisson_set_bcs",
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
-    "ins_poisson_set_precision", "ins_poisson_set_element_pass", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
+    "ins_poisson_set_precision", "ins_poisson_set_element_pass", "ins_poisson_set_mm_factor",
+    "ins_poisson_set_mm_const", "ins_poisson_set_inv_mass_factor", "ins_poisson_set_inv_mass_recp_factor", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
     "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
@@ -126,6 +128,10 @@ def _load():
         "ins_poisson_set_solver": [vp, C.c_int],
         "ins_poisson_set_precision": [vp, C.c_int],
         "ins_poisson_set_element_pass": [vp, C.c_int],
+        "ins_poisson_set_mm_factor": [vp, vp, C.c_int],
+        "ins_poisson_set_mm_const": [vp, C.c_double],
+        "ins_poisson_set_inv_mass_factor": [vp, C.c_double],
+        "ins_poisson_set_inv_mass_recp_factor": [vp, vp, C.c_int],
         "ins_poisson_set_pmg": [vp, vp, C.c_int, vp, vp, vp, dbl, dbl, C.c_int, C.c_int],
         "ins_poisson_set_coarse": [vp, C.c_int, C.c_int],
         "ins_poisson_coarse_matrix": [vp, C.POINTER(i64), C.POINTER(i64), vp, vp, vp],
@@ -328,6 +334,28 @@ class PoissonSolver:
         else:
             f = factor if isinstance(factor, int) else _f64(factor)
             _check(lib.ins_poisson_set_factor(self._h, _ptr(f), layout))
+
+    def set_mm_factor(self, mm, layout=LAYOUT_HOST_AOS):
+        """Per-node mass-term factor of the viscous operator (cells x Np); None clears."""
+        if mm is None:
+            _check(lib.ins_poisson_set_mm_factor(self._h, None, layout))
+        else:
+            f = mm if isinstance(mm, int) else _f64(mm)
+            _check(lib.ins_poisson_set_mm_factor(self._h, _ptr(f), layout))
+
+    def set_mm_const(self, mm):
+        """Constant mass-term factor of the viscous operator; 0 = none."""
+        _check(lib.ins_poisson_set_mm_const(self._h, float(mm)))
+
+    def set_inv_mass_factor(self, f):
+        _check(lib.ins_poisson_set_inv_mass_factor(self._h, float(f)))
+
+    def set_inv_mass_recp_factor(self, f, layout=LAYOUT_HOST_AOS):
+        if f is None:
+            _check(lib.ins_poisson_set_inv_mass_recp_factor(self._h, None, layout))
+        else:
+            a = f if isinstance(f, int) else _f64(f)
+            _check(lib.ins_poisson_set_inv_mass_recp_factor(self._h, _ptr(a), layout))
 
     def set_bcs(self, bc, layout=LAYOUT_HOST_AOS):
         b = None if bc is None else (bc if isinstance(bc, int) else _f64(bc))

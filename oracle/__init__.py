@@ -58,6 +58,12 @@ def _load():
     lib.oracle_ref_element.restype = ci
     lib.oracle_interp.argtypes = [ci, ci, ci, vp]
     lib.oracle_interp.restype = ci
+    lib.oracle_set_mm_factor.argtypes = [vp, vp]
+    lib.oracle_set_mm_factor.restype = None
+    lib.oracle_set_mm_const.argtypes = [vp, d]
+    lib.oracle_set_mm_const.restype = None
+    lib.oracle_solve_inv_mass.argtypes = [vp, vp, vp, d, d, ci, d, vp, C.POINTER(ci), C.POINTER(d)]
+    lib.oracle_solve_inv_mass.restype = ci
     return lib
 
 
@@ -148,6 +154,26 @@ class Oracle:
     def set_factor(self, k):
         k = None if k is None else np.ascontiguousarray(k, dtype=np.float64)
         lib.oracle_set_factor(self._h, _p(k))
+
+    def set_mm_factor(self, mm):
+        """Per-node mass-term factor of the viscous operator (cells x Np), None = none."""
+        mm = None if mm is None else np.ascontiguousarray(mm, dtype=np.float64)
+        lib.oracle_set_mm_factor(self._h, _p(mm))
+
+    def set_mm_const(self, mm):
+        """Constant mass-term factor of the viscous operator, 0 = none."""
+        lib.oracle_set_mm_const(self._h, float(mm))
+
+    def solve_inv_mass(self, rhs, x0=None, rtol=1e-10, atol=0.0, max_iter=5000, factor=1.0,
+                       recp_factor=None):
+        """Flexible PCG with the inverse-mass preconditioner (times factor, or divided by
+        the per-node recp_factor)."""
+        x = np.zeros(self.n_cells * self.Np) if x0 is None else self._vec(x0).copy()
+        rf = None if recp_factor is None else self._vec(recp_factor)
+        it, rr = C.c_int(), C.c_double()
+        rc = lib.oracle_solve_inv_mass(self._h, _p(self._vec(rhs)), _p(x), rtol, atol, max_iter,
+                                       float(factor), _p(rf), C.byref(it), C.byref(rr))
+        return x.reshape(self.n_cells, self.Np), rc, it.value, rr.value
 
     def _vec(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -13,6 +13,10 @@
  *    O(h^{N+1}) manufactured-solution convergence;
  *  - solvers: dense Cholesky solve on tiny meshes; p-MG converges to the same
  *    solution as Jacobi-PCG.
+ *  - mass term: A.1 - A_0.1 = J M mm exactly (Neumann, where the SIPG part of A.1
+ *    vanishes), its diagonal is the diagonal of the assembled term, Helmholtz
+ *    manufactured solutions converge at O(h^{N+1}); inverse-mass PCG reaches the
+ *    dense solution.
  */
 #define _GNU_SOURCE
 #include "ins_oracle.h"
@@ -419,6 +423,8 @@ struct oracle {
   int* cfb;      /* E x Nf: bface index of a cell face, or -1 */
   double* k;
   int pen_order; /* order in the penalty 2(N+1)(N+2) */
+  double* mm;    /* E x Np mass-term factor (FactorMMPoissonMatrixFree) or NULL */
+  double mmc;    /* constant mass-term factor (MMPoissonMatrixFree), 0 = none */
 };
 
 static int g_threads = 0;
@@ -607,7 +613,7 @@ void oracle_destroy(oracle_t* o) {
   if (!o) return;
   free(o->coords); free(o->c2n); free(o->geof); free(o->nrm); free(o->sJ); free(o->fsc);
   free(o->f2c); free(o->fnum); free(o->b2c); free(o->bnum); free(o->bct);
-  free(o->mapLpos); free(o->cfb); free(o->k);
+  free(o->mapLpos); free(o->cfb); free(o->k); free(o->mm);
   free(o);
 }
 
@@ -631,6 +637,16 @@ void oracle_set_factor(oracle_t* o, const double* k) {
     memcpy(o->k, k, sizeof(double) * o->E * o->R.Np);
   }
 }
+
+void oracle_set_mm_factor(oracle_t* o, const double* mm) {
+  free(o->mm);
+  o->mm = NULL;
+  if (mm) {
+    o->mm = malloc(sizeof(double) * o->E * o->R.Np);
+    memcpy(o->mm, mm, sizeof(double) * o->E * o->R.Np);
+  }
+}
+void oracle_set_mm_const(oracle_t* o, double mm) { o->mmc = mm; }
 
 void oracle_node_coords(const oracle_t* o, double* xyz) {
   const int dim = o->dim, nv = dim + 1, Np = o->R.Np;
@@ -787,6 +803,36 @@ static void apply_op(const oracle_t* o, const refel* R, const int* map, const do
   free(q); free(jump); free(avg);
 }
 
+/* Mass term of the viscous (Helmholtz) operator on the finest order:
+ * per-node factor  out_m += J sum_n mm_n M_mn u_n   (fvmf_2d_mult_mm_geof.h:1-26),
+ * constant factor  out_m += mm J sum_n M_mn u_n     (vmf_2d_mult_mm_geof.h:1-26). */
+static void mass_term(const oracle_t* o, const double* in, double* out) {
+  if (!o->mm && o->mmc == 0.0) return;
+  const int Np = o->R.Np, ng = o->dim * o->dim + 1;
+  OMP_FOR
+  for (int64_t e = 0; e < o->E; ++e) {
+    const double J = o->geof[e * ng + o->dim * o->dim];
+    const double* u = in + e * Np;
+    for (int m = 0; m < Np; ++m) {
+      double a = 0;
+      for (int n = 0; n < Np; ++n)
+        a += (o->mm ? o->mm[e * Np + n] : o->mmc) * o->R.M[m * Np + n] * u[n];
+      out[e * Np + m] += a * J;
+    }
+  }
+}
+
+/* Its diagonal: diag_i += mm_i J M_ii (fvmf_2d_mm_diag.h:1-9). */
+static void mass_diag(const oracle_t* o, double* diag) {
+  if (!o->mm && o->mmc == 0.0) return;
+  const int Np = o->R.Np, ng = o->dim * o->dim + 1;
+  for (int64_t e = 0; e < o->E; ++e) {
+    const double J = o->geof[e * ng + o->dim * o->dim];
+    for (int i = 0; i < Np; ++i)
+      diag[e * Np + i] += (o->mm ? o->mm[e * Np + i] : o->mmc) * J * o->R.M[i * Np + i];
+  }
+}
+
 /* normal-derivative matrix rows scaled by k: Dn[i][j] = k_i * n.(grad l_j)(x_i) */
 static void normal_dmat(const oracle_t* o, const refel* R, int64_t e, int f, const double* k, double* Dn) {
   const int dim = o->dim, Np = R->Np, Nf = R->Nf, ng = dim * dim + 1;
@@ -870,12 +916,14 @@ static double* ref_tau(const oracle_t* o, const refel* R, const int* map, const 
 void oracle_mult(oracle_t* o, const double* in, double* out) {
   double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
   apply_op(o, &o->R, o->mapLpos, o->k, tau, in, out);
+  mass_term(o, in, out);
   free(tau);
 }
 
 void oracle_diag(oracle_t* o, int consistent, double* out) {
   double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
   calc_diag(o, &o->R, o->k, tau, consistent, out);
+  mass_diag(o, out);
   free(tau);
 }
 
@@ -971,8 +1019,10 @@ static int fcg(oracle_t* o, const refel* R, const int* map, const double* k, con
   const double tol = fmax(rtol * bn, atol);
   int it = 0, status = 1;
   double rr = 0;
+  const int fine = R->Np == o->R.Np;  /* the mass term belongs to the finest order */
   for (int restart = 0; restart < 50; ++restart) {
     apply_op(o, R, map, k, tau, x, Ap);
+    if (fine) mass_term(o, x, Ap);
     for (int64_t i = 0; i < n; ++i) r[i] = b[i] - Ap[i];
     rr = vdot(n, r, r);
     if (!isfinite(rr)) { status = -3; break; }
@@ -985,6 +1035,7 @@ static int fcg(oracle_t* o, const refel* R, const int* map, const double* k, con
     int bd = 0;
     while (it < max_iter) {
       apply_op(o, R, map, k, tau, p, Ap);
+      if (fine) mass_term(o, p, Ap);
       const double pap = vdot(n, p, Ap);
       if (!(pap > 0)) { bd = 1; break; }
       const double alpha = rz / pap;
@@ -1024,10 +1075,45 @@ int oracle_solve_jacobi(oracle_t* o, const double* rhs, double* x, double rtol, 
   double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
   double* d = malloc(sizeof(double) * n);
   calc_diag(o, &o->R, o->k, tau, 1, d);
+  mass_diag(o, d);
   for (int64_t i = 0; i < n; ++i) d[i] = 1.0 / d[i];
   jac_ctx c = {n, d};
   const int rc = fcg(o, &o->R, o->mapLpos, o->k, tau, rhs, x, rtol, atol, max_iter, nullspace, jac_precond, &c, iters, relres);
   free(tau); free(d);
+  return rc;
+}
+
+/* Inverse-mass preconditioner of the viscous solve: z = (J M)^-1 r per cell
+ * (mesh->inv_mass, viscous_solver.cpp:334-345), times inv_mass_factor
+ * (pre_factor_inv_mass, :347-357) or divided by a per-node factor
+ * (pre_recp_dat_factor_inv_mass, :359-366; cg_recp_factor_inv_mass.h). */
+typedef struct { const oracle_t* o; const double* Minv; double f; const double* recp; } im_ctx;
+static void im_precond(void* vc, const double* r, double* z) {
+  const im_ctx* c = vc;
+  const oracle_t* o = c->o;
+  const int Np = o->R.Np, ng = o->dim * o->dim + 1;
+  for (int64_t e = 0; e < o->E; ++e) {
+    const double J = o->geof[e * ng + o->dim * o->dim];
+    for (int i = 0; i < Np; ++i) {
+      double a = 0;
+      for (int j = 0; j < Np; ++j) a += c->Minv[i * Np + j] * r[e * Np + j];
+      a /= J;
+      z[e * Np + i] = c->recp ? a / c->recp[e * Np + i] : a * c->f;
+    }
+  }
+}
+
+int oracle_solve_inv_mass(oracle_t* o, const double* rhs, double* x, double rtol, double atol,
+                          int max_iter, double inv_mass_factor, const double* recp_factor,
+                          int* iters, double* relres) {
+  const int Np = o->R.Np;
+  double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
+  double* Minv = malloc(sizeof(double) * Np * Np);
+  if (inv_dense(o->R.M, Np, Minv) != 0) { free(tau); free(Minv); return -1; }
+  im_ctx c = {o, Minv, inv_mass_factor, recp_factor};
+  const int rc = fcg(o, &o->R, o->mapLpos, o->k, tau, rhs, x, rtol, atol, max_iter, 0, im_precond, &c,
+                     iters, relres);
+  free(tau); free(Minv);
   return rc;
 }
 

@@ -37,6 +37,15 @@ void oracle_node_coords(const oracle_t* o, double* xyz); /* dim x cells x Np */
 void oracle_set_bc_types(oracle_t* o, const int* bc_types);
 void oracle_set_factor(oracle_t* o, const double* k); /* cells x Np, NULL = 1 */
 void oracle_set_threads(int n);
+/* Mass term of the viscous (Helmholtz) operator on the finest order: per-node factor
+ * (cells x Np, NULL = none) or a constant (0 = none). */
+void oracle_set_mm_factor(oracle_t* o, const double* mm);
+void oracle_set_mm_const(oracle_t* o, double mm);
+/* Flexible PCG preconditioned by the inverse mass matrix, scaled by inv_mass_factor or
+ * divided by recp_factor (cells x Np) when that is not NULL. */
+int oracle_solve_inv_mass(oracle_t* o, const double* rhs, double* x, double rtol, double atol,
+                          int max_iter, double inv_mass_factor, const double* recp_factor,
+                          int* iters, double* relres);
 /* Order used in the penalty 2(N+1)(N+2)fscale (default: the element order). */
 void oracle_set_penalty_order(oracle_t* o, int n);
 
