@@ -13,6 +13,8 @@ Prints one JSON line. `value` is GDoF/s = DoF x solves / wall time.
 Beside it the line carries the other forms of the same solve:
   mixed_vs_fp64  the same cube at rel res 1e-8, FP64 V-cycle vs FP32 V-cycle;
   coarse_amg     the same cube with the vertex-space AMG coarse solve;
+  viscous        the velocity step's Helmholtz solve on the same cube (mass term
+                 g0 Re/dt, inverse-mass PCG, rel res 1e-8);
   precision_form 48^3x6 tets, N=4, div(1/rho grad p) = f with rho = 1000 inside a
                  sphere, pure Neumann, rel res 1e-8: plain FP64 path, FP64 with the
                  AMG coarse solve, FP32 V-cycle inside the FP64 flexible CG;
@@ -83,6 +85,7 @@ def parse():
     ap.add_argument("--precision-n", type=int, default=48, help="cube size of the precision form")
     ap.add_argument("--precision-order", type=int, default=4)
     ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--skip-viscous", action="store_true")
     ap.add_argument("--precision-only", action="store_true",
                     help="run only the precision-form leg and print its JSON (a profiling aid)")
     ap.add_argument("--batch-problems", type=int, default=256)
@@ -313,6 +316,46 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
     out["speedup_mixed_over_plain_fp64"] = out["mixed_amg"]["gdofs"] / out["plain_fp64"]["gdofs"]
     out["speedup_mixed_over_fp64_amg"] = out["mixed_amg"]["gdofs"] / out["fp64_amg"]["gdofs"]
     return out
+
+
+def viscous_leg(ip, torch, n, order, steps, reynolds=100.0, g0=1.5, seed=20261207):
+    """The viscous (Helmholtz) solve of the velocity step: -lap u + (g0 Re/dt) u = f on the
+    Kuhn cube, Dirichlet, dt = h/(N+1)^2 (CFL 1 at unit velocity), inverse-mass PCG (factor
+    dt/(g0 Re)) to rel res 1e-8 (the reference's viscous-solve r_tol),
+    u = sin(pi x) sin(pi y) sin(pi z)."""
+    mesh = ip.Mesh.cube(n, 0.1, seed)
+    s = ip.PoissonSolver(mesh, order)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    dt = (1.0 / n) / (order + 1) ** 2
+    mm = g0 * reynolds / dt
+    s.set_mm_const(mm)
+    s.set_solver(ip.SOLVER_INV_MASS)
+    s.set_inv_mass_factor(1.0 / mm)
+    s.set_tol_and_iter(1e-8, 0.0, 1000)
+    s.init()
+    xyz = s.node_coords()
+    u, f = manufactured_sin(xyz)
+    b = s.mass(f + mm * u)
+    dof = s.n_cells * s.Np
+
+    def to_soa(x):
+        soa = np.zeros((s.Np, s.stride))
+        soa[:, :s.n_cells] = x.T
+        return torch.from_numpy(soa).cuda()
+
+    b_d = to_soa(b)
+    x_d = torch.zeros_like(b_d)
+    wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1)
+    x = x_d.cpu().numpy()[:, :s.n_cells].T
+    e = x - u
+    err = float(np.sqrt(np.sum(e * s.mass(e))) / np.sqrt(np.sum(u * s.mass(u))))
+    return {"workload": f"cube {n}^3x6 Kuhn tets, N={order}, -lap u + (g0 Re/dt) u = f, Re={reynolds:g}, "
+                        f"dt={dt:.3g}, Dirichlet, inverse-mass PCG, FP64, rel res 1e-8",
+            "dof": dof, "mass_factor": mm, "gdofs": dof * steps / wall * 1e-9,
+            "ms_per_solve": wall / steps * 1e3, "iterations": [st["iterations"] for st in stats],
+            "rel_residual": max(st["rel_residual"] for st in stats),
+            "converged": all(st["converged"] for st in stats),
+            "relative_discretisation_error": err}
 
 
 def mfma_issued_flops_per_cell(order):
@@ -664,6 +707,10 @@ def main():
         out["coarse_amg"] = coarse_amg_leg(ip, torch, s, b_d, x_d, dof, 2)
     del s, b_d, x_d, mesh
     torch.cuda.empty_cache()
+    if not a.skip_viscous:
+        log("viscous (Helmholtz) leg")
+        out["viscous"] = viscous_leg(ip, torch, a.n, a.order, 3)
+        torch.cuda.empty_cache()
     if not a.skip_precision:
         log("precision form leg")
         out["precision_form"] = precision_leg(ip, torch, a.precision_n, a.precision_order, 2,
