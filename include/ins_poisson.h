@@ -221,6 +221,17 @@ int ins_poisson_take_timing(ins_poisson_t h, double* trace_ms, int64_t* trace_la
  * profiling is on; launches counts the timed ones. */
 int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* launches);
 
+/* Gradient of a nodal field of the solver's order with a central flux:
+ * DGMesh::grad_with_central_flux (src/solvers/2d/ins_solver.cpp:337; toolkit code not vendored:
+ * strong form, p* = {p} on interior faces, p* = p- on boundary faces). dpdz is unused in 2D. */
+int ins_poisson_grad_central(ins_poisson_t h, const double* p, double* dpdx, double* dpdy, double* dpdz,
+                             int layout);
+/* Velocity update of the projection, uTT = uT - dt grad_h p (ins_2d_proj_rhs.h:1-10, after
+ * grad_with_central_flux at src/solvers/2d/ins_solver.cpp:337-339); uTz/uTTz unused in 2D. */
+int ins_poisson_project_velocity(ins_poisson_t h, double dt, const double* p, const double* uTx,
+                                 const double* uTy, const double* uTz, double* uTTx, double* uTTy,
+                                 double* uTTz, int layout);
+
 /* Achievable peaks of the current device, measured in about a second: streaming copy
  * bandwidth (GB/s, bytes read + written) and 16x16x4 matrix-core throughput in FP32 and
  * FP64 (TFLOP/s). No counterpart in the reference: the denominators of the measured

@@ -581,6 +581,26 @@ int ins_poisson_take_coarse_timing(ins_poisson_t h, double* apply_ms, int64_t* l
   });
 }
 
+int ins_poisson_grad_central(ins_poisson_t h, const double* p, double* dpdx, double* dpdy, double* dpdz,
+                             int layout) {
+  return guarded([&] {
+    double* const g[3] = {dpdx, dpdy, dpdz};
+    S(h).grad_central(p, g, nullptr, 0.0, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_project_velocity(ins_poisson_t h, double dt, const double* p, const double* uTx,
+                                 const double* uTy, const double* uTz, double* uTTx, double* uTTy,
+                                 double* uTTz, int layout) {
+  return guarded([&] {
+    double* const g[3] = {uTTx, uTTy, uTTz};
+    const double* const u[3] = {uTx, uTy, uTz};
+    S(h).grad_central(p, g, u, dt, to_layout(layout));
+    return INS_OK;
+  });
+}
+
 int ins_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops) {
   return guarded([&] {
     insp::measure_device_peaks(copy_gbs, mfma_f32_tflops, mfma_f64_tflops);

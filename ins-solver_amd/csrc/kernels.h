@@ -90,6 +90,14 @@ void launch_mass_diag(const MeshView& m, const LevelView& l, int np, const doubl
 void launch_inv_mass(int np, int64_t E, int64_t stride, const double* J, const double* Minv,
                      double scale, const double* recp, const double* r, double* z, hipStream_t s);
 
+// Gradient with central flux (proj_kernels.hip): out[x] = grad_h p, or u[x] - dt grad_h p
+// when u[x] is not null (x < dim; unused entries null). G: grad(xi) per cell, SoA rows
+// (r*dim + x); lift: Np x (Nf*Nfp) = M^-1 EMAT of the order.
+void launch_grad_central(int dim, int N, int64_t E, int64_t stride, const double* G, const int* nbr,
+                         const int* code, const double* C64, const int* tab, const double* lift,
+                         const double* p, double dt, const double* const u[3], double* const out[3],
+                         hipStream_t s);
+
 // Achievable peaks of the current device (probe_kernels.hip): streaming copy bandwidth
 // (GB/s, read + write) and FP32 / FP64 16x16x4 matrix-core throughput (TFLOP/s).
 void measure_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);

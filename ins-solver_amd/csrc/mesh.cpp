@@ -205,6 +205,51 @@ void build_topology(HostMesh& m) {
   }
 }
 
+namespace {
+// Affine map of cell e: J = det(dx/dxi) and G[a][x] = d xi_a / d x.
+void cell_metric(const HostMesh& m, int64_t e, double G[3][3], double& J) {
+  const int d = m.dim, nv = d + 1;
+  const int* c = &m.cell2nodes[(size_t)e * nv];
+  double xr[3][3];  // xr[ref dir][phys coord]
+  for (int a = 0; a < d; ++a)
+    for (int x = 0; x < d; ++x)
+      xr[a][x] = 0.5 * (m.coords[(size_t)c[a + 1] * d + x] - m.coords[(size_t)c[0] * d + x]);
+  if (d == 2) {
+    J = xr[0][0] * xr[1][1] - xr[1][0] * xr[0][1];
+    G[0][0] = xr[1][1] / J;
+    G[0][1] = -xr[1][0] / J;
+    G[1][0] = -xr[0][1] / J;
+    G[1][1] = xr[0][0] / J;
+  } else {
+    auto cross = [](const double* a, const double* b, double* o) {
+      o[0] = a[1] * b[2] - a[2] * b[1];
+      o[1] = a[2] * b[0] - a[0] * b[2];
+      o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double c12[3], c20[3], c01[3];
+    cross(xr[1], xr[2], c12);
+    cross(xr[2], xr[0], c20);
+    cross(xr[0], xr[1], c01);
+    J = xr[0][0] * c12[0] + xr[0][1] * c12[1] + xr[0][2] * c12[2];
+    for (int x = 0; x < 3; ++x) {
+      G[0][x] = c12[x] / J;
+      G[1][x] = c20[x] / J;
+      G[2][x] = c01[x] / J;
+    }
+  }
+}
+}  // namespace
+
+void mesh_inverse_jacobians(const HostMesh& m, double* G) {
+  const int d = m.dim;
+  for (int64_t e = 0; e < m.n_cells; ++e) {
+    double g[3][3], J;
+    cell_metric(m, e, g, J);
+    for (int a = 0; a < d; ++a)
+      for (int x = 0; x < d; ++x) G[((size_t)e * d + a) * d + x] = g[a][x];
+  }
+}
+
 void build_geometry(HostMesh& m) {
   const int d = m.dim, nv = d + 1;
   const int nk = d == 2 ? 3 : 6;

@@ -43,6 +43,9 @@ HostMesh make_cube_mesh(int n, double jitter, uint64_t seed);
 void build_topology(HostMesh& m);
 void build_geometry(HostMesh& m);
 
+// d xi_a / d x_x of every cell, cells x d x d.
+void mesh_inverse_jacobians(const HostMesh& m, double* G);
+
 struct RefElement;
 // Nodal coordinates of an order-N element set, dim x cells x Np.
 void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz);

@@ -215,6 +215,60 @@ void PoissonSolver::set_inv_mass_recp_factor(const double* f, Layout layout) {
   import_vec(f, inv_mass_recp_.p, layout, np());
 }
 
+void PoissonSolver::grad_central(const double* p, double* const g[3], const double* const u[3], double dt,
+                                 Layout layout) {
+  if (!initialised_) init();
+  const int d = mesh_.dim, Np = np(), Nf = d + 1, Nfp = nfp();
+  if (!p) throw InvalidArgument("null pressure");
+  for (int x = 0; x < d; ++x)
+    if (!g[x]) throw InvalidArgument("null gradient component");
+  if (u && !std::isfinite(dt)) throw InvalidArgument("dt must be finite");
+  const int64_t E = mesh_.n_cells;
+  if (grad_xi_.empty()) {
+    std::vector<double> G((size_t)E * d * d), soa((size_t)d * d * stride_, 0.0);
+    mesh_inverse_jacobians(mesh_, G.data());
+    for (int64_t e = 0; e < E; ++e)
+      for (int k = 0; k < d * d; ++k) soa[(size_t)k * stride_ + e] = G[(size_t)e * d * d + k];
+    grad_xi_.upload(soa.data(), soa.size());
+    // LIFT = M^-1 EMAT, EMAT[i][(f, j)] = face mass of face f at volume node i
+    const std::vector<double> Mi = mat_inverse(fine_re_.M, Np);
+    const int NF = Nf * Nfp;
+    std::vector<double> lift((size_t)Np * NF, 0.0);
+    for (int i = 0; i < Np; ++i)
+      for (int f = 0; f < Nf; ++f)
+        for (int j = 0; j < Nfp; ++j) {
+          double a = 0;
+          for (int l = 0; l < Nfp; ++l)
+            a += Mi[(size_t)i * Np + fine_re_.fmask[f * Nfp + l]] *
+                 fine_re_.Mface[((size_t)f * Nfp + l) * Nfp + j];
+          lift[(size_t)i * NF + f * Nfp + j] = a;
+        }
+    lift_.upload(lift.data(), lift.size());
+  }
+  const size_t n = (size_t)Np * stride_;
+  DBuf<double> pd;
+  pd.alloc(n);
+  import_vec(p, pd.p, layout, Np);
+  DBuf<double> ud[3], od[3];
+  const double* up[3] = {nullptr, nullptr, nullptr};
+  double* op[3] = {nullptr, nullptr, nullptr};
+  for (int x = 0; x < d; ++x) {
+    od[x].alloc(n);
+    op[x] = od[x].p;
+    if (u) {
+      if (!u[x]) throw InvalidArgument("null velocity component");
+      ud[x].alloc(n);
+      import_vec(u[x], ud[x].p, layout, Np);
+      up[x] = ud[x].p;
+    }
+  }
+  const Level& l0 = *levels_[0];
+  launch_grad_central(d, N_, E, stride_, grad_xi_.p, nbr_.p, code_.p, l0.C64.p, l0.tab.p, lift_.p, pd.p,
+                      dt, up, op, stream_);
+  for (int x = 0; x < d; ++x) export_vec(od[x].p, g[x], layout, Np);
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
 void PoissonSolver::set_bcs(const double* bc, Layout layout) {
   if (!bc) {
     have_bc_ = false;

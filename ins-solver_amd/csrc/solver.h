@@ -150,6 +150,10 @@ class PoissonSolver {
   // (set_inv_mass_recp_factor; nullptr = use the scalar).
   void set_inv_mass_factor(double f) { inv_mass_factor_ = f; }
   void set_inv_mass_recp_factor(const double* f, Layout layout);
+  // Pressure projection (DGMesh::grad_with_central_flux + ins_2d_proj_rhs): g[x] = grad_h p,
+  // or g[x] = u[x] - dt grad_h p when u is given; vectors of the solver's order.
+  void grad_central(const double* p, double* const g[3], const double* const u[3], double dt,
+                    Layout layout);
   void set_bcs(const double* bc, Layout layout);    // bfaces x Nfp, nullptr = none
   void set_nullspace(bool ns) {
     if (ns != nullspace_ && pmg_.coarse == CoarseType::AMG) initialised_ = false;  // coarsest inverse
@@ -250,6 +254,8 @@ class PoissonSolver {
   double inv_mass_factor_ = 1;
   DBuf<double> inv_mass_recp_;  // per-node divisor of the inverse-mass preconditioner, or empty
   DBuf<double> minv_;           // inverse reference mass matrix of the finest order
+  DBuf<double> grad_xi_;        // d xi_r / d x_x per cell, SoA rows r*dim + x (built on first use)
+  DBuf<double> lift_;           // M^-1 EMAT of the finest order, Np x (Nf*Nfp)
   bool has_mass_term() const { return var_mm_ || mm_const_ != 0; }
 
   std::vector<std::unique_ptr<Level>> levels_;

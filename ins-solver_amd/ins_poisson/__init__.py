@@ -59,7 +59,7 @@ EXPORTED = [
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
-    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_ref_element", "ins_ref_interp",
+    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_poisson_grad_central", "ins_poisson_project_velocity", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
     "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
@@ -153,6 +153,8 @@ def _load():
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
         "ins_poisson_take_coarse_timing": [vp, dp, C.POINTER(i64)],
         "ins_device_peaks": [dp, dp, dp],
+        "ins_poisson_grad_central": [vp, vp, vp, vp, vp, C.c_int],
+        "ins_poisson_project_velocity": [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, C.c_int],
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
         "ins_ref_interp": [C.c_int, C.c_int, C.c_int, vp],
         "ins_batch_problem_bytes": [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.POINTER(i64)],
@@ -334,6 +336,26 @@ class PoissonSolver:
         else:
             f = factor if isinstance(factor, int) else _f64(factor)
             _check(lib.ins_poisson_set_factor(self._h, _ptr(f), layout))
+
+    def grad_central(self, p):
+        """Gradient with central flux of a host AoS field (cells x Np): (dim, cells, Np)."""
+        pa = _f64(p)
+        g = np.zeros((3, self.n_cells, self.Np))
+        _check(lib.ins_poisson_grad_central(self._h, _ptr(pa), _ptr(g[0]), _ptr(g[1]),
+                                            _ptr(g[2]) if self.dim == 3 else None, LAYOUT_HOST_AOS))
+        return g[:self.dim]
+
+    def project_velocity(self, dt, p, uT):
+        """uTT = uT - dt grad_h p for host AoS fields; uT is (dim, cells, Np)."""
+        pa = _f64(p)
+        u = [_f64(uT[x]) for x in range(self.dim)]
+        out = np.zeros((3, self.n_cells, self.Np))
+        uz = _ptr(u[2]) if self.dim == 3 else None
+        _check(lib.ins_poisson_project_velocity(self._h, float(dt), _ptr(pa), _ptr(u[0]), _ptr(u[1]), uz,
+                                                _ptr(out[0]), _ptr(out[1]),
+                                                _ptr(out[2]) if self.dim == 3 else None,
+                                                LAYOUT_HOST_AOS))
+        return out[:self.dim]
 
     def set_mm_factor(self, mm, layout=LAYOUT_HOST_AOS):
         """Per-node mass-term factor of the viscous operator (cells x Np); None clears."""

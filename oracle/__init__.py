@@ -58,6 +58,8 @@ def _load():
     lib.oracle_ref_element.restype = ci
     lib.oracle_interp.argtypes = [ci, ci, ci, vp]
     lib.oracle_interp.restype = ci
+    lib.oracle_grad_central.argtypes = [vp, vp, vp, vp, vp]
+    lib.oracle_grad_central.restype = None
     lib.oracle_set_mm_factor.argtypes = [vp, vp]
     lib.oracle_set_mm_factor.restype = None
     lib.oracle_set_mm_const.argtypes = [vp, d]
@@ -154,6 +156,13 @@ class Oracle:
     def set_factor(self, k):
         k = None if k is None else np.ascontiguousarray(k, dtype=np.float64)
         lib.oracle_set_factor(self._h, _p(k))
+
+    def grad_central(self, p):
+        """Gradient with central flux: (dim, cells, Np)."""
+        p = self._vec(p)
+        g = np.zeros((3, self.n_cells, self.Np))
+        lib.oracle_grad_central(self._h, _p(p), _p(g[0]), _p(g[1]), _p(g[2]))
+        return g[:self.dim]
 
     def set_mm_factor(self, mm):
         """Per-node mass-term factor of the viscous operator (cells x Np), None = none."""

@@ -985,6 +985,67 @@ double oracle_l2sq(oracle_t* o, const double* e) {
   return s;
 }
 
+/* Gradient with central flux (DGMesh::grad_with_central_flux of the un-vendored toolkit,
+ * called at src/solvers/2d/ins_solver.cpp:337 before project_velocity). Parity unpinned
+ * against the toolkit's code; this restates the textbook strong form (Hesthaven & Warburton,
+ * Nodal DG Methods, 2008, sec. 7.1 with a central flux):
+ *   grad_h p = sum_r grad(xi_r) D_r p + M^-1 sum_f MF_f (fscale_f n_f (p* - p-)),
+ *   p* = (p- + p+)/2 on interior faces and p* = p- on boundary faces. */
+void oracle_grad_central(oracle_t* o, const double* p, double* gx, double* gy, double* gz) {
+  const refel* R = &o->R;
+  const int dim = o->dim, Np = R->Np, Nfp = R->Nfp, Nf = R->Nf, NF = Nf * Nfp, ng = dim * dim + 1;
+  const int64_t E = o->E;
+  double* out[3] = {gx, gy, gz};
+  double Minv[MAXNP * MAXNP];
+  inv_dense(R->M, Np, Minv);
+  double* jump = calloc((size_t)E * NF, sizeof(double)); /* p* - p- = (p+ - p-)/2 */
+  for (int64_t f = 0; f < o->F; ++f) {
+    const int L = o->f2c[2 * f], Rc = o->f2c[2 * f + 1], fL = o->fnum[2 * f], fR = o->fnum[2 * f + 1];
+    for (int j = 0; j < Nfp; ++j) {
+      const int iL = R->fmask[fL * Nfp + j];
+      const int pR = o->mapLpos[f * Nfp + j];
+      const int iR = R->fmask[fR * Nfp + pR];
+      const double d = 0.5 * (p[(size_t)Rc * Np + iR] - p[(size_t)L * Np + iL]);
+      jump[(size_t)L * NF + fL * Nfp + j] = d;
+      jump[(size_t)Rc * NF + fR * Nfp + pR] = -d;
+    }
+  }
+  OMP_FOR
+  for (int64_t e = 0; e < E; ++e) {
+    const double* g = o->geof + e * ng;
+    const double* pe = p + e * Np;
+    double pr[3][MAXNP];
+    for (int r = 0; r < dim; ++r)
+      for (int i = 0; i < Np; ++i) {
+        double a = 0;
+        for (int j = 0; j < Np; ++j) a += R->D[r][i * Np + j] * pe[j];
+        pr[r][i] = a;
+      }
+    for (int x = 0; x < dim; ++x) {
+      double flux[MAXNP];
+      for (int i = 0; i < Np; ++i) flux[i] = 0;
+      for (int f = 0; f < Nf; ++f) {
+        const double c = o->fsc[e * Nf + f] * o->nrm[(e * Nf + f) * dim + x];
+        double v[MAXNP];
+        for (int i = 0; i < Np; ++i) v[i] = 0;
+        for (int j = 0; j < Nfp; ++j) v[R->fmask[f * Nfp + j]] = c * jump[(size_t)e * NF + f * Nfp + j];
+        for (int i = 0; i < Np; ++i) {
+          double a = 0;
+          for (int k = 0; k < Np; ++k) a += R->MF[f][i * Np + k] * v[k];
+          flux[i] += a;
+        }
+      }
+      for (int i = 0; i < Np; ++i) {
+        double a = 0, b = 0;
+        for (int r = 0; r < dim; ++r) a += g[r * dim + x] * pr[r][i];
+        for (int k = 0; k < Np; ++k) b += Minv[i * Np + k] * flux[k];
+        out[x][e * Np + i] = a + b;
+      }
+    }
+  }
+  free(jump);
+}
+
 /* ------------------------------------------------------------------ */
 /* solvers                                                             */
 /* ------------------------------------------------------------------ */

@@ -58,6 +58,8 @@ void oracle_diag(oracle_t* o, int consistent, double* out);
 /* rhs += boundary terms (fvmf_2d_apply_bc.h / vmf_2d_apply_bc.h), bc is bfaces x Nfp */
 void oracle_apply_bc(oracle_t* o, const double* bc, double* rhs);
 void oracle_mass(oracle_t* o, const double* in, double* out);
+/* Gradient with central flux of a nodal field (cells x Np each component; gz unused in 2D). */
+void oracle_grad_central(oracle_t* o, const double* p, double* gx, double* gy, double* gz);
 /* Mass-weighted squared L2 norm, sum_e 1^T J M (e o e) (l2_error_vortex.cpp:25-44). */
 double oracle_l2sq(oracle_t* o, const double* e);
 
