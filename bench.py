@@ -1,7 +1,8 @@
 """Benchmark of the DG Poisson solve on one MI355X.
 
 Default workload (the 3D plain-FP64 configuration): unit cube, 64^3 hexes split
-into 6 Kuhn tetrahedra each (1,572,864 tets), vertices jittered +-0.1h, order
+into 6 Kuhn tetrahedra each (1,572,864 tets), vertices jittered +-0.1h, cells
+grouped by face pattern (ins_mesh_renumber, as OP2's op_renumber would), order
 N=3 (Np=20, 31,457,280 DoF), Dirichlet, manufactured u = sin(pi x) sin(pi y)
 sin(pi z); flexible CG preconditioned by a p-multigrid V-cycle 3->2->1 with
 Chebyshev(3) smoothing and Jacobi-CG on order 1; outer tolerance 1e-10 (true
@@ -151,7 +152,7 @@ def config0_leg(ip, reps=5):
 def cpu_baseline(n, order, rtol, coarse_iters):
     import oracle as O
     import ins_poisson as ip
-    mesh = ip.Mesh.cube(n, 0.1, 20261203)
+    mesh = cube(ip, n, 20261203)
     coords, c2n = mesh.arrays()
     o = O.Oracle(coords, c2n, order)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -171,8 +172,15 @@ def cpu_baseline(n, order, rtol, coarse_iters):
                       f"rtol {rtol:g}, {it} its, {dt:.1f} s, converged={rc == 0}"}
 
 
+def cube(ip, n, seed):
+    """Jittered Kuhn-tet cube with its cells grouped by shape class (ins_mesh_renumber)."""
+    mesh = ip.Mesh.cube(n, 0.1, seed)
+    mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
+    return mesh
+
+
 def cube_solver(ip, n, order, precision, rtol, coarse_iters):
-    mesh = ip.Mesh.cube(n, 0.1, 20261202)
+    mesh = cube(ip, n, 20261202)
     s = ip.PoissonSolver(mesh, order)
     s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
     s.set_solver(ip.SOLVER_PMULTIGRID)
@@ -261,7 +269,7 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
     rel res 1e-8. Runs the plain FP64 path (Jacobi-CG coarse), FP64 with the AMG coarse solve and
     the FP32 V-cycle with the AMG coarse solve inside the FP64 flexible CG; reports GDoF/s of each
     and the relative L2 (mass norm, mean removed) of each solution against the FP64 AMG one."""
-    mesh = ip.Mesh.cube(n, 0.1, seed)
+    mesh = cube(ip, n, seed)
     s = ip.PoissonSolver(mesh, order)
     s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
     s.set_nullspace(True)
@@ -323,7 +331,7 @@ def viscous_leg(ip, torch, n, order, steps, reynolds=100.0, g0=1.5, seed=2026120
     Kuhn cube, Dirichlet, dt = h/(N+1)^2 (CFL 1 at unit velocity), inverse-mass PCG (factor
     dt/(g0 Re)) to rel res 1e-8 (the reference's viscous-solve r_tol),
     u = sin(pi x) sin(pi y) sin(pi z)."""
-    mesh = ip.Mesh.cube(n, 0.1, seed)
+    mesh = cube(ip, n, seed)
     s = ip.PoissonSolver(mesh, order)
     s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
     dt = (1.0 / n) / (order + 1) ** 2
@@ -419,6 +427,7 @@ def batch_problems(ip, count, seed=20261204):
         N = int(rng.integers(2, 5))
         k, m = (int(v) for v in rng.integers(1, 5, size=2))
         mesh = ip.Mesh.square(n, jitter=0.1, seed=seed + i)
+        mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
         xyz = mesh.node_coords(N)
         f = ((k * k + m * m) * np.pi ** 2 * np.sin(k * np.pi * xyz[0]) * np.sin(m * np.pi * xyz[1]))
         probs.append(dict(mesh=mesh, order=N, rhs=f, rhs_nodal=True, rtol=1e-8, max_iter=500))
@@ -505,6 +514,7 @@ def serve_requests(ip, count, seed=20261205):
         else:
             n = max(2, int(round((cells / 6) ** (1 / 3))))
             mesh = ip.Mesh.cube(n, 0.1, seed + i)
+        mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
         xyz = mesh.node_coords(N)
         k = None if contrast == 1.0 else 1.0 / density(xyz, 1.0 / n, contrast)
         f = rng.standard_normal(xyz[0].shape)
@@ -678,7 +688,8 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f64+f32",
         "data": "synthetic: seeded jittered Kuhn-tet cube, manufactured u=sin(pi x)sin(pi y)sin(pi z)",
         "config": {
-            "workload": f"3D cube {a.n}^3x6 Kuhn tets, N={a.order}, Dirichlet, p-MG "
+            "workload": f"3D cube {a.n}^3x6 Kuhn tets (cells grouped by shape class), N={a.order}, "
+                        f"Dirichlet, p-MG "
                         f"{'->'.join(map(str, orders))} Chebyshev(3) + Jacobi-CG({a.coarse_iters}) "
                         f"coarse, flexible CG to rel res {a.rtol:g}, {a.precision}",
             "cells": cells, "dof": dof, "order": a.order, "rtol": a.rtol,

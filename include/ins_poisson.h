@@ -100,6 +100,16 @@ int ins_mesh_get_faces(ins_mesh_t m, int* face2cells, int* faceNum, int* bface2c
  * dats x, y, z that callers evaluate right-hand sides on); host only. */
 int ins_mesh_node_coords(ins_mesh_t m, int order, double* xyz);
 void ins_mesh_destroy(ins_mesh_t m);
+/* Renumbers the mesh's cells in place for the solver's face gathers, as OP2's op_renumber
+ * reorders sets before a run (callers then lay their dats out in the new order):
+ *   INS_RENUMBER_SHAPE_CLASS  group cells of the same shape class (edge directions from the
+ *                             first vertex, rounded), keeping their relative order; on a
+ *                             structured simplex subdivision a wave's neighbour values are
+ *                             then contiguous.
+ * perm (n_cells, may be NULL) receives new cell i = old cell perm[i]. No counterpart in the
+ * reference's Poisson path. */
+enum { INS_RENUMBER_SHAPE_CLASS = 1 };
+int ins_mesh_renumber(ins_mesh_t m, int method, int64_t* perm);
 
 /* ---- solver handle: PoissonMatrixFreeDiag{2D,3D} + LinearSolver ---- */
 

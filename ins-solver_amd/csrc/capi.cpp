@@ -2,6 +2,7 @@
 // status code and records a message for ins_poisson_last_error().
 #include "../../include/ins_poisson.h"
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <string>
@@ -208,6 +209,17 @@ int ins_mesh_create(const ins_mesh_desc* d, ins_mesh_t* out) {
       throw;
     }
     *out = m;
+    return INS_OK;
+  });
+}
+
+int ins_mesh_renumber(ins_mesh_t mh, int method, int64_t* perm) {
+  return guarded([&] {
+    if (!mh) throw insp::InvalidArgument("null mesh");
+    if (method != INS_RENUMBER_SHAPE_CLASS) throw insp::InvalidArgument("unsupported renumbering");
+    const std::vector<int64_t> p = insp::shape_class_order(mh->m);
+    mh->m = insp::renumber_cells(mh->m, p);
+    if (perm) std::copy(p.begin(), p.end(), perm);
     return INS_OK;
   });
 }

@@ -326,4 +326,49 @@ void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz) {
   }
 }
 
+std::vector<int64_t> shape_class_order(const HostMesh& m) {
+  const int d = m.dim, nv = d + 1;
+  std::vector<int64_t> key((size_t)m.n_cells), perm((size_t)m.n_cells);
+  for (int64_t e = 0; e < m.n_cells; ++e) {
+    // shape class: the edge vectors from the first vertex, each component rounded against
+    // the largest one (-1, 0, 1); congruent cells of a structured subdivision share it
+    const int* c = &m.cell2nodes[(size_t)e * nv];
+    double comp[3][3], mx = 0;
+    for (int a = 1; a < nv; ++a)
+      for (int x = 0; x < d; ++x) {
+        comp[a - 1][x] = m.coords[(size_t)c[a] * d + x] - m.coords[(size_t)c[0] * d + x];
+        mx = std::max(mx, std::fabs(comp[a - 1][x]));
+      }
+    int64_t shape = 0;
+    for (int a = 0; a < d; ++a)
+      for (int x = 0; x < d; ++x) shape = shape * 3 + (int64_t)std::lround(comp[a][x] / mx) + 1;
+    key[(size_t)e] = shape;
+    perm[(size_t)e] = e;
+  }
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; });
+  return perm;
+}
+
+HostMesh renumber_cells(const HostMesh& m, const std::vector<int64_t>& perm) {
+  if ((int64_t)perm.size() != m.n_cells) throw std::invalid_argument("permutation size");
+  const int nv = m.dim + 1;
+  std::vector<char> seen((size_t)m.n_cells, 0);
+  HostMesh r;
+  r.dim = m.dim;
+  r.n_nodes = m.n_nodes;
+  r.n_cells = m.n_cells;
+  r.coords = m.coords;
+  r.cell2nodes.resize(m.cell2nodes.size());
+  for (int64_t i = 0; i < m.n_cells; ++i) {
+    const int64_t o = perm[(size_t)i];
+    if (o < 0 || o >= m.n_cells || seen[(size_t)o]) throw std::invalid_argument("not a permutation");
+    seen[(size_t)o] = 1;
+    for (int k = 0; k < nv; ++k) r.cell2nodes[(size_t)i * nv + k] = m.cell2nodes[(size_t)o * nv + k];
+  }
+  build_topology(r);
+  build_geometry(r);
+  return r;
+}
+
 }  // namespace insp

@@ -43,6 +43,15 @@ HostMesh make_cube_mesh(int n, double jitter, uint64_t seed);
 void build_topology(HostMesh& m);
 void build_geometry(HostMesh& m);
 
+// A cell order that groups cells of the same shape class (edge directions from the first
+// vertex, rounded), stable within a class: new cell i = old cell perm[i]. On a structured
+// simplex subdivision (the Kuhn cube) a class is one cell type in grid order: the cells of
+// a wave then find their neighbours at a common offset through the same local nodes, so
+// the face gathers are contiguous.
+std::vector<int64_t> shape_class_order(const HostMesh& m);
+// The same mesh with cells renumbered, new cell i = old cell perm[i].
+HostMesh renumber_cells(const HostMesh& m, const std::vector<int64_t>& perm);
+
 // d xi_a / d x_x of every cell, cells x d x d.
 void mesh_inverse_jacobians(const HostMesh& m, double* G);
 

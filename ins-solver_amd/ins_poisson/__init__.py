@@ -35,6 +35,7 @@ SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 ELEMENT_PASS_AUTO, ELEMENT_PASS_VECTOR, ELEMENT_PASS_MATRIX = 0, 1, 2
+RENUMBER_SHAPE_CLASS = 1
 COARSE_JACOBI_CG = 0
 COARSE_AMG = 1
 BC_DIRICHLET = 0
@@ -59,7 +60,7 @@ EXPORTED = [
     "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
-    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_poisson_grad_central", "ins_poisson_project_velocity", "ins_ref_element", "ins_ref_interp",
+    "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_mesh_renumber", "ins_poisson_grad_central", "ins_poisson_project_velocity", "ins_ref_element", "ins_ref_interp",
     "ins_batch_config_default", "ins_batch_problem_bytes", "ins_batch_amg_bytes", "ins_batch_create", "ins_batch_destroy",
     "ins_batch_add", "ins_batch_iterate", "ins_batch_take_finished", "ins_batch_result",
     "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
@@ -153,6 +154,7 @@ def _load():
         "ins_poisson_take_timing": [vp, dp, C.POINTER(i64), dp, C.POINTER(i64)],
         "ins_poisson_take_coarse_timing": [vp, dp, C.POINTER(i64)],
         "ins_device_peaks": [dp, dp, dp],
+        "ins_mesh_renumber": [vp, C.c_int, vp],
         "ins_poisson_grad_central": [vp, vp, vp, vp, vp, C.c_int],
         "ins_poisson_project_velocity": [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp, C.c_int],
         "ins_ref_element": [C.c_int, C.c_int, vp, vp, vp, vp, vp],
@@ -284,6 +286,12 @@ class Mesh:
         bnum = np.empty(self.n_bfaces, dtype=np.int32)
         _check(lib.ins_mesh_get_faces(self._h, _ptr(f2c), _ptr(fnum), _ptr(b2c), _ptr(bnum)))
         return f2c, fnum, b2c, bnum
+
+    def renumber(self, method=RENUMBER_SHAPE_CLASS):
+        """Renumbers the cells in place; returns perm (new cell i = old cell perm[i])."""
+        perm = np.zeros(self.n_cells, dtype=np.int64)
+        _check(lib.ins_mesh_renumber(self._h, method, _ptr(perm)))
+        return perm
 
     def node_coords(self, order):
         """Nodal coordinates of order-`order` elements, (dim, cells, Np); host only."""
