@@ -2,7 +2,8 @@
 
 Default workload (the 3D plain-FP64 configuration): unit cube, 64^3 hexes split
 into 6 Kuhn tetrahedra each (1,572,864 tets), vertices jittered +-0.1h, cells
-grouped by face pattern (ins_mesh_renumber, as OP2's op_renumber would), order
+grouped by shape class within runs of 384 cells (ins_mesh_renumber, as OP2's
+op_renumber would), order
 N=3 (Np=20, 31,457,280 DoF), Dirichlet, manufactured u = sin(pi x) sin(pi y)
 sin(pi z); flexible CG preconditioned by a p-multigrid V-cycle 3->2->1 with
 Chebyshev(3) smoothing and Jacobi-CG on order 1; outer tolerance 1e-10 (true
@@ -46,12 +47,13 @@ METRIC = "Poisson GDoF/s & solves/s to 1e-8 rel res, mixed vs FP64; p99 solve la
 # flops 6*d*Np^2 + 2(d+1)*Np*Nf*Nfp + 4d^2*Np + Nf*Nfp*(4d+6), bytes 2*Np*8 + 10*8 + Nf*5*8 + Nf*8.
 FLOPS_PER_CELL = {(3, 3): 15040, (3, 2): 4512, (3, 1): 1032, (3, 4): 41190}
 BYTES_PER_CELL_F64 = {(3, 3): 592, (3, 2): 432, (3, 1): 336, (3, 4): 832}
-# HBM bytes of one trace + element pass on the 64^3x6 cube at N=3, FP64, measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench (--copy-probe), calibrated by
-# a plain 256 MiB device copy: traffic = 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE.
-MEASURED_TRAFFIC = {(64, 3, "fp64"): 4.760e9}
+# HBM bytes of one trace + element pass on the 64^3x6 cube at N=3, FP64 (cells grouped by shape
+# class within runs of 384), measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
+# this bench, calibrated by the 512 MiB device copy of ins_device_peaks:
+# traffic = 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE.
+MEASURED_TRAFFIC = {(64, 3, "fp64"): 3.070e9}
 # same runs, order-1 apply of the coarse Jacobi-CG (k_apply_p1<3, double>): bytes per launch
-MEASURED_TRAFFIC_P1 = {(64, "fp64"): 0.588e9}
+MEASURED_TRAFFIC_P1 = {(64, "fp64"): 0.431e9}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_MATRIX_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
@@ -173,9 +175,9 @@ def cpu_baseline(n, order, rtol, coarse_iters):
 
 
 def cube(ip, n, seed):
-    """Jittered Kuhn-tet cube with its cells grouped by shape class (ins_mesh_renumber)."""
+    """Jittered Kuhn-tet cube, cells grouped by shape class within runs of 64 x 6 cells."""
     mesh = ip.Mesh.cube(n, 0.1, seed)
-    mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
+    mesh.renumber(ip.RENUMBER_SHAPE_CLASS_WINDOWED)
     return mesh
 
 
@@ -427,7 +429,7 @@ def batch_problems(ip, count, seed=20261204):
         N = int(rng.integers(2, 5))
         k, m = (int(v) for v in rng.integers(1, 5, size=2))
         mesh = ip.Mesh.square(n, jitter=0.1, seed=seed + i)
-        mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
+        mesh.renumber(ip.RENUMBER_SHAPE_CLASS_WINDOWED)
         xyz = mesh.node_coords(N)
         f = ((k * k + m * m) * np.pi ** 2 * np.sin(k * np.pi * xyz[0]) * np.sin(m * np.pi * xyz[1]))
         probs.append(dict(mesh=mesh, order=N, rhs=f, rhs_nodal=True, rtol=1e-8, max_iter=500))
@@ -514,7 +516,7 @@ def serve_requests(ip, count, seed=20261205):
         else:
             n = max(2, int(round((cells / 6) ** (1 / 3))))
             mesh = ip.Mesh.cube(n, 0.1, seed + i)
-        mesh.renumber(ip.RENUMBER_SHAPE_CLASS)
+        mesh.renumber(ip.RENUMBER_SHAPE_CLASS_WINDOWED)
         xyz = mesh.node_coords(N)
         k = None if contrast == 1.0 else 1.0 / density(xyz, 1.0 / n, contrast)
         f = rng.standard_normal(xyz[0].shape)
@@ -688,7 +690,7 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f64+f32",
         "data": "synthetic: seeded jittered Kuhn-tet cube, manufactured u=sin(pi x)sin(pi y)sin(pi z)",
         "config": {
-            "workload": f"3D cube {a.n}^3x6 Kuhn tets (cells grouped by shape class), N={a.order}, "
+            "workload": f"3D cube {a.n}^3x6 Kuhn tets (cells grouped by shape class in runs of 384), N={a.order}, "
                         f"Dirichlet, p-MG "
                         f"{'->'.join(map(str, orders))} Chebyshev(3) + Jacobi-CG({a.coarse_iters}) "
                         f"coarse, flexible CG to rel res {a.rtol:g}, {a.precision}",

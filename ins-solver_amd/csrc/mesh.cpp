@@ -326,7 +326,7 @@ void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz) {
   }
 }
 
-std::vector<int64_t> shape_class_order(const HostMesh& m) {
+std::vector<int64_t> shape_class_order(const HostMesh& m, bool windowed) {
   const int d = m.dim, nv = d + 1;
   std::vector<int64_t> key((size_t)m.n_cells), perm((size_t)m.n_cells);
   for (int64_t e = 0; e < m.n_cells; ++e) {
@@ -345,8 +345,19 @@ std::vector<int64_t> shape_class_order(const HostMesh& m) {
     key[(size_t)e] = shape;
     perm[(size_t)e] = e;
   }
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; });
+  const auto by_class = [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; };
+  // windowed: sort within runs of 64 K cells (K classes), so a wave of 64 cells still holds one
+  // class on a structured subdivision while a cell's neighbours stay a few windows away;
+  // with many classes (an unstructured mesh) the whole mesh is one window
+  int64_t window = m.n_cells;
+  if (windowed) {
+    std::vector<int64_t> classes(key);
+    std::sort(classes.begin(), classes.end());
+    const int64_t K = std::unique(classes.begin(), classes.end()) - classes.begin();
+    if (K <= 32) window = 64 * K;
+  }
+  for (int64_t b = 0; b < m.n_cells; b += window)
+    std::stable_sort(perm.begin() + b, perm.begin() + std::min(m.n_cells, b + window), by_class);
   return perm;
 }
 

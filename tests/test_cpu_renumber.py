@@ -22,3 +22,36 @@ def test_shape_class_renumbering(ip, oracle, dim):
     assert np.abs(o1.mult(x[perm]) - o0.mult(x)[perm]).max() < 1e-12 * np.abs(o0.mult(x)).max()
     # a structured subdivision has 2 (2D) or 6 (3D) classes, each kept in its old order
     assert np.sum(np.diff(perm) < 0) == (1 if dim == 2 else 5)
+
+
+def test_windowed_shape_class_renumbering(ip, oracle):
+    """Windowed grouping: each run of 64 K cells (K = 6 classes on the Kuhn cube) is a
+    permutation of the same run, each 64-cell wave holds one class, and the operator of the
+    renumbered mesh is the permuted operator."""
+    n = 8
+    glob = ip.Mesh.cube(n, 0.1, 4)
+    pg = glob.renumber(ip.RENUMBER_SHAPE_CLASS)
+    E = len(pg)
+    cls = np.empty(E, dtype=np.int64)
+    cls[pg] = np.arange(E) // (E // 6)               # class of each old cell, from the global grouping
+    mesh = ip.Mesh.cube(n, 0.1, 4)
+    coords, c0 = mesh.arrays()
+    perm = mesh.renumber(ip.RENUMBER_SHAPE_CLASS_WINDOWED)
+    _, c1 = mesh.arrays()
+    W = 64 * 6
+    assert E % W == 0
+    for b in range(0, E, W):
+        assert np.array_equal(np.sort(perm[b:b + W]), np.arange(b, b + W))
+    waves = cls[perm].reshape(-1, 64)
+    assert (waves == waves[:, :1]).all()
+    assert np.array_equal(c1, c0[perm])
+    o0 = oracle.Oracle(coords, c0, 1)
+    o1 = oracle.Oracle(coords, c1, 1)
+    x = np.random.default_rng(7).standard_normal((E, o0.Np))
+    assert np.abs(o1.mult(x[perm]) - o0.mult(x)[perm]).max() < 1e-12 * np.abs(o0.mult(x)).max()
+
+
+def test_unknown_renumbering_is_rejected(ip):
+    mesh = ip.Mesh.square(3)
+    with pytest.raises(Exception):
+        mesh.renumber(99)
