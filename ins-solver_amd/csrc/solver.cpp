@@ -670,19 +670,20 @@ void PoissonSolver::coarse_cg(int l) {
   Work<T>& w = lv.work<T>();
   const int64_t n = (int64_t)lv.L.Np * stride_;
   RedScratch rs{partial_.p};
+  // Jacobi-CG carried in the preconditioned residual z = D^-1 r alone: per iteration the
+  // apply (p.Ap fused), z -= alpha D^-1 Ap with r.z = sum z^2 D, then x and p in one pass.
   fill<T>(n, w.x.p, (T)0, stream_);
-  copy<T>(n, w.b.p, w.r.p, stream_);
-  jacobi<T>(n, w.r.p, w.inv.p, w.z.p, stream_);
+  jacobi<T>(n, w.b.p, w.inv.p, w.z.p, stream_);
   copy<T>(n, w.z.p, w.p.p, stream_);
-  dot<T>(n, w.r.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
+  dot<T>(n, w.b.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
   int a = S_CG_A, b = S_CG_B;
   for (int it = 0; it < pmg_.coarse_iters; ++it) {
     apply_level<T>(l, w.p.p, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
     ++smoother_applies_;
     finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
-    cg_update_xr<T>(n, w.x.p, w.r.p, w.p.p, w.Ap.p, w.inv.p, w.z.p, scal_.p, a, S_CG_PAP, b, rs,
+    cg_update_z<T>(n, w.z.p, w.Ap.p, w.inv.p, scal_.p, a, S_CG_PAP, b, rs, stream_);
+    cg_update_xp<T>(n, w.x.p, w.p.p, w.z.p, scal_.p, a, S_CG_PAP, b, it + 1 == pmg_.coarse_iters,
                     stream_);
-    cg_update_p<T>(n, w.p.p, w.z.p, scal_.p, b, a, stream_);
     std::swap(a, b);
   }
 }

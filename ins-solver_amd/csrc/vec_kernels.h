@@ -41,6 +41,16 @@ void cg_update_xr(int64_t n, T* x, T* r, const T* p, const T* Ap, const T* invdi
 // beta = scal[num]/scal[den]; p = z + beta p
 template <typename T>
 void cg_update_p(int64_t n, T* p, const T* z, const double* scal, int num, int den, hipStream_t s);
+// Jacobi-CG in z = invdiag * r: alpha = scal[num]/scal[den]; z -= alpha invdiag Ap;
+// scal[out] = r.z = sum z^2 / invdiag (entries with invdiag = 0 add nothing)
+template <typename T>
+void cg_update_z(int64_t n, T* z, const T* Ap, const T* invdiag, double* scal, int num, int den,
+                 int out, RedScratch rs, hipStream_t s);
+// alpha = scal[anum]/scal[aden], beta = scal[bnum]/scal[anum]; x += alpha p; p = z + beta p
+// unless last
+template <typename T>
+void cg_update_xp(int64_t n, T* x, T* p, const T* z, const double* scal, int anum, int aden,
+                  int bnum, bool last, hipStream_t s);
 // z = invdiag * r
 template <typename T>
 void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s);

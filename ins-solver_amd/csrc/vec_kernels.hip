@@ -48,14 +48,24 @@ __device__ __forceinline__ void write_partials(double (&v)[K], double* partial) 
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
+// One block adds the partials of each value: 1024 threads, four independent loads in flight per
+// thread (thread t owns entries t + 1024 (4j + u)), combined in a fixed order.
+constexpr int kFinBlock = 1024;
+__global__ void __launch_bounds__(kFinBlock)
 k_finalize(int nb, int nvals, const double* __restrict__ partial, double* __restrict__ scal,
            int slot) {
   for (int k = 0; k < nvals; ++k) {
-    double v[1] = {0.0};
-    for (int i = threadIdx.x; i < nb; i += kBlock) v[0] += partial[k * kRedBlocks + i];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) scal[slot + k] = v[0];
+    const double* src = partial + (size_t)k * kRedBlocks;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nb; i += 4 * kFinBlock) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = i + u * kFinBlock;
+        if (j < nb) a[u] += src[j];
+      }
+    }
+    const double t = block_sum1<kFinBlock>((a[0] + a[1]) + (a[2] + a[3]));
+    if (threadIdx.x == 0) scal[slot + k] = t;
   }
 }
 
@@ -125,6 +135,39 @@ k_cg_update_p(int64_t n, T* __restrict__ p, const T* __restrict__ z, const doubl
   const T beta = (T)safe_div(scal[num], scal[den]);
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     p[i] = fma(beta, p[i], z[i]);
+}
+
+// Jacobi-CG carried in z = D^-1 r (r is never stored): z -= alpha D^-1 Ap, and r.z = sum z^2 / D^-1
+// over the cells (padding entries hold D^-1 = 0 and z = 0).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_z(int64_t n, T* __restrict__ z, const T* __restrict__ Ap, const T* __restrict__ invdiag,
+              const double* __restrict__ scal, int num, int den, double* __restrict__ partial) {
+  const T a = (T)safe_div(scal[num], scal[den]);
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T di = invdiag[i];
+    const T zi = fma(-a * di, Ap[i], z[i]);
+    z[i] = zi;
+    const double zd = (double)zi;
+    v[0] += di != (T)0 ? zd * zd / (double)di : 0.0;
+  }
+  write_partials<1>(v, partial);
+}
+
+// alpha = scal[anum]/scal[aden], beta = scal[bnum]/scal[anum]: x += alpha p, then p = z + beta p
+// (p is left as it is when last is set).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_xp(int64_t n, T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
+               const double* __restrict__ scal, int anum, int aden, int bnum, bool last) {
+  const T a = (T)safe_div(scal[anum], scal[aden]);
+  const T beta = (T)safe_div(scal[bnum], scal[anum]);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T pi = p[i];
+    x[i] = fma(a, pi, x[i]);
+    if (!last) p[i] = fma(beta, pi, z[i]);
+  }
 }
 
 template <typename T>
@@ -254,7 +297,7 @@ int red_blocks(int64_t n) {
 }
 
 void finalize(int nb, int nvals, const double* partial, double* scal, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nvals, partial, scal, slot);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, nvals, partial, scal, slot);
   launch_check("k_finalize");
 }
 
@@ -263,7 +306,7 @@ void dot(int64_t n, const T* a, const T* b, const T* c, const T* d, int nd, doub
          RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
   hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(kBlock), 0, s, n, a, b, c, d, nd, rs.partial); launch_check("k_dot");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, nd, rs.partial, scal, slot); launch_check("k_finalize");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, nd, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 template <typename T>
@@ -271,14 +314,14 @@ void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, in
               RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
   hipLaunchKernelGGL((k_dot_flex<T>), dim3(nb), dim3(kBlock), 0, s, n, r, z, zold, rs.partial); launch_check("k_dot_flex");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
               RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial); launch_check("k_residual");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 template <typename T>
@@ -287,13 +330,29 @@ void cg_update_xr(int64_t n, T* x, T* r, const T* p, const T* Ap, const T* invdi
   const int nb = red_blocks(n);
   hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(nb), dim3(kBlock), 0, s, n, x, r, p, Ap, invdiag, z,
                      scal, num, den, rs.partial); launch_check("k_cg_update_xr");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
 }
 
 template <typename T>
 void cg_update_p(int64_t n, T* p, const T* z, const double* scal, int num, int den, hipStream_t s) {
   hipLaunchKernelGGL((k_cg_update_p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p, z, scal,
                      num, den); launch_check("k_cg_update_p");
+}
+
+template <typename T>
+void cg_update_z(int64_t n, T* z, const T* Ap, const T* invdiag, double* scal, int num, int den,
+                 int out, RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_cg_update_z<T>), dim3(nb), dim3(kBlock), 0, s, n, z, Ap, invdiag, scal, num,
+                     den, rs.partial); launch_check("k_cg_update_z");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
+}
+
+template <typename T>
+void cg_update_xp(int64_t n, T* x, T* p, const T* z, const double* scal, int anum, int aden,
+                  int bnum, bool last, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update_xp<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, p, z, scal,
+                     anum, aden, bnum, last); launch_check("k_cg_update_xp");
 }
 
 template <typename T>
@@ -344,7 +403,7 @@ template <typename T>
 void vsum(int64_t n, const T* x, double* scal, int slot, RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
   hipLaunchKernelGGL((k_vsum<T>), dim3(nb), dim3(kBlock), 0, s, n, x, rs.partial); launch_check("k_vsum");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
 void soa_from_aos(int rows, int64_t E, int64_t stride, const double* aos, double* soa,
@@ -384,6 +443,10 @@ void scal_copy(double* scal, int dst, int src, hipStream_t s) {
   template void cg_update_xr<T>(int64_t, T*, T*, const T*, const T*, const T*, T*, double*, int,  \
                                 int, int, RedScratch, hipStream_t);                               \
   template void cg_update_p<T>(int64_t, T*, const T*, const double*, int, int, hipStream_t);      \
+  template void cg_update_z<T>(int64_t, T*, const T*, const T*, double*, int, int, int,           \
+                               RedScratch, hipStream_t);                                          \
+  template void cg_update_xp<T>(int64_t, T*, T*, const T*, const double*, int, int, int, bool,    \
+                                hipStream_t);                                                     \
   template void jacobi<T>(int64_t, const T*, const T*, T*, hipStream_t);                          \
   template void cheb_step<T>(int64_t, const T*, const T*, const T*, T*, T*, T, T, bool,           \
                              hipStream_t);                                                        \
