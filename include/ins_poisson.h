@@ -246,7 +246,8 @@ int ins_poisson_project_velocity(ins_poisson_t h, double dt, const double* p, co
                                  double* uTTz, int layout);
 
 /* Achievable peaks of the current device, measured in about a second: streaming copy
- * bandwidth (GB/s, bytes read + written) and 16x16x4 matrix-core throughput in FP32 and
+ * bandwidth (GB/s, bytes read + written; the faster of a plain and a non-temporal copy of
+ * 512 MiB, one 16-byte element per thread) and 16x16x4 matrix-core throughput in FP32 and
  * FP64 (TFLOP/s). No counterpart in the reference: the denominators of the measured
  * rooflines beside the published figures. Any output may be NULL. */
 int ins_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);

