@@ -219,22 +219,19 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
 
 // Order 1 in one pass: the gradient of a P1 cell is constant, so the neighbour's
 // flux trace k n.grad(u) is recomputed from its Np nodal values and metric instead
-// of being stored by a trace pass. Same operator as apply_cell.
-__device__ __forceinline__ double face_alpha_rt(int dim, int f, int r) {
-  if (dim == 2) {
-    switch (f) {
-      case 0: return r == 1 ? -1.0 : 0.0;
-      case 1: return 1.0;
-      default: return r == 0 ? -1.0 : 0.0;
+// of being stored by a trace pass. Same operator as apply_cell. The face coefficients
+// alpha_{f,r} (face_alpha) are 1 for every r on the slanted face f = DIM-1 and -1 for one r
+// on the others: r = DIM-1-f before the slanted face, DIM-f after it.
+constexpr bool face_alpha_is_rows(int dim) {
+  for (int f = 0; f <= dim; ++f)
+    for (int r = 0; r < dim; ++r) {
+      const int row = f < dim - 1 ? dim - 1 - f : dim - f;
+      const double want = f == dim - 1 ? 1.0 : (r == row ? -1.0 : 0.0);
+      if (face_alpha(dim, f, r) != want) return false;
     }
-  }
-  switch (f) {
-    case 0: return r == 2 ? -1.0 : 0.0;
-    case 1: return r == 1 ? -1.0 : 0.0;
-    case 2: return 1.0;
-    default: return r == 0 ? -1.0 : 0.0;
-  }
+  return true;
 }
+static_assert(face_alpha_is_rows(2) && face_alpha_is_rows(3), "face coefficient rows");
 
 template <int DIM, typename T, int VAR>
 __device__ __forceinline__ double
@@ -310,13 +307,22 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
 #pragma unroll
           for (int s = 0; s < DIM; ++s) gn[s] = fma(C[L.off_D + (s * Np) * Np + m], um, gn[s]);
         }
+        // the neighbour's sum_r alpha_{fn,r} K[r][s]: the sum of the rows of K on its slanted
+        // face (fn = DIM-1), minus one row of K on the faces through its vertex 0, so only the
+        // entries that face needs are gathered
+        if (fn == DIM - 1) {
 #pragma unroll
-        for (int s = 0; s < DIM; ++s) {
-          T b = 0;
+          for (int s = 0; s < DIM; ++s) {
+            T b = 0;
 #pragma unroll
-          for (int r = 0; r < DIM; ++r)
-            b += (T)face_alpha_rt(DIM, fn, r) * (T)Kt[kidx(DIM, r, s) * stride + nb];
-          tn0 = fma(b, gn[s], tn0);
+            for (int r = 0; r < DIM; ++r) b += (T)Kt[kidx(DIM, r, s) * stride + nb];
+            tn0 = fma(b, gn[s], tn0);
+          }
+        } else {
+          const int rr = fn < DIM - 1 ? DIM - 1 - fn : DIM - fn;
+#pragma unroll
+          for (int s = 0; s < DIM; ++s)
+            tn0 = fma(-(T)Kt[(int64_t)kidx(DIM, rr, s) * stride + nb], gn[s], tn0);
         }
       } else {
         mode = bctype[-nb - 1] == 0 ? 1 : 2;
