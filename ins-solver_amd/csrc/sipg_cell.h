@@ -233,7 +233,9 @@ constexpr bool face_alpha_is_rows(int dim) {
 }
 static_assert(face_alpha_is_rows(2) && face_alpha_is_rows(3), "face coefficient rows");
 
-template <int DIM, typename T, int VAR>
+// PACK: nbr holds one word per face, (neighbour << 5) | code inside the mesh and the
+// boundary index -(bface+1) on the boundary, and code is not read.
+template <int DIM, typename T, int VAR, bool PACK = false>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -290,12 +292,19 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
         beta[s] = b;
         tm0 = fma(b, g[s], tm0);
       }
-      const int nb = nbr[f * stride + e];
+      int nb, c = 0;
+      if constexpr (PACK) {
+        const int w = nbr[f * stride + e];
+        nb = w >= 0 ? w >> 5 : w;
+        c = w & 31;
+      } else {
+        nb = nbr[f * stride + e];
+      }
       int mode = 0;
       int tbase = 0;
       T tn0 = 0;
       if (nb >= 0) {
-        const int c = code[f * stride + e];
+        if constexpr (!PACK) c = code[f * stride + e];
         const int fn = c / L.nperm;
         tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
         T gn[DIM];

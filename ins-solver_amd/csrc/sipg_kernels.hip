@@ -70,7 +70,7 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
 }
 
 
-template <int DIM, typename T, bool VAR>
+template <int DIM, typename T, bool VAR, bool PACK>
 __global__ void __launch_bounds__(256)
 k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
            const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -80,7 +80,7 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_p1_cell<DIM, T, VAR>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
+    dotv = apply_p1_cell<DIM, T, VAR, PACK>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
                                       as_const(tab), out);
   if (partial) {
     const double t = block_sum1<256>(dotv);
@@ -353,21 +353,20 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
   }
   const T* C = static_cast<const T*>(l.C);
   if (l.N == 1) {
+    // packed neighbour words when the mesh has them (one 4-byte load per face instead of two)
+    const bool pack = m.nbc != nullptr;
+    const int* nw = pack ? m.nbc : m.nbr;
+#define INSP_P1(D, V, P)                                                                          \
+  hipLaunchKernelGGL((k_apply_p1<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, u,  \
+                     m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial)
     if (m.dim == 2) {
-      if (kfac)
-        hipLaunchKernelGGL((k_apply_p1<2, T, true>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
-                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
-      else
-        hipLaunchKernelGGL((k_apply_p1<2, T, false>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
-                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+      if (kfac) { if (pack) INSP_P1(2, true, true); else INSP_P1(2, true, false); }
+      else { if (pack) INSP_P1(2, false, true); else INSP_P1(2, false, false); }
     } else {
-      if (kfac)
-        hipLaunchKernelGGL((k_apply_p1<3, T, true>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
-                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
-      else
-        hipLaunchKernelGGL((k_apply_p1<3, T, false>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride,
-                           u, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C, l.tab, out, partial);
+      if (kfac) { if (pack) INSP_P1(3, true, true); else INSP_P1(3, true, false); }
+      else { if (pack) INSP_P1(3, false, true); else INSP_P1(3, false, false); }
     }
+#undef INSP_P1
     launch_check("k_apply_p1");
     return;
   }

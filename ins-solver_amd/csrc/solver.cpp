@@ -120,6 +120,11 @@ PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
   J_.upload(jj.data(), jj.size());
   nbr_.upload(nb.data(), nb.size());
   code_.upload(cd.data(), cd.size());
+  if (E < (int64_t(1) << 26) && Nf * nperm <= 32) {
+    for (size_t i = 0; i < nb.size(); ++i)
+      if (nb[i] >= 0) nb[i] = (nb[i] << 5) | cd[i];
+    nbc_.upload(nb.data(), nb.size());
+  }
   std::vector<int> bt((size_t)std::max<int64_t>(mesh_.n_bfaces, 1), 0);
   bctype_.upload(bt.data(), bt.size());
   scal_.alloc(S_COUNT);
@@ -149,6 +154,7 @@ MeshView PoissonSolver::mesh_view() const {
   m.nbr = nbr_.p;
   m.code = code_.p;
   m.bctype = bctype_.p;
+  m.nbc = nbc_.empty() ? nullptr : nbc_.p;
   return m;
 }
 
