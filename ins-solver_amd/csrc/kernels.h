@@ -19,7 +19,7 @@ struct MeshView {
   const int* code;     // [Nf][stride] neighbour face * nperm + vertex permutation
   const int* bctype;   // [n_bfaces] 0 Dirichlet, 1 Neumann
   // [Nf][stride] (neighbour << 5) | code inside the mesh, -(bface+1) on the boundary; null when
-  // the mesh has 2^26 cells or more (the order-1 element pass then reads nbr and code)
+  // the mesh has 2^26 cells or more (the element passes then read nbr and code)
   const int* nbc = nullptr;
 };
 

@@ -87,8 +87,9 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
   }
 }
 
-// One cell of the element pass; returns u.(A u) over the cell.
-template <int DIM, int N, typename T, int VAR>
+// One cell of the element pass; returns u.(A u) over the cell. PACK: nbr holds the packed
+// words (neighbour << 5) | code, -(bface+1) on the boundary, and code is not read.
+template <int DIM, int N, typename T, int VAR, bool PACK = false>
 __device__ __forceinline__ double
 apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
            const double* __restrict__ Kt, const double* __restrict__ tauS,
@@ -142,11 +143,19 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
       for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
       beta[s] = b;
     }
-    const int nb = nbr[f * stride + e];
+    int nb, c = 0;
+    if constexpr (PACK) {
+      const int w = nbr[f * stride + e];
+      nb = w >= 0 ? w >> 5 : w;
+      c = w & 31;
+    } else {
+      nb = nbr[f * stride + e];
+    }
     int mode = 0;  // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms)
     int tbase = 0;
     if (nb >= 0) {
-      tbase = L.tab_nbr + (f * Nf * L.nperm + code[f * stride + e]) * Nfp;
+      if constexpr (!PACK) c = code[f * stride + e];
+      tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
     } else {
       mode = bctype[-nb - 1] == 0 ? 1 : 2;
     }

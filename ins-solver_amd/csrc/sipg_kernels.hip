@@ -50,7 +50,7 @@ k_trace(int64_t E, int64_t stride, const T* __restrict__ u, const double* __rest
   if (e < E) trace_cell<DIM, N, T, VAR>(e, stride, u, Kt, kfac, as_const(C), tr);
 }
 
-template <int DIM, int N, typename T, bool VAR>
+template <int DIM, int N, typename T, bool VAR, bool PACK>
 __global__ void __launch_bounds__(256)
 k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
         const double* __restrict__ Kt, const double* __restrict__ tauS,
@@ -60,7 +60,7 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_cell<DIM, N, T, VAR>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
+    dotv = apply_cell<DIM, N, T, VAR, PACK>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
                                       as_const(tab), out);
   // optional fused u.(A u) partial sum for CG step lengths
   if (partial) {
@@ -370,16 +370,16 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
     launch_check("k_apply_p1");
     return;
   }
-  if (kfac)
-    INSP_DISPATCH_N(m.dim, l.N,
-                    hipLaunchKernelGGL((k_apply<D_, N_, T, true>), grid_for(m.E), dim3(256), 0, s,
-                                       m.E, m.stride, u, tr, m.Kt, l.tauS, m.nbr, m.code,
-                                       m.bctype, kfac, C, l.tab, out, partial));
-  else
-    INSP_DISPATCH_N(m.dim, l.N,
-                    hipLaunchKernelGGL((k_apply<D_, N_, T, false>), grid_for(m.E), dim3(256), 0, s,
-                                       m.E, m.stride, u, tr, m.Kt, l.tauS, m.nbr, m.code,
-                                       m.bctype, kfac, C, l.tab, out, partial));
+  const bool pack = m.nbc != nullptr;
+  const int* nw = pack ? m.nbc : m.nbr;
+#define INSP_APPLY(V, P)                                                                          \
+  INSP_DISPATCH_N(m.dim, l.N,                                                                     \
+                  hipLaunchKernelGGL((k_apply<D_, N_, T, V, P>), grid_for(m.E), dim3(256), 0, s,  \
+                                     m.E, m.stride, u, tr, m.Kt, l.tauS, nw, m.code, m.bctype,    \
+                                     kfac, C, l.tab, out, partial))
+  if (kfac) { if (pack) INSP_APPLY(true, true); else INSP_APPLY(true, false); }
+  else { if (pack) INSP_APPLY(false, true); else INSP_APPLY(false, false); }
+#undef INSP_APPLY
 }
 
 int apply_blocks(int64_t E) { return (int)((E + 255) / 256); }

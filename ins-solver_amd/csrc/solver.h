@@ -244,7 +244,7 @@ class PoissonSolver {
   // device mesh
   DBuf<double> Kt_, J_;
   DBuf<int> nbr_, code_, bctype_;
-  DBuf<int> nbc_;  // packed neighbour words of the order-1 element pass (empty: not packed)
+  DBuf<int> nbc_;  // packed neighbour words of the element passes (empty: not packed)
   DBuf<double> bc_;
   bool have_bc_ = false;
   bool var_factor_ = false;
