@@ -500,7 +500,6 @@ oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coord
   o->c2n = malloc(sizeof(int) * n_cells * nv);
   memcpy(o->c2n, cell2nodes, sizeof(int) * n_cells * nv);
   if (build_refel(dim, order, &o->R)) { oracle_destroy(o); return NULL; }
-  const int Nfp = o->R.Nfp;
   const int64_t E = n_cells;
   /* orientation: positive measure (swap the last two vertices otherwise) */
   for (int64_t e = 0; e < E; ++e) {
