@@ -109,9 +109,12 @@ void ins_mesh_destroy(ins_mesh_t m);
  *   INS_RENUMBER_SHAPE_CLASS_WINDOWED  the same grouping within runs of 64 K cells (K shape
  *                             classes, at most 32; more classes: one run), so a cell's
  *                             neighbours stay close to it in memory.
+ *   INS_RENUMBER_SHAPE_CLASS_TILED  the same grouping within space tiles (cubes or squares
+ *                             holding about 64 cells of each class, in lexicographic order), so
+ *                             neighbours in every direction mostly share a cell's tile.
  * perm (n_cells, may be NULL) receives new cell i = old cell perm[i]. No counterpart in the
  * reference's Poisson path. */
-enum { INS_RENUMBER_SHAPE_CLASS = 1, INS_RENUMBER_SHAPE_CLASS_WINDOWED = 2 };
+enum { INS_RENUMBER_SHAPE_CLASS = 1, INS_RENUMBER_SHAPE_CLASS_WINDOWED = 2, INS_RENUMBER_SHAPE_CLASS_TILED = 3 };
 int ins_mesh_renumber(ins_mesh_t m, int method, int64_t* perm);
 
 /* ---- solver handle: PoissonMatrixFreeDiag{2D,3D} + LinearSolver ---- */

@@ -216,10 +216,11 @@ int ins_mesh_create(const ins_mesh_desc* d, ins_mesh_t* out) {
 int ins_mesh_renumber(ins_mesh_t mh, int method, int64_t* perm) {
   return guarded([&] {
     if (!mh) throw insp::InvalidArgument("null mesh");
-    if (method != INS_RENUMBER_SHAPE_CLASS && method != INS_RENUMBER_SHAPE_CLASS_WINDOWED)
+    if (method != INS_RENUMBER_SHAPE_CLASS && method != INS_RENUMBER_SHAPE_CLASS_WINDOWED &&
+        method != INS_RENUMBER_SHAPE_CLASS_TILED)
       throw insp::InvalidArgument("unsupported renumbering");
-    const std::vector<int64_t> p =
-        insp::shape_class_order(mh->m, method == INS_RENUMBER_SHAPE_CLASS_WINDOWED);
+    const std::vector<int64_t> p = insp::shape_class_order(
+        mh->m, method == INS_RENUMBER_SHAPE_CLASS ? 0 : (method == INS_RENUMBER_SHAPE_CLASS_WINDOWED ? 1 : 2));
     mh->m = insp::renumber_cells(mh->m, p);
     if (perm) std::copy(p.begin(), p.end(), perm);
     return INS_OK;

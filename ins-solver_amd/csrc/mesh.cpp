@@ -326,14 +326,16 @@ void mesh_node_coords(const HostMesh& m, const RefElement& re, double* xyz) {
   }
 }
 
-std::vector<int64_t> shape_class_order(const HostMesh& m, bool windowed) {
+std::vector<int64_t> shape_class_order(const HostMesh& m, int mode) {
   const int d = m.dim, nv = d + 1;
-  std::vector<int64_t> key((size_t)m.n_cells), perm((size_t)m.n_cells);
-  for (int64_t e = 0; e < m.n_cells; ++e) {
+  const int64_t E = m.n_cells;
+  std::vector<int64_t> key((size_t)E), perm((size_t)E);
+  double vol = 0;  // total measure
+  for (int64_t e = 0; e < E; ++e) {
     // shape class: the edge vectors from the first vertex, each component rounded against
     // the largest one (-1, 0, 1); congruent cells of a structured subdivision share it
     const int* c = &m.cell2nodes[(size_t)e * nv];
-    double comp[3][3], mx = 0;
+    double comp[3][3] = {}, mx = 0;
     for (int a = 1; a < nv; ++a)
       for (int x = 0; x < d; ++x) {
         comp[a - 1][x] = m.coords[(size_t)c[a] * d + x] - m.coords[(size_t)c[0] * d + x];
@@ -344,20 +346,54 @@ std::vector<int64_t> shape_class_order(const HostMesh& m, bool windowed) {
       for (int x = 0; x < d; ++x) shape = shape * 3 + (int64_t)std::lround(comp[a][x] / mx) + 1;
     key[(size_t)e] = shape;
     perm[(size_t)e] = e;
+    vol += d == 2 ? std::fabs(comp[0][0] * comp[1][1] - comp[0][1] * comp[1][0]) / 2
+                  : std::fabs(comp[0][0] * (comp[1][1] * comp[2][2] - comp[1][2] * comp[2][1]) -
+                              comp[0][1] * (comp[1][0] * comp[2][2] - comp[1][2] * comp[2][0]) +
+                              comp[0][2] * (comp[1][0] * comp[2][1] - comp[1][1] * comp[2][0])) / 6;
   }
+  std::vector<int64_t> classes(key);
+  std::sort(classes.begin(), classes.end());
+  const int64_t K = std::unique(classes.begin(), classes.end()) - classes.begin();
   const auto by_class = [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; };
+  if (mode == 2 && K <= 32 && E > 0) {
+    // tiled: cubes (squares) of side s holding about 64 cells of each class, s^d = 64 K |cell|;
+    // tiles in lexicographic order, inside a tile by class and old order. On the Kuhn cube a
+    // tile is 4^3 hexes, a wave one class of one tile, and most neighbours share the tile.
+    const double side = std::pow(64.0 * (double)K * vol / (double)E, 1.0 / d);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int x = 0; x < d; ++x) lo[x] = hi[x] = m.coords[(size_t)x];
+    for (int64_t v = 0; v < m.n_nodes; ++v)
+      for (int x = 0; x < d; ++x) {
+        lo[x] = std::min(lo[x], m.coords[(size_t)v * d + x]);
+        hi[x] = std::max(hi[x], m.coords[(size_t)v * d + x]);
+      }
+    int64_t nt[3] = {1, 1, 1};
+    for (int x = 0; x < d; ++x) nt[x] = (int64_t)std::floor((hi[x] - lo[x]) / side) + 1;
+    std::vector<int64_t> tile((size_t)E);
+    for (int64_t e = 0; e < E; ++e) {
+      const int* c = &m.cell2nodes[(size_t)e * nv];
+      int64_t t = 0;
+      for (int x = d - 1; x >= 0; --x) {
+        double cx = 0;
+        for (int a = 0; a < nv; ++a) cx += m.coords[(size_t)c[a] * d + x];
+        const int64_t i = std::min<int64_t>(nt[x] - 1, std::max<int64_t>(0, (int64_t)std::floor((cx / nv - lo[x]) / side)));
+        t = t * nt[x] + i;
+      }
+      tile[(size_t)e] = t;
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+      return tile[(size_t)a] != tile[(size_t)b] ? tile[(size_t)a] < tile[(size_t)b]
+                                                : key[(size_t)a] < key[(size_t)b];
+    });
+    return perm;
+  }
   // windowed: sort within runs of 64 K cells (K classes), so a wave of 64 cells still holds one
   // class on a structured subdivision while a cell's neighbours stay a few windows away;
   // with many classes (an unstructured mesh) the whole mesh is one window
-  int64_t window = m.n_cells;
-  if (windowed) {
-    std::vector<int64_t> classes(key);
-    std::sort(classes.begin(), classes.end());
-    const int64_t K = std::unique(classes.begin(), classes.end()) - classes.begin();
-    if (K <= 32) window = 64 * K;
-  }
-  for (int64_t b = 0; b < m.n_cells; b += window)
-    std::stable_sort(perm.begin() + b, perm.begin() + std::min(m.n_cells, b + window), by_class);
+  int64_t window = E;
+  if (mode == 1 && K <= 32) window = 64 * K;
+  for (int64_t b = 0; b < E; b += window)
+    std::stable_sort(perm.begin() + b, perm.begin() + std::min(E, b + window), by_class);
   return perm;
 }
 

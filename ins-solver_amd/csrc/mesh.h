@@ -47,10 +47,11 @@ void build_geometry(HostMesh& m);
 // vertex, rounded), stable within a class: new cell i = old cell perm[i]. On a structured
 // simplex subdivision (the Kuhn cube) a class is one cell type in grid order: the cells of
 // a wave then find their neighbours at a common offset through the same local nodes, so
-// the face gathers are contiguous. windowed: the grouping is done within runs of 64 K cells
-// (K classes, at most 32), which keeps each wave to one class and a cell's neighbours within
-// a few runs of it.
-std::vector<int64_t> shape_class_order(const HostMesh& m, bool windowed);
+// the face gathers are contiguous. mode 0: over the whole mesh; 1 (windowed): within runs of
+// 64 K cells (K classes, at most 32), which keeps each wave to one class and a cell's
+// neighbours within a few runs of it; 2 (tiled): within space tiles holding about 64 cells of
+// each class, tiles in lexicographic order, so most neighbours share a cell's tile.
+std::vector<int64_t> shape_class_order(const HostMesh& m, int mode);
 // The same mesh with cells renumbered, new cell i = old cell perm[i].
 HostMesh renumber_cells(const HostMesh& m, const std::vector<int64_t>& perm);
 

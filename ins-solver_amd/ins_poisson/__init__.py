@@ -35,7 +35,7 @@ SOLVER_PMULTIGRID = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 ELEMENT_PASS_AUTO, ELEMENT_PASS_VECTOR, ELEMENT_PASS_MATRIX = 0, 1, 2
-RENUMBER_SHAPE_CLASS, RENUMBER_SHAPE_CLASS_WINDOWED = 1, 2
+RENUMBER_SHAPE_CLASS, RENUMBER_SHAPE_CLASS_WINDOWED, RENUMBER_SHAPE_CLASS_TILED = 1, 2, 3
 COARSE_JACOBI_CG = 0
 COARSE_AMG = 1
 BC_DIRICHLET = 0

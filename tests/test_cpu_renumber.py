@@ -55,3 +55,42 @@ def test_unknown_renumbering_is_rejected(ip):
     mesh = ip.Mesh.square(3)
     with pytest.raises(Exception):
         mesh.renumber(99)
+
+
+def test_tiled_shape_class_renumbering(ip, oracle):
+    """Tiled grouping on the Kuhn cube (8^3 hexes): tiles of 4^3 hexes hold 64 cells of each of
+    the 6 classes, so each run of 384 cells lies in one 4h cube, each 64-cell wave holds one
+    class, and the operator of the renumbered mesh is the permuted operator."""
+    n = 8
+    glob = ip.Mesh.cube(n, 0.1, 4)
+    pg = glob.renumber(ip.RENUMBER_SHAPE_CLASS)
+    E = len(pg)
+    cls = np.empty(E, dtype=np.int64)
+    cls[pg] = np.arange(E) // (E // 6)
+    mesh = ip.Mesh.cube(n, 0.1, 4)
+    coords, c0 = mesh.arrays()
+    perm = mesh.renumber(ip.RENUMBER_SHAPE_CLASS_TILED)
+    _, c1 = mesh.arrays()
+    assert np.array_equal(np.sort(perm), np.arange(E))
+    assert np.array_equal(c1, c0[perm])
+    waves = cls[perm].reshape(-1, 64)
+    assert (waves == waves[:, :1]).all()
+    cen = coords[c1].mean(axis=1)                      # (E, 3) centroids in the new order
+    tiles = np.floor(cen / (4.0 / n)).astype(int).reshape(-1, 384, 3)
+    assert (tiles == tiles[:, :1, :]).all()
+    assert len({tuple(t) for t in tiles[:, 0, :]}) == E // 384
+    o0 = oracle.Oracle(coords, c0, 1)
+    o1 = oracle.Oracle(coords, c1, 1)
+    x = np.random.default_rng(9).standard_normal((E, o0.Np))
+    assert np.abs(o1.mult(x[perm]) - o0.mult(x)[perm]).max() < 1e-12 * np.abs(o0.mult(x)).max()
+
+
+def test_tiled_renumbering_2d_and_ragged(ip):
+    """2D squares (2 classes, tiles of 8x8 squares) and a cube whose side is no multiple of the
+    tile: still a permutation of the same cells."""
+    for mesh in (ip.Mesh.square(12, jitter=0.1, seed=2), ip.Mesh.cube(5, 0.1, 3)):
+        _, c0 = mesh.arrays()
+        perm = mesh.renumber(ip.RENUMBER_SHAPE_CLASS_TILED)
+        _, c1 = mesh.arrays()
+        assert np.array_equal(np.sort(perm), np.arange(len(perm)))
+        assert np.array_equal(c1, c0[perm])
