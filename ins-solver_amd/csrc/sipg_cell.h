@@ -87,15 +87,43 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
   }
 }
 
+// Where the element pass leaves (A u) at flat index k: stored as it is, or consumed by a
+// functor out(k, (A u)_k, u_k).
+template <typename T>
+__device__ __forceinline__ void store_out(T* __restrict__ o, int64_t k, T acc, T) {
+  o[k] = acc;
+}
+template <typename T, class O>
+__device__ __forceinline__ void store_out(const O& o, int64_t k, T acc, T u) {
+  o(k, acc, u);
+}
+
+// One Chebyshev step taken in place of the store: res = D^-1 (f - A x), d = c1 d + c2 res (d is
+// not read when c1 = 0), x_next = x + d into another buffer (the pass still reads x).
+template <typename T>
+struct ChebOut {
+  const T* f;
+  const T* inv;
+  T* d;
+  T* x_next;
+  T c1, c2;
+  __device__ __forceinline__ void operator()(int64_t k, T acc, T u) const {
+    const T res = inv[k] * (f[k] - acc);
+    const T di = c1 == (T)0 ? c2 * res : fma(c1, d[k], c2 * res);
+    d[k] = di;
+    x_next[k] = u + di;
+  }
+};
+
 // One cell of the element pass; returns u.(A u) over the cell. PACK: nbr holds the packed
 // words (neighbour << 5) | code, -(bface+1) on the boundary, and code is not read.
-template <int DIM, int N, typename T, int VAR, bool PACK = false>
+template <int DIM, int N, typename T, int VAR, bool PACK = false, class Out = T*>
 __device__ __forceinline__ double
 apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
            const double* __restrict__ Kt, const double* __restrict__ tauS,
            const int* __restrict__ nbr, const int* __restrict__ code,
            const int* __restrict__ bctype, const T* __restrict__ kfac, cptr<T> __restrict__ C,
-           cptr<int> __restrict__ tab, T* __restrict__ out) {
+           cptr<int> __restrict__ tab, Out out) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, N);
@@ -220,7 +248,7 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
   double dotv = 0.0;
 #pragma unroll
   for (int i = 0; i < Np; ++i) {
-    out[i * stride + e] = acc[i];
+    store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
     dotv += (double)uu[i] * (double)acc[i];
   }
   return dotv;

@@ -88,6 +88,22 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
   }
 }
 
+// Element pass with the Chebyshev step of the smoother fused into its output (orders >= 2).
+template <int DIM, int N, typename T, bool VAR, bool PACK>
+__global__ void __launch_bounds__(256)
+k_apply_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
+             const double* __restrict__ Kt, const double* __restrict__ tauS,
+             const int* __restrict__ nbr, const int* __restrict__ code,
+             const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
+             const int* __restrict__ tab, const T* __restrict__ f, const T* __restrict__ inv,
+             T* __restrict__ d, T* __restrict__ x_next, T c1, T c2) {
+  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e < E)
+    (void)apply_cell<DIM, N, T, VAR, PACK>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac,
+                                           as_const(C), as_const(tab),
+                                           ChebOut<T>{f, inv, d, x_next, c1, c2});
+}
+
 template <int DIM, int N, bool VAR>
 __global__ void __launch_bounds__(256)
 k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* __restrict__ tauS,
@@ -383,6 +399,34 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
 }
 
 int apply_blocks(int64_t E) { return (int)((E + 255) / 256); }
+
+template <typename T>
+bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
+                       const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
+                       hipStream_t s) {
+  if (l.N < 2 || (l.frag && mfma_apply_supported(m.dim, l.N))) return false;
+  if (x_next == u) throw std::invalid_argument("fused Chebyshev step: x_next must not alias x");
+  if (m.E == 0) return true;
+  const T* C = static_cast<const T*>(l.C);
+  const bool pack = m.nbc != nullptr;
+  const int* nw = pack ? m.nbc : m.nbr;
+#define INSP_CHEB(V, P)                                                                           \
+  INSP_DISPATCH_N(m.dim, l.N,                                                                     \
+                  hipLaunchKernelGGL((k_apply_cheb<D_, N_, T, V, P>), grid_for(m.E), dim3(256), 0, \
+                                     s, m.E, m.stride, u, tr, m.Kt, l.tauS, nw, m.code, m.bctype, \
+                                     kfac, C, l.tab, f, inv, d, x_next, c1, c2))
+  if (kfac) { if (pack) INSP_CHEB(true, true); else INSP_CHEB(true, false); }
+  else { if (pack) INSP_CHEB(false, true); else INSP_CHEB(false, false); }
+#undef INSP_CHEB
+  launch_check("k_apply_cheb");
+  return true;
+}
+template bool launch_apply_cheb<double>(const MeshView&, const LevelView&, const double*,
+                                        const double*, const double*, const double*, const double*,
+                                        double*, double*, double, double, hipStream_t);
+template bool launch_apply_cheb<float>(const MeshView&, const LevelView&, const float*,
+                                       const float*, const float*, const float*, const float*,
+                                       float*, float*, float, float, hipStream_t);
 
 template void launch_trace<double>(const MeshView&, const LevelView&, const double*, const double*,
                                    double*, hipStream_t);

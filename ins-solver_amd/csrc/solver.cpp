@@ -24,6 +24,7 @@ int64_t round_up(int64_t v, int64_t m) { return ((v + m - 1) / m) * m; }
 template <typename T>
 struct PoissonSolver::Work {
   DBuf<T> x, b, ax, d;     // V-cycle vectors
+  DBuf<T> x2;              // smoothing levels: the next iterate of a fused Chebyshev step
   DBuf<T> r, z, p, Ap;     // coarse CG vectors (coarsest level only)
   DBuf<T> tr;              // face traces
   DBuf<T> inv;             // inverse diagonal
@@ -530,6 +531,7 @@ void PoissonSolver::init() {
       w.tr.alloc(nt);
       if (type_ == SolverType::PMULTIGRID) {
         w.x.alloc(n); w.b.alloc(n); w.ax.alloc(n); w.d.alloc(n);
+        if (l + 1 < levels_.size() && lv.L.Np > (mesh_.dim + 1)) w.x2.alloc(n);
         if (l + 1 == levels_.size() && pmg_.coarse == CoarseType::JACOBI_CG) {
           w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n);
         }
@@ -653,19 +655,34 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
-  if (zero_init) {
+  // A step after an apply: fused into the element pass where the level has the vector pass and
+  // no mass term (the next iterate goes to x2, then x and x2 trade places), else apply + stream.
+  const MeshView mv = mesh_view();
+  const LevelView v = level_view(l, std::is_same<T, float>::value);
+  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && v.N >= 2 &&
+                    !(v.frag && mfma_apply_supported(mv.dim, v.N));
+  auto step = [&](T c1, T c2) {
+    ++smoother_applies_;
+    if (fuse) {
+      if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
+      const T* k = var_factor_ ? w.k.p : nullptr;
+      launch_trace<T>(mv, v, x, k, w.tr.p, stream_);
+      if (!launch_apply_cheb<T>(mv, v, x, w.tr.p, k, f, w.inv.p, w.d.p, w.x2.p, c1, c2, stream_))
+        throw std::logic_error("fused Chebyshev step not available on this level");
+      std::swap(w.x, w.x2);
+      x = w.x.p;
+      return;
+    }
+    apply_level<T>(l, x, w.ax.p);
+    cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, c1, c2, false, stream_);
+  };
+  if (zero_init)
     cheb_step<T>(n, f, nullptr, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), true, stream_);
-  } else {
-    apply_level<T>(l, x, w.ax.p);
-    ++smoother_applies_;
-    cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), false, stream_);
-  }
+  else
+    step((T)0, (T)(1.0 / theta));
   for (int k = 2; k <= K; ++k) {
-    apply_level<T>(l, x, w.ax.p);
-    ++smoother_applies_;
     const double rho = 1.0 / (2.0 * sigma - rho_old);
-    cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, (T)(rho * rho_old), (T)(2.0 * rho / delta),
-                 false, stream_);
+    step((T)(rho * rho_old), (T)(2.0 * rho / delta));
     rho_old = rho;
   }
 }
