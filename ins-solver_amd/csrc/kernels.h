@@ -42,10 +42,10 @@ template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
                   T* out, hipStream_t s, double* partial = nullptr);
 int apply_blocks(int64_t E);
-// Element pass of orders >= 2 (vector kernels) with one Chebyshev step of the smoother in place
-// of the store: res = inv (f - A u), d = c1 d + c2 res (d not read when c1 = 0),
+// Element pass (vector kernels, order 1 included) with one Chebyshev step of the smoother in
+// place of the store: res = inv (f - A u), d = c1 d + c2 res (d not read when c1 = 0),
 // x_next = u + d. x_next must differ from u. Returns false, launching nothing, where the level
-// takes another element pass (order 1, matrix cores).
+// takes the matrix-core element pass.
 template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,

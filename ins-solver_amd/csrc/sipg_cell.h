@@ -272,13 +272,13 @@ static_assert(face_alpha_is_rows(2) && face_alpha_is_rows(3), "face coefficient 
 
 // PACK: nbr holds one word per face, (neighbour << 5) | code inside the mesh and the
 // boundary index -(bface+1) on the boundary, and code is not read.
-template <int DIM, typename T, int VAR, bool PACK = false>
+template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
               const int* __restrict__ code, const int* __restrict__ bctype,
               const T* __restrict__ kfac, cptr<T> __restrict__ C, cptr<int> __restrict__ tab,
-              T* __restrict__ out) {
+              Out out) {
   constexpr OpLayout L = op_layout(DIM, 1);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, 1);
@@ -406,7 +406,7 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     }
 #pragma unroll
     for (int i = 0; i < Np; ++i) {
-      out[i * stride + e] = acc[i];
+      store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
       dotv += (double)uu[i] * (double)acc[i];
     }
   }

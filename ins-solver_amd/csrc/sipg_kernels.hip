@@ -104,6 +104,22 @@ k_apply_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const T* __rest
                                            ChebOut<T>{f, inv, d, x_next, c1, c2});
 }
 
+// Order-1 element pass with the Chebyshev step fused into its output.
+template <int DIM, typename T, bool VAR, bool PACK>
+__global__ void __launch_bounds__(256)
+k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
+                const double* __restrict__ tauS, const int* __restrict__ nbr,
+                const int* __restrict__ code, const int* __restrict__ bctype,
+                const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
+                const T* __restrict__ f, const T* __restrict__ inv, T* __restrict__ d,
+                T* __restrict__ x_next, T c1, T c2) {
+  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e < E)
+    (void)apply_p1_cell<DIM, T, VAR, PACK>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac,
+                                           as_const(C), as_const(tab),
+                                           ChebOut<T>{f, inv, d, x_next, c1, c2});
+}
+
 template <int DIM, int N, bool VAR>
 __global__ void __launch_bounds__(256)
 k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* __restrict__ tauS,
@@ -404,12 +420,27 @@ template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
                        hipStream_t s) {
-  if (l.N < 2 || (l.frag && mfma_apply_supported(m.dim, l.N))) return false;
+  if (l.frag && mfma_apply_supported(m.dim, l.N)) return false;
   if (x_next == u) throw std::invalid_argument("fused Chebyshev step: x_next must not alias x");
   if (m.E == 0) return true;
   const T* C = static_cast<const T*>(l.C);
   const bool pack = m.nbc != nullptr;
   const int* nw = pack ? m.nbc : m.nbr;
+  if (l.N == 1) {
+#define INSP_P1C(D, V, P)                                                                         \
+  hipLaunchKernelGGL((k_apply_p1_cheb<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, \
+                     u, m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, inv, d, x_next, c1, c2)
+    if (m.dim == 2) {
+      if (kfac) { if (pack) INSP_P1C(2, true, true); else INSP_P1C(2, true, false); }
+      else { if (pack) INSP_P1C(2, false, true); else INSP_P1C(2, false, false); }
+    } else {
+      if (kfac) { if (pack) INSP_P1C(3, true, true); else INSP_P1C(3, true, false); }
+      else { if (pack) INSP_P1C(3, false, true); else INSP_P1C(3, false, false); }
+    }
+#undef INSP_P1C
+    launch_check("k_apply_p1_cheb");
+    return true;
+  }
 #define INSP_CHEB(V, P)                                                                           \
   INSP_DISPATCH_N(m.dim, l.N,                                                                     \
                   hipLaunchKernelGGL((k_apply_cheb<D_, N_, T, V, P>), grid_for(m.E), dim3(256), 0, \

@@ -531,7 +531,7 @@ void PoissonSolver::init() {
       w.tr.alloc(nt);
       if (type_ == SolverType::PMULTIGRID) {
         w.x.alloc(n); w.b.alloc(n); w.ax.alloc(n); w.d.alloc(n);
-        if (l + 1 < levels_.size() && lv.L.Np > (mesh_.dim + 1)) w.x2.alloc(n);
+        if (l + 1 < levels_.size() || pmg_.coarse == CoarseType::AMG) w.x2.alloc(n);
         if (l + 1 == levels_.size() && pmg_.coarse == CoarseType::JACOBI_CG) {
           w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n);
         }
@@ -659,7 +659,7 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   // no mass term (the next iterate goes to x2, then x and x2 trade places), else apply + stream.
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
-  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && v.N >= 2 &&
+  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() &&
                     !(v.frag && mfma_apply_supported(mv.dim, v.N));
   auto step = [&](T c1, T c2) {
     ++smoother_applies_;
