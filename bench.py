@@ -91,6 +91,9 @@ def parse():
     ap.add_argument("--skip-viscous", action="store_true")
     ap.add_argument("--precision-only", action="store_true",
                     help="run only the precision-form leg and print its JSON (a profiling aid)")
+    ap.add_argument("--apply-only", type=int, default=0, metavar="ORDER",
+                    help="run only --steps FP64 operator applies of this order on the --n cube and "
+                         "print their mean time (a profiling aid for counter runs)")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -193,6 +196,29 @@ def cube_solver(ip, n, order, precision, rtol, coarse_iters):
     s.set_precision(ip.PRECISION_MIXED if precision == "mixed" else ip.PRECISION_FP64)
     s.init()
     return mesh, s, orders
+
+
+def apply_leg(ip, torch, n, order, steps):
+    """steps FP64 operator applies (device SoA, Dirichlet) on the bench cube, timed apart with HIP
+    events on the solver's stream: the workload of counter runs on one kernel."""
+    mesh = cube(ip, n, 20261202)
+    s = ip.PoissonSolver(mesh, order)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    s.set_solver(ip.SOLVER_JACOBI)
+    s.init()
+    x = torch.rand((s.Np, s.stride), dtype=torch.float64, device="cuda")
+    x[:, s.n_cells:] = 0
+    y = torch.zeros_like(x)
+    s.take_timing()
+    s.set_profiling(True)
+    for _ in range(max(steps, 1)):
+        s.mult((x.data_ptr(), y.data_ptr()), layout=ip.LAYOUT_DEVICE_SOA)
+    torch.cuda.synchronize()
+    s.set_profiling(False)
+    t = s.take_timing()
+    k = max(t["apply_launches"], 1)
+    return {"workload": f"cube {n}^3x6 Kuhn tets, order {order}, FP64 apply", "cells": s.n_cells,
+            "launches": t["apply_launches"], "trace_ms": t["trace_ms"] / k, "apply_ms": t["apply_ms"] / k}
 
 
 def timed_solves(s, ip, torch, b_d, x_d, steps, warmup, profile=False, tag=None):
@@ -590,6 +616,9 @@ def main():
     if a.precision_only:
         res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps, peaks=peaks)
         print(json.dumps({"precision_form": res}), flush=True)
+        return
+    if a.apply_only:
+        print(json.dumps({"apply_only": apply_leg(ip, torch, a.n, a.apply_only, a.steps)}), flush=True)
         return
     if a.copy_probe:
         # a plain copy of known size: 256 MiB read + 256 MiB written per launch
