@@ -1,0 +1,209 @@
+"""Independent SIPG assembly by quadrature: a checker for the oracle's operator.
+
+Nothing here shares code or tables with the oracle or the product. Each cell's
+Lagrange basis is built from its own nodal points (a monomial Vandermonde in
+scaled physical coordinates); volume and face integrals use collapsed
+Gauss-Legendre rules on the physical simplices; faces, normals, measures and
+face nodes are found from the vertex coordinates alone. The bilinear form is
+the textbook symmetric interior penalty form written with the reference's
+conventions (Appendix A of the reference kernels; each term cites its kernel):
+
+  a(u, v) = sum_K int_K Pi(k grad u) . grad v                      fvmf_2d_grad.h:8-11
+          + sum_F int_F (tau/2 [u] - n . {Pi(k grad u)}) v          fvmf_2d_mult_flux.h:9-10
+          + sum_F int_F Pi_F(-1/2 k [u]) n . grad v                 fvmf_2d_mult_flux.h:11-14
+  tau = 2(N+1)(N+2) max(fscale_L max_F k_L, fscale_R max_F k_R)    fvmf_2d_calc_tau_faces.h:9
+  Dirichlet faces: [u] = 2u, {q} = q                               vmf_2d_mult_avg_jump.h:32-38
+  tau_b = 2(N+1)(N+2) fscale max_F k                               fvmf_2d_calc_tau_bfaces.h:2-10
+
+with fscale = sJ / J, sJ = |F| / |F_ref| (|F_ref| = 2 in 2D and 3D) and
+J = |K| / |K_ref| (|K_ref| = 2 in 2D, 4/3 in 3D): the Hesthaven-Warburton
+surface-to-volume ratio. Pi(.) is the nodal interpolant of the product at the
+cell's nodes (the reference forms k * grad u node by node), Pi_F the
+interpolant through the face's nodes.
+"""
+import itertools
+import math
+
+import numpy as np
+
+
+def _gl01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1), 0.5 * w
+
+
+def unit_simplex_rule(d, n):
+    """Collapsed Gauss-Legendre rule on the unit d-simplex (vertex 0 at the origin).
+
+    Returns points (Q, d) and weights summing to 1/d!. Exact for polynomials of
+    total degree <= 2n - 1 - (d - 1)."""
+    if d == 1:
+        x, w = _gl01(n)
+        return x[:, None], w
+    x, w = _gl01(n)
+    pts, wts = [], []
+    for idx in itertools.product(range(n), repeat=d):
+        a = [x[i] for i in idx]
+        wt = 1.0
+        for i in idx:
+            wt *= w[i]
+        # Duffy map: y0 = a0, y1 = a1 (1 - a0), y2 = a2 (1 - a0)(1 - a1)
+        y, s = [], 1.0
+        for j in range(d):
+            y.append(a[j] * s)
+            if j + 1 < d:
+                wt *= (1 - a[j]) ** (d - 1 - j)
+            s *= (1 - a[j])
+        pts.append(y)
+        wts.append(wt)
+    return np.array(pts), np.array(wts)
+
+
+def monomial_exponents(dim, N):
+    return [e for e in itertools.product(range(N + 1), repeat=dim) if sum(e) <= N]
+
+
+class CellBasis:
+    """Lagrange basis of P_N through a cell's nodal points."""
+
+    def __init__(self, nodes, N):
+        self.dim = nodes.shape[1]
+        self.c = nodes.mean(axis=0)
+        self.h = np.ptp(nodes, axis=0).max()
+        self.ex = monomial_exponents(self.dim, N)
+        V = self._phi(nodes)
+        self.C = np.linalg.inv(V)
+
+    def _phi(self, x):
+        z = (np.atleast_2d(x) - self.c) / self.h
+        return np.stack([np.prod(z ** np.array(e), axis=1) for e in self.ex], axis=1)
+
+    def _dphi(self, x, d):
+        z = (np.atleast_2d(x) - self.c) / self.h
+        cols = []
+        for e in self.ex:
+            if e[d] == 0:
+                cols.append(np.zeros(z.shape[0]))
+                continue
+            e2 = list(e)
+            e2[d] -= 1
+            cols.append(e[d] * np.prod(z ** np.array(e2), axis=1) / self.h)
+        return np.stack(cols, axis=1)
+
+    def val(self, x):
+        """(Q, Np): basis values at points x."""
+        return self._phi(x) @ self.C
+
+    def grad(self, x):
+        """(dim, Q, Np): basis gradients at points x."""
+        return np.stack([self._dphi(x, d) @ self.C for d in range(self.dim)])
+
+
+def simplex_measure(V):
+    d = V.shape[1]
+    m = V.shape[0] - 1
+    E = (V[1:] - V[0]).T  # d x m
+    return math.sqrt(abs(np.linalg.det(E.T @ E))) / math.factorial(m)
+
+
+def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.0, sym_scale=1.0):
+    """Dense SIPG matrix (E*Np)^2 in the cells x Np node order of `xyz`.
+
+    coords (nn, dim), c2n (E, dim+1), xyz (dim, E, Np) nodal points;
+    bc_type_of(centroid) -> 0 Dirichlet / 1 Neumann (default Dirichlet);
+    k (E, Np) nodal coefficient or None (k = 1); tau_scale and sym_scale multiply the
+    penalty and the symmetry term (1 for the reference's form; other values let a test
+    show that a slip in either is detected)."""
+    dim = coords.shape[1]
+    E, Np = xyz.shape[1], xyz.shape[2]
+    nq = nq or N + 3
+    kk = np.ones((E, Np)) if k is None else np.asarray(k, dtype=float).reshape(E, Np)
+    pen = 2.0 * (N + 1) * (N + 2) * tau_scale
+    Kref = 2.0 if dim == 2 else 4.0 / 3.0
+    Fref = 2.0
+    qv, wv = unit_simplex_rule(dim, nq)
+    qf, wf = unit_simplex_rule(dim - 1, nq)
+    nodes = [xyz[:, e, :].T for e in range(E)]
+    bases = [CellBasis(nodes[e], N) for e in range(E)]
+    G = []  # nodal gradient matrices G[e][d][i, j] = d_d l_j(x_i)
+    for e in range(E):
+        G.append(bases[e].grad(nodes[e]))
+    A = np.zeros((E * Np, E * Np))
+    vol = np.zeros(E)
+
+    def blk(a, b):
+        return slice(a * Np, (a + 1) * Np), slice(b * Np, (b + 1) * Np)
+
+    for e in range(E):
+        V = coords[c2n[e]]
+        vol[e] = simplex_measure(V)
+        xq = V[0] + qv @ (V[1:] - V[0])
+        w = wv * vol[e] * math.factorial(dim)
+        L = bases[e].val(xq)
+        dL = bases[e].grad(xq)
+        for d in range(dim):
+            A[blk(e, e)] += dL[d].T @ (w[:, None] * (L @ (kk[e][:, None] * G[e][d])))
+
+    faces = {}
+    for e in range(E):
+        for vs in itertools.combinations(range(dim + 1), dim):
+            key = tuple(sorted(int(c2n[e][v]) for v in vs))
+            faces.setdefault(key, []).append(e)
+
+    def face_nodes(e, FV):
+        # nodes of cell e on the plane/line through the face vertices FV
+        n = outward_normal(coords[c2n[e]], FV)
+        dist = (nodes[e] - FV[0]) @ n
+        return np.nonzero(np.abs(dist) < 1e-10 * bases[e].h)[0]
+
+    def outward_normal(Vc, FV):
+        if dim == 2:
+            t = FV[1] - FV[0]
+            n = np.array([t[1], -t[0]])
+        else:
+            n = np.cross(FV[1] - FV[0], FV[2] - FV[0])
+        n = n / np.linalg.norm(n)
+        if (Vc.mean(axis=0) - FV[0]) @ n > 0:
+            n = -n
+        return n
+
+    for key, cells in faces.items():
+        FV = coords[list(key)]
+        area = simplex_measure(FV)
+        xq = FV[0] + qf @ (FV[1:] - FV[0])
+        w = wf * area * math.factorial(dim - 1)
+        sJ = area / Fref
+        fs = {e: sJ / (vol[e] / Kref) for e in cells}
+        fn = {e: face_nodes(e, FV) for e in cells}
+        kmax = {e: kk[e][fn[e]].max() for e in cells}
+        if len(cells) == 2:
+            tau = pen * max(fs[e] * kmax[e] for e in cells)
+            for e, o in (cells, cells[::-1]):
+                n = outward_normal(coords[c2n[e]], FV)
+                Le, Lo = bases[e].val(xq), bases[o].val(xq)
+                ndL = np.tensordot(n, bases[e].grad(xq), axes=1)  # (Q, Np)
+                qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
+                qo = sum(n[d] * (Lo @ (kk[o][:, None] * G[o][d])) for d in range(dim))
+                A[blk(e, e)] += Le.T @ (w[:, None] * (0.5 * tau * Le - 0.5 * qe))
+                A[blk(e, o)] += Le.T @ (w[:, None] * (-0.5 * tau * Lo - 0.5 * qo))
+                # symmetric term: Pi_F(-1/2 k_e (u_e - u_o)) through e's face nodes
+                S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
+                Ie = np.eye(Np)[fn[e]]
+                Uo = bases[o].val(nodes[e][fn[e]])  # u_o at e's face nodes
+                A[blk(e, e)] += -0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Ie))
+                A[blk(e, o)] += 0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Uo))
+        else:
+            e = cells[0]
+            centroid = FV.mean(axis=0)
+            bt = 0 if bc_type_of is None else bc_type_of(centroid)
+            if bt != 0:
+                continue
+            tau = pen * fs[e] * kmax[e]
+            n = outward_normal(coords[c2n[e]], FV)
+            Le = bases[e].val(xq)
+            ndL = np.tensordot(n, bases[e].grad(xq), axes=1)
+            qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
+            A[blk(e, e)] += Le.T @ (w[:, None] * (tau * Le - qe))
+            S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
+            A[blk(e, e)] += -sym_scale * ndL.T @ (w[:, None] * (S @ np.eye(Np)[fn[e]]))
+    return A
