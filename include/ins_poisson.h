@@ -86,8 +86,39 @@ typedef struct {
 int ins_mesh_generate_square(int nx, int ny, double jitter, uint64_t seed, ins_mesh_t* out);
 /* Unit cube, n^3 hexes split into 6 Kuhn tetrahedra, interior vertices jittered. */
 int ins_mesh_generate_cube(int n, double jitter, uint64_t seed, ins_mesh_t* out);
-/* Mesh from user arrays (cells are re-oriented positively). */
+/* Mesh from user arrays. Negatively oriented cells are re-oriented (their last two vertices
+ * swapped; ins_mesh_get returns the cell2nodes in use), and the library numbers the faces and
+ * boundary faces itself (ins_mesh_get_faces). */
 int ins_mesh_create(const ins_mesh_desc* desc, ins_mesh_t* out);
+/* The OP2 face sets of a caller's mesh (the HDF5 mesh's maps face2cells / bface2cells and dats
+ * faceNum / bfaceNum, src/ins2d.cpp:192; local faces numbered 0:(v0,v1) 1:(v1,v2) 2:(v2,v0) in
+ * 2D and 0:(v0,v1,v2) 1:(v0,v1,v3) 2:(v1,v2,v3) 3:(v0,v2,v3) in 3D). */
+typedef struct {
+  int64_t n_faces;
+  const int* face2cells; /* n_faces x 2 */
+  const int* faceNum;    /* n_faces x 2: local face on each side */
+  int64_t n_bfaces;
+  const int* bface2cells; /* n_bfaces */
+  const int* bfaceNum;    /* n_bfaces */
+} ins_mesh_faces_desc;
+/* Mesh from user arrays that keeps the caller's face and boundary-face order, as the
+ * reference's DGMesh2D/3D(filename) reads it (src/ins2d.cpp:192): bc_types and bc_data of the
+ * solvers built on it are indexed by the caller's bface numbers. Every face must be a face of
+ * the mesh with the same local face numbers, each given once; negatively oriented cells are
+ * rejected (INS_ERR_INVALID naming the cell), as the caller's face numbers and nodal data refer
+ * to its own vertex order. A face may join two boundary faces whose vertices match by a
+ * translation (a periodic face). ins_mesh_renumber re-derives the face sets in the library's
+ * order, and is refused on meshes with periodic faces. */
+int ins_mesh_create_op2(const ins_mesh_desc* desc, const ins_mesh_faces_desc* faces,
+                        ins_mesh_t* out);
+/* Replaces DGMesh2D/3D(filename) (src/ins2d.cpp:192): reads an OP2 HDF5 mesh file (sets nodes,
+ * cells, faces, bfaces; maps cell2nodes, face2cells, bface2cells; dats node_coords, faceNum or
+ * edgeNum, bfaceNum; the files of grids/h5) and builds the mesh as ins_mesh_create_op2 does,
+ * keeping the file's face and bface order. Faces that join two boundary faces whose vertices
+ * match by a translation are periodic faces (the meshes with a periodicFace dat). Reads the
+ * HDF5 subset OP2 writes (superblock 0/1, contiguous or compact datasets, no compression)
+ * without an HDF5 library; other files give INS_ERR_INVALID. */
+int ins_mesh_load_op2_hdf5(const char* path, ins_mesh_t* out);
 int ins_mesh_info(ins_mesh_t m, int* dim, int64_t* n_nodes, int64_t* n_cells, int64_t* n_faces,
                   int64_t* n_bfaces);
 /* Any output pointer may be NULL. */
@@ -127,6 +158,13 @@ int ins_poisson_info(ins_poisson_t h, int* np, int* nfp, int64_t* n_cells, int64
                      int64_t* n_bfaces);
 /* Nodal coordinates, dim x n_cells x Np (the toolkit dats x, y, z). */
 int ins_poisson_node_coords(ins_poisson_t h, double* xyz);
+/* Stride of every INS_LAYOUT_DEVICE_SOA vector the handle is given afterwards, dat[i*stride +
+ * cell]: the padded set size of the caller's OP2 GPU dats (-DDG_OP2_SOA with OP_AUTO_SOA,
+ * Makefile:40-41, code_gen.sh:31-33; getSetSizeFromOpArg, ls_reinit_gpu.cu:13,71-76). 0 (the
+ * default) = the stride reported by ins_poisson_info; otherwise at least n_cells, and then the
+ * entries of padding cells (cell >= n_cells) are neither read nor written. Face dats
+ * (cells x Nf*Nfp) share the stride. */
+int ins_poisson_set_dat_stride(ins_poisson_t h, int64_t stride);
 
 /* PoissonMatrix::set_bc_types(op_dat bfaces x 1) - src/solvers/2d/ins_solver.cpp:205. */
 int ins_poisson_set_bc_types(ins_poisson_t h, const int* bc_types, int64_t n_bfaces);

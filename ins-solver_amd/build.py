@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "ins_poisson", "libins_poisson.so")
 ARCH = os.environ.get("INS_POISSON_ARCH", "gfx950")
 
-SOURCES = ["refelem.cpp", "mesh.cpp", "solver.cpp", "batch.cpp", "server.cpp", "amg.cpp",
+SOURCES = ["refelem.cpp", "mesh.cpp", "hdf5_reader.cpp", "solver.cpp", "batch.cpp", "server.cpp", "amg.cpp",
            "capi.cpp", "sipg_kernels.hip", "vec_kernels.hip", "batch_kernels.hip",
            "amg_kernels.hip", "block_kernels.hip", "rhs_kernels.hip", "sipg_mfma.hip",
            "probe_kernels.hip", "mass_kernels.hip",

@@ -10,6 +10,7 @@
 #include <deque>
 
 #include "batch.h"
+#include "hdf5_reader.h"
 #include "mesh.h"
 #include "server.h"
 #include "solver.h"
@@ -43,6 +44,9 @@ int guarded(F&& f) {
   } catch (const insp::HipError& e) {
     g_last_error = e.what();
     return INS_ERR_HIP;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return INS_ERR_INVALID;
   } catch (const std::bad_alloc&) {
     g_last_error = "out of host memory";
     return INS_ERR_NOMEM;
@@ -209,6 +213,101 @@ int ins_mesh_create(const ins_mesh_desc* d, ins_mesh_t* out) {
       throw;
     }
     *out = m;
+    return INS_OK;
+  });
+}
+
+namespace {
+// A mesh with the caller's OP2 face sets (ins_mesh_create_op2, ins_mesh_load_op2_hdf5).
+ins_mesh* make_op2_mesh(int dim, int64_t n_nodes, const double* coords, int64_t n_cells,
+                        const int* cell2nodes, int64_t n_faces, const int* face2cells,
+                        const int* faceNum, int64_t n_bfaces, const int* bface2cells,
+                        const int* bfaceNum) {
+  if ((dim != 2 && dim != 3) || n_nodes < 1 || n_cells < 1 || !coords || !cell2nodes || n_faces < 0 ||
+      n_bfaces < 0)
+    throw insp::InvalidArgument("invalid mesh description");
+  const int nv = dim + 1;
+  for (int64_t i = 0; i < n_cells * nv; ++i)
+    if (cell2nodes[i] < 0 || cell2nodes[i] >= n_nodes)
+      throw insp::InvalidArgument("cell2nodes index out of range");
+  auto m = std::make_unique<ins_mesh>();
+  m->m.dim = dim;
+  m->m.n_nodes = n_nodes;
+  m->m.n_cells = n_cells;
+  m->m.coords.assign(coords, coords + n_nodes * dim);
+  m->m.cell2nodes.assign(cell2nodes, cell2nodes + n_cells * nv);
+  // The caller's local face numbers and nodal data refer to its own vertex order, so a
+  // negatively oriented cell cannot be re-oriented here: it is rejected.
+  const int64_t neg = insp::first_negative_cell(m->m);
+  if (neg >= 0)
+    throw insp::InvalidArgument("cell " + std::to_string(neg) +
+                                " is negatively oriented (its vertices must be listed counter-"
+                                "clockwise in 2D, with positive volume in 3D)");
+  insp::build_topology(m->m);
+  insp::adopt_op2_faces(m->m, n_faces, face2cells, faceNum, n_bfaces, bface2cells, bfaceNum);
+  insp::build_geometry(m->m);
+  return m.release();
+}
+}  // namespace
+
+int ins_mesh_create_op2(const ins_mesh_desc* d, const ins_mesh_faces_desc* f, ins_mesh_t* out) {
+  return guarded([&] {
+    if (!d || !f || !out) throw insp::InvalidArgument("invalid mesh description");
+    *out = make_op2_mesh(d->dim, d->n_nodes, d->coords, d->n_cells, d->cell2nodes, f->n_faces,
+                         f->face2cells, f->faceNum, f->n_bfaces, f->bface2cells, f->bfaceNum);
+    return INS_OK;
+  });
+}
+
+int ins_mesh_load_op2_hdf5(const char* path, ins_mesh_t* out) {
+  return guarded([&] {
+    if (!path || !out) throw insp::InvalidArgument("null path or output");
+    const insp::Hdf5File f(path);
+    auto pick = [&](const char* a, const char* b) -> std::string {
+      if (f.has(a)) return a;
+      if (f.has(b)) return b;
+      return "";
+    };
+    auto set_size = [&](const char* a, const char* b) -> int64_t {
+      const std::string n = pick(a, b);
+      if (n.empty()) return -1;
+      const std::vector<int32_t> v = f.read_int(n);
+      if (v.size() != 1) throw insp::InvalidArgument(std::string(path) + ": set " + n + " is not a size");
+      return v[0];
+    };
+    std::vector<uint64_t> dc, dm;
+    const std::vector<double> coords = f.read_double("node_coords", &dc);
+    const std::vector<int32_t> c2n = f.read_int("cell2nodes", &dm);
+    if (dc.size() != 2 || (dc[1] != 2 && dc[1] != 3))
+      throw insp::InvalidArgument(std::string(path) + ": node_coords must be nodes x 2 or nodes x 3");
+    const int dim = (int)dc[1];
+    if (dm.size() != 2 || dm[1] != (uint64_t)dim + 1)
+      throw insp::InvalidArgument(std::string(path) + ": cell2nodes must be cells x (dim+1)");
+    const int64_t n_nodes = (int64_t)dc[0], n_cells = (int64_t)dm[0];
+    if (set_size("nodes", "nodes") >= 0 && set_size("nodes", "nodes") != n_nodes)
+      throw insp::InvalidArgument(std::string(path) + ": set nodes disagrees with node_coords");
+    if (set_size("cells", "cells") >= 0 && set_size("cells", "cells") != n_cells)
+      throw insp::InvalidArgument(std::string(path) + ": set cells disagrees with cell2nodes");
+    const std::string f2c_name = pick("face2cells", "edge2cells"), fnum_name = pick("faceNum", "edgeNum");
+    if (f2c_name.empty() || fnum_name.empty())
+      throw insp::InvalidArgument(std::string(path) + ": no face2cells / faceNum (edgeNum) maps");
+    std::vector<uint64_t> df, dn;
+    const std::vector<int32_t> f2c = f.read_int(f2c_name, &df), fnum = f.read_int(fnum_name, &dn);
+    if (df.size() != 2 || df[1] != 2 || dn != df)
+      throw insp::InvalidArgument(std::string(path) + ": face maps must be faces x 2");
+    const int64_t n_faces = (int64_t)df[0];
+    const std::string b2c_name = pick("bface2cells", "bedge2cells"), bnum_name = pick("bfaceNum", "bedgeNum");
+    std::vector<int32_t> b2c, bnum;
+    if (!b2c_name.empty() && !bnum_name.empty()) {
+      b2c = f.read_int(b2c_name);
+      bnum = f.read_int(bnum_name);
+      if (b2c.size() != bnum.size())
+        throw insp::InvalidArgument(std::string(path) + ": bface maps differ in size");
+    } else if (set_size("bfaces", "bedges") > 0) {
+      throw insp::InvalidArgument(std::string(path) + ": boundary faces without bface2cells / bfaceNum");
+    }
+    *out = make_op2_mesh(dim, n_nodes, coords.data(), n_cells, c2n.data(), n_faces, f2c.data(),
+                         fnum.data(), (int64_t)b2c.size(), b2c.data(), bnum.data());
     return INS_OK;
   });
 }
@@ -528,6 +627,13 @@ int ins_poisson_solve(ins_poisson_t h, const double* rhs, double* ans, int layou
 int ins_poisson_mult(ins_poisson_t h, const double* in, double* out, int layout) {
   return guarded([&] {
     S(h).mult(in, out, to_layout(layout), false);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_dat_stride(ins_poisson_t h, int64_t stride) {
+  return guarded([&] {
+    S(h).set_dat_stride(stride);
     return INS_OK;
   });
 }

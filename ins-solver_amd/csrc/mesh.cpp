@@ -5,6 +5,7 @@
 #include <cmath>
 #include <random>
 #include <stdexcept>
+#include <string>
 
 #include "refelem.h"
 
@@ -205,6 +206,104 @@ void build_topology(HostMesh& m) {
   }
 }
 
+int64_t first_negative_cell(const HostMesh& m) {
+  for (int64_t e = 0; e < m.n_cells; ++e)
+    if (signed_measure(m, e) < 0) return e;
+  return -1;
+}
+
+namespace {
+// Permutation p (the index used by nbr_perm) with theirs[k] = mine[perm[p][k]] for the face
+// vertices of two cells, matched by coordinates up to one common translation: the identity
+// translation on a shared face, a period on a periodic one. -1 when none matches.
+int periodic_face_perm(const HostMesh& m, int64_t e, int f, int64_t en, int fn) {
+  static const int perms2[2][3] = {{0, 1, 0}, {1, 0, 0}};
+  static const int perms3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  const int d = m.dim, nv = d + 1, nperm = d == 2 ? 2 : 6;
+  const int* fv = face_vertices(d);
+  double mine[3][3], theirs[3][3];
+  double scale = 0;
+  for (int k = 0; k < d; ++k)
+    for (int x = 0; x < d; ++x) {
+      mine[k][x] = m.coords[(size_t)m.cell2nodes[(size_t)e * nv + fv[f * d + k]] * d + x];
+      theirs[k][x] = m.coords[(size_t)m.cell2nodes[(size_t)en * nv + fv[fn * d + k]] * d + x];
+    }
+  for (int x = 0; x < d; ++x) scale = std::max(scale, std::fabs(mine[1][x] - mine[0][x]));
+  const double tol = 1e-9 * scale;
+  for (int p = 0; p < nperm; ++p) {
+    const int* src = d == 2 ? perms2[p] : perms3[p];
+    double t[3];
+    for (int x = 0; x < d; ++x) t[x] = theirs[0][x] - mine[src[0]][x];
+    bool ok = true;
+    for (int k = 1; k < d && ok; ++k)
+      for (int x = 0; x < d; ++x) ok &= std::fabs(theirs[k][x] - mine[src[k]][x] - t[x]) <= tol;
+    if (ok) return p;
+  }
+  return -1;
+}
+}  // namespace
+
+void adopt_op2_faces(HostMesh& m, int64_t n_faces, const int* face2cells, const int* faceNum,
+                     int64_t n_bfaces, const int* bface2cells, const int* bfaceNum) {
+  const int Nf = m.dim + 1;
+  auto bad = [](const std::string& what) { throw std::invalid_argument("face maps: " + what); };
+  if (n_faces < 0 || n_bfaces < 0) bad("negative set size");
+  if ((n_faces > 0 && (!face2cells || !faceNum)) || (n_bfaces > 0 && (!bface2cells || !bfaceNum)))
+    bad("null map");
+  std::vector<char> seen((size_t)m.n_cells * Nf, 0);
+  auto slot = [&](int c, int f) -> size_t {
+    if (c < 0 || c >= m.n_cells || f < 0 || f >= Nf) bad("cell or local face out of range");
+    return (size_t)c * Nf + f;
+  };
+  int64_t periodic = 0;
+  for (int64_t i = 0; i < n_faces; ++i) {
+    const int c0 = face2cells[2 * i], c1 = face2cells[2 * i + 1];
+    const int f0 = faceNum[2 * i], f1 = faceNum[2 * i + 1];
+    const size_t s0 = slot(c0, f0), s1 = slot(c1, f1);
+    if (seen[s0] || seen[s1]) bad("face " + std::to_string(i) + " given twice");
+    seen[s0] = seen[s1] = 1;
+    if (m.nbr[s0] == c1 && m.nbr_face[s0] == f1) continue;
+    // a periodic face: both sides lie on the mesh boundary and match by a translation
+    const int p01 = m.nbr[s0] < 0 && m.nbr[s1] < 0 && c0 != c1 ? periodic_face_perm(m, c0, f0, c1, f1) : -1;
+    const int p10 = p01 >= 0 ? periodic_face_perm(m, c1, f1, c0, f0) : -1;
+    if (p10 < 0)
+      bad("face " + std::to_string(i) + " (cells " + std::to_string(c0) + ", " + std::to_string(c1) +
+          ", local faces " + std::to_string(f0) + ", " + std::to_string(f1) +
+          ") is neither a face of the mesh with those local face numbers nor a periodic pair");
+    m.nbr[s0] = c1;
+    m.nbr_face[s0] = f1;
+    m.nbr_perm[s0] = p01;
+    m.nbr[s1] = c0;
+    m.nbr_face[s1] = f0;
+    m.nbr_perm[s1] = p10;
+    ++periodic;
+  }
+  const int64_t interior = m.n_faces + periodic, boundary = m.n_bfaces - 2 * periodic;
+  if (n_faces != interior)
+    bad("n_faces " + std::to_string(n_faces) + " but the mesh has " + std::to_string(interior) +
+        " interior faces");
+  if (n_bfaces != boundary)
+    bad("n_bfaces " + std::to_string(n_bfaces) + " but the mesh has " + std::to_string(boundary) +
+        " boundary faces");
+  for (int64_t b = 0; b < n_bfaces; ++b) {
+    const size_t sb = slot(bface2cells[b], bfaceNum[b]);
+    if (m.nbr[sb] >= 0)
+      bad("bface " + std::to_string(b) + " (cell " + std::to_string(bface2cells[b]) + ", local face " +
+          std::to_string(bfaceNum[b]) + ") is not on the boundary");
+    if (seen[sb]) bad("bface " + std::to_string(b) + " given twice");
+    seen[sb] = 1;
+  }
+  m.n_faces = n_faces;
+  m.n_bfaces = n_bfaces;
+  m.n_periodic = periodic;
+  m.face2cells.assign(face2cells, face2cells + 2 * n_faces);
+  m.faceNum.assign(faceNum, faceNum + 2 * n_faces);
+  m.bface2cells.assign(bface2cells, bface2cells + n_bfaces);
+  m.bfaceNum.assign(bfaceNum, bfaceNum + n_bfaces);
+  for (int64_t b = 0; b < n_bfaces; ++b)
+    m.nbr[(size_t)bface2cells[b] * Nf + bfaceNum[b]] = -(int)b - 1;
+}
+
 namespace {
 // Affine map of cell e: J = det(dx/dxi) and G[a][x] = d xi_a / d x.
 void cell_metric(const HostMesh& m, int64_t e, double G[3][3], double& J) {
@@ -399,6 +498,8 @@ std::vector<int64_t> shape_class_order(const HostMesh& m, int mode) {
 
 HostMesh renumber_cells(const HostMesh& m, const std::vector<int64_t>& perm) {
   if ((int64_t)perm.size() != m.n_cells) throw std::invalid_argument("permutation size");
+  if (m.n_periodic > 0)
+    throw std::invalid_argument("meshes with periodic faces keep the caller's cell numbering");
   const int nv = m.dim + 1;
   std::vector<char> seen((size_t)m.n_cells, 0);
   HostMesh r;

@@ -25,6 +25,7 @@ struct HostMesh {
   int64_t n_bfaces = 0;
   std::vector<int> bface2cells; // n_bfaces
   std::vector<int> bfaceNum;    // n_bfaces
+  int64_t n_periodic = 0;       // interior faces joining two boundary faces by a translation
 
   // Geometry (build_geometry): per cell J and the metric K[r][s] = grad(xi_r).grad(xi_s)
   // scaled by J (Ktilde), stored as 3 (2D: rr, rs, ss) or 6 (3D: rr, rs, rt, ss, st, tt).
@@ -41,6 +42,15 @@ HostMesh make_cube_mesh(int n, double jitter, uint64_t seed);
 
 // Orients cells positively, then builds faces and neighbour tables.
 void build_topology(HostMesh& m);
+// Index of the first cell with a negative signed measure, or -1.
+int64_t first_negative_cell(const HostMesh& m);
+// Adopts a caller's OP2 face sets (face2cells/faceNum, bface2cells/bfaceNum, in the caller's
+// order) after build_topology: every face must be one of the mesh's faces with the same local
+// face numbers (either side first), or a periodic pair of two boundary faces whose vertices
+// match up to one translation, and each exactly once. The boundary-face numbering then
+// follows the caller's bface order (bc_types and bc_data are indexed by it).
+void adopt_op2_faces(HostMesh& m, int64_t n_faces, const int* face2cells, const int* faceNum,
+                     int64_t n_bfaces, const int* bface2cells, const int* bfaceNum);
 void build_geometry(HostMesh& m);
 
 // A cell order that groups cells of the same shape class (edge directions from the first

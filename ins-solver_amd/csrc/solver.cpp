@@ -542,6 +542,11 @@ void PoissonSolver::init() {
   }
   const size_t n0 = (size_t)np() * stride_;
   b_.alloc(n0); x_.alloc(n0); r_.alloc(n0); z_.alloc(n0); zold_.alloc(n0); p_.alloc(n0); Ap_.alloc(n0);
+  if (mixed_refinement()) {
+    z32_.alloc(n0); zold32_.alloc(n0); p32_.alloc(n0); Ap32_.alloc(n0);
+  } else {
+    z32_.release(); zold32_.release(); p32_.release(); Ap32_.release();
+  }
   if (type_ == SolverType::PMULTIGRID) estimate_eigenvalues();
   amg_.reset();
   if (type_ == SolverType::PMULTIGRID && pmg_.coarse == CoarseType::AMG) build_coarse_space();
@@ -1001,6 +1006,83 @@ void PoissonSolver::precondition(const double* r, double* z) {
   }
 }
 
+bool PoissonSolver::mixed_refinement() const {
+  return prec_ == Precision::MIXED && type_ == SolverType::PMULTIGRID;
+}
+
+// Flexible CG with the FP32 V-cycle, FP32 operator and FP32 Krylov vectors; the iterate x,
+// the right-hand side and every true residual r = b - A x are FP64, and the inner products and
+// step lengths are accumulated and formed in FP64 on the device. The recursively updated FP32
+// residual is replaced by the true FP64 residual (iterative refinement without a restart of
+// the Krylov recurrence) whenever it has fallen by kRefineRatio since the last replacement or
+// below the tolerance; the solve stops only on a true residual.
+int PoissonSolver::fcg_mixed(double tol, int* iters, double* rr_out, int64_t* fine_applies) {
+  constexpr double kRefineRatio = 1e-2;
+  const int64_t n = (int64_t)np() * stride_;
+  RedScratch rs{partial_.p};
+  Level& l0 = *levels_[0];
+  Work<float>& w = l0.w32;
+  float* r = w.b.p;  // the V-cycle reads its right-hand side from the level's b
+  auto true_residual = [&]() {
+    apply_level<double>(0, x_.p, Ap_.p);
+    ++*fine_applies;
+    residual_f32(n, b_.p, Ap_.p, r, scal_.p, S_RR, rs, stream_);
+    return read_scalar(S_RR);
+  };
+  auto precondition32 = [&]() {
+    vcycle<float>(0);
+    std::swap(z32_, w.x);  // the V-cycle's result becomes z; w.x is rewritten by the next one
+    if (nullspace_) {
+      vsum<float>(n, z32_.p, scal_.p, S_MEAN, rs, stream_);
+      sub_mean<float>(np(), mesh_.n_cells, stride_, z32_.p, scal_.p, S_MEAN,
+                      (double)np() * mesh_.n_cells, stream_);
+    }
+  };
+  int it = 0;
+  double rr = true_residual();
+  *rr_out = rr;
+  if (!std::isfinite(rr)) return -3;
+  if (std::sqrt(rr) <= tol) return 0;
+  double last_true = std::sqrt(rr);
+  precondition32();
+  dot<float>(n, r, z32_.p, nullptr, nullptr, 1, scal_.p, S_RZ, rs, stream_);
+  copy<float>(n, z32_.p, p32_.p, stream_);
+  int status = 1;
+  while (it < max_iter_) {
+    apply_level<float>(0, p32_.p, Ap32_.p, partial_.p);  // p.Ap fused into the element pass
+    finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_PAP, stream_);
+    cg_update_xr_mixed(n, x_.p, r, p32_.p, Ap32_.p, scal_.p, S_RZ, S_PAP, S_RR, rs, stream_);
+    ++it;
+    read_scalar(0);
+    const double pap = host_scal_[S_PAP];
+    rr = host_scal_[S_RR];
+    if (!(pap > 0) || !std::isfinite(rr) || !std::isfinite(pap)) { status = -3; break; }
+    if (std::sqrt(rr) <= tol || std::sqrt(rr) <= kRefineRatio * last_true) {
+      rr = true_residual();
+      *rr_out = rr;
+      if (!std::isfinite(rr)) { status = -3; break; }
+      if (std::sqrt(rr) <= tol) { status = 0; break; }
+      last_true = std::sqrt(rr);
+    }
+    if (it >= max_iter_) break;
+    std::swap(z32_, zold32_);
+    precondition32();
+    dot<float>(n, r, z32_.p, nullptr, nullptr, 1, scal_.p, S_RZN, rs, stream_);
+    dot_flex<float>(n, r, z32_.p, zold32_.p, scal_.p, S_BNUM, rs, stream_);
+    cg_update_p<float>(n, p32_.p, z32_.p, scal_.p, S_BNUM, S_RZ, stream_);
+    scal_copy(scal_.p, S_RZ, S_RZN, stream_);
+  }
+  if (status == 1) {
+    // not converged within max_iter: report the true residual of the final iterate
+    rr = true_residual();
+    *rr_out = rr;
+    if (!std::isfinite(rr)) status = -3;
+    else if (std::sqrt(rr) <= tol) status = 0;
+  }
+  *iters = it;
+  return status;
+}
+
 void PoissonSolver::import_vec(const double* src, double* dst, Layout layout, int rows) {
   const int64_t E = mesh_.n_cells;
   if (!src) throw InvalidArgument("null vector");
@@ -1013,10 +1095,15 @@ void PoissonSolver::import_vec(const double* src, double* dst, Layout layout, in
     INSP_HIP(hipStreamSynchronize(stream_));
   } else if (layout == Layout::DEVICE_AOS) {
     soa_from_aos(rows, E, stride_, src, dst, stream_);
-  } else {
+  } else if (dat_stride() == stride_) {
     if (src != dst)
       INSP_HIP(hipMemcpyAsync(dst, src, (size_t)rows * stride_ * sizeof(double),
                               hipMemcpyDeviceToDevice, stream_));
+    zero_padding<double>(rows, E, stride_, dst, stream_);
+  } else {
+    // the caller's dat stride (OP2 pads its SoA sets by its own rule)
+    INSP_HIP(hipMemcpy2DAsync(dst, stride_ * sizeof(double), src, dat_stride() * sizeof(double),
+                              E * sizeof(double), rows, hipMemcpyDeviceToDevice, stream_));
     zero_padding<double>(rows, E, stride_, dst, stream_);
   }
 }
@@ -1033,10 +1120,20 @@ void PoissonSolver::export_vec(const double* src, double* dst, Layout layout, in
     INSP_HIP(hipStreamSynchronize(stream_));
   } else if (layout == Layout::DEVICE_AOS) {
     aos_from_soa(rows, E, stride_, src, dst, stream_);
+  } else if (dat_stride() != stride_) {
+    // only the caller's cells are written; its padding entries are left as they are
+    INSP_HIP(hipMemcpy2DAsync(dst, dat_stride() * sizeof(double), src, stride_ * sizeof(double),
+                              E * sizeof(double), rows, hipMemcpyDeviceToDevice, stream_));
   } else if (src != dst) {
     INSP_HIP(hipMemcpyAsync(dst, src, (size_t)rows * stride_ * sizeof(double),
                             hipMemcpyDeviceToDevice, stream_));
   }
+}
+
+void PoissonSolver::set_dat_stride(int64_t s) {
+  if (s != 0 && s < mesh_.n_cells)
+    throw InvalidArgument("dat stride must be 0 (the solver's own) or at least the number of cells");
+  dat_stride_ = s;
 }
 
 int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveStats* stats) {
@@ -1065,6 +1162,8 @@ int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveSta
   double rr_true = 0;
   if (!std::isfinite(bnorm)) {
     status = -3;
+  } else if (mixed_refinement()) {
+    status = fcg_mixed(std::max(rtol_ * bnorm, atol_), &it, &rr_true, &fine_applies);
   } else {
     const double tol = std::max(rtol_ * bnorm, atol_);
     for (int restart = 0;; ++restart) {
@@ -1176,7 +1275,7 @@ void PoissonSolver::pressure_rhs(double b0, double b1, double dt, const double* 
     Mfine_.upload(fine_re_.M.data(), fine_re_.M.size());
     Efine_.upload(fine_re_.EMAT.data(), fine_re_.EMAT.size());
   }
-  if (layout == Layout::DEVICE_SOA) {
+  if (layout == Layout::DEVICE_SOA && dat_stride() == stride_) {
     launch_pressure_rhs(mesh_.dim, N_, mesh_.n_cells, stride_, J_.p, Mfine_.p, Efine_.p, b0, b1, dt, dPdN,
                         dPdNOld, div, stream_);
     INSP_HIP(hipStreamSynchronize(stream_));

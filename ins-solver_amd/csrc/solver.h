@@ -135,6 +135,10 @@ class PoissonSolver {
   int np() const { return fine_re_.Np; }
   int nfp() const { return fine_re_.Nfp; }
   int64_t stride() const { return stride_; }
+  // Stride of the caller's device SoA vectors (Layout::DEVICE_SOA), dat[i*stride + cell]:
+  // 0 = the solver's own stride(); otherwise at least n_cells (an OP2 dat's padded set size).
+  void set_dat_stride(int64_t s);
+  int64_t dat_stride() const { return dat_stride_ ? dat_stride_ : stride_; }
   int64_t n_cells() const { return mesh_.n_cells; }
   hipStream_t stream() const { return stream_; }
   const RefElement& ref() const { return fine_re_; }
@@ -229,6 +233,10 @@ class PoissonSolver {
   template <typename T>
   void coarse_amg(int l);
   void precondition(const double* r, double* z);
+  // FP32 inner iterations with FP64 iterate, true residuals and step lengths (p-multigrid
+  // with Precision::MIXED); returns the solve status.
+  bool mixed_refinement() const;
+  int fcg_mixed(double tol, int* iters, double* rr_out, int64_t* fine_applies);
   void import_vec(const double* src, double* dst, Layout layout, int rows);
   void export_vec(const double* src, double* dst, Layout layout, int rows);
   double read_scalar(int slot);
@@ -240,6 +248,7 @@ class PoissonSolver {
   RefElement fine_re_;
   hipStream_t stream_;
   bool own_stream_ = false;
+  int64_t dat_stride_ = 0;  // caller's SoA stride, 0 = stride_
   int64_t stride_;
   // device mesh
   DBuf<double> Kt_, J_;
@@ -281,6 +290,8 @@ class PoissonSolver {
 
   // outer FCG vectors (FP64, fine level)
   DBuf<double> b_, x_, r_, z_, zold_, p_, Ap_;
+  // FP32 Krylov vectors of the mixed solve (its residual lives in the fine level's w32.b)
+  DBuf<float> z32_, zold32_, p32_, Ap32_;
   DBuf<double> scal_;
   DBuf<double> partial_;
   double* host_scal_ = nullptr;  // pinned

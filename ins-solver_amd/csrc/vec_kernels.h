@@ -33,6 +33,13 @@ void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, in
 // r = b - Ax, scal[slot] = |r|^2
 void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
               RedScratch rs, hipStream_t s);
+// r = b - Ax computed in FP64 and stored in FP32; scal[slot] = |b - Ax|^2 (FP64)
+void residual_f32(int64_t n, const double* b, const double* Ax, float* r, double* scal, int slot,
+                  RedScratch rs, hipStream_t s);
+// alpha = scal[num]/scal[den]; x (FP64) += alpha p; r (FP32) -= alpha Ap; scal[out] = |r|^2
+// accumulated in FP64
+void cg_update_xr_mixed(int64_t n, double* x, float* r, const float* p, const float* Ap, double* scal,
+                        int num, int den, int out, RedScratch rs, hipStream_t s);
 // alpha = scal[num]/scal[den]; x += alpha p; r -= alpha Ap; scal[out] = |r|^2
 // (or r.z with z = invdiag*r when invdiag is given).
 template <typename T>

@@ -107,6 +107,36 @@ k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ A
 
 __device__ __forceinline__ double safe_div(double a, double b) { return b != 0.0 ? a / b : 0.0; }
 
+// True residual in FP64, stored in FP32 for the inner iterations of the mixed solve.
+__global__ void __launch_bounds__(kBlock)
+k_residual_f32(int64_t n, const double* __restrict__ b, const double* __restrict__ Ax,
+               float* __restrict__ r, double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double t = b[i] - Ax[i];
+    r[i] = (float)t;
+    v[0] += t * t;
+  }
+  write_partials<1>(v, partial);
+}
+
+// x (FP64) += alpha p, r (FP32) -= alpha Ap, |r|^2 accumulated in FP64.
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_xr_mixed(int64_t n, double* __restrict__ x, float* __restrict__ r,
+                     const float* __restrict__ p, const float* __restrict__ Ap,
+                     const double* __restrict__ scal, int num, int den, double* __restrict__ partial) {
+  const double alpha = safe_div(scal[num], scal[den]);
+  const float a = (float)alpha;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    x[i] = fma(alpha, (double)p[i], x[i]);
+    const float ri = fmaf(-a, Ap[i], r[i]);
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  write_partials<1>(v, partial);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_cg_update_xr(int64_t n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
@@ -327,6 +357,21 @@ void residual(int64_t n, const double* b, const double* Ax, double* r, double* s
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial); launch_check("k_residual");
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
+}
+
+void residual_f32(int64_t n, const double* b, const double* Ax, float* r, double* scal, int slot,
+                  RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_residual_f32, dim3(nb), dim3(kBlock), 0, s, n, b, Ax, r, rs.partial); launch_check("k_residual_f32");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
+}
+
+void cg_update_xr_mixed(int64_t n, double* x, float* r, const float* p, const float* Ap, double* scal,
+                        int num, int den, int out, RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_cg_update_xr_mixed, dim3(nb), dim3(kBlock), 0, s, n, x, r, p, Ap, scal, num, den,
+                     rs.partial); launch_check("k_cg_update_xr_mixed");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
 }
 
 template <typename T>

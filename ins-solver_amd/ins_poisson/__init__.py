@@ -49,9 +49,10 @@ ERR_BREAKDOWN = -3
 ERR_NOMEM = -4
 
 EXPORTED = [
-    "ins_mesh_generate_square", "ins_mesh_generate_cube", "ins_mesh_create", "ins_mesh_info",
+    "ins_mesh_generate_square", "ins_mesh_generate_cube", "ins_mesh_create", "ins_mesh_create_op2", "ins_mesh_load_op2_hdf5",
+    "ins_mesh_info",
     "ins_mesh_get", "ins_mesh_get_faces", "ins_mesh_node_coords", "ins_mesh_destroy", "ins_poisson_create",
-    "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords",
+    "ins_poisson_destroy", "ins_poisson_info", "ins_poisson_node_coords", "ins_poisson_set_dat_stride",
     "ins_poisson_set_bc_types", "ins_poisson_set_factor", "ins_poisson_set_bcs",
     "ins_poisson_set_nullspace", "ins_poisson_set_tol_and_iter", "ins_poisson_set_solver",
     "ins_poisson_set_precision", "ins_poisson_set_element_pass", "ins_poisson_set_mm_factor",
@@ -78,6 +79,12 @@ class InsPoissonError(RuntimeError):
 class MeshDesc(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_nodes", C.c_int64), ("coords", C.POINTER(C.c_double)),
                 ("n_cells", C.c_int64), ("cell2nodes", C.POINTER(C.c_int))]
+
+
+class MeshFacesDesc(C.Structure):
+    _fields_ = [("n_faces", C.c_int64), ("face2cells", C.POINTER(C.c_int)),
+                ("faceNum", C.POINTER(C.c_int)), ("n_bfaces", C.c_int64),
+                ("bface2cells", C.POINTER(C.c_int)), ("bfaceNum", C.POINTER(C.c_int))]
 
 
 class SolveStats(C.Structure):
@@ -112,6 +119,8 @@ def _load():
         "ins_mesh_generate_square": [C.c_int, C.c_int, dbl, C.c_uint64, C.POINTER(vp)],
         "ins_mesh_generate_cube": [C.c_int, dbl, C.c_uint64, C.POINTER(vp)],
         "ins_mesh_create": [C.POINTER(MeshDesc), C.POINTER(vp)],
+        "ins_mesh_create_op2": [C.POINTER(MeshDesc), C.POINTER(MeshFacesDesc), C.POINTER(vp)],
+        "ins_mesh_load_op2_hdf5": [C.c_char_p, C.POINTER(vp)],
         "ins_mesh_info": [vp, C.POINTER(C.c_int), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
                           C.POINTER(i64)],
         "ins_mesh_get": [vp, vp, vp],
@@ -121,6 +130,7 @@ def _load():
         "ins_poisson_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(i64),
                              C.POINTER(i64), C.POINTER(i64)],
         "ins_poisson_node_coords": [vp, vp],
+        "ins_poisson_set_dat_stride": [vp, i64],
         "ins_poisson_set_bc_types": [vp, vp, i64],
         "ins_poisson_set_factor": [vp, vp, C.c_int],
         "ins_poisson_set_bcs": [vp, vp, C.c_int],
@@ -273,6 +283,31 @@ class Mesh:
         _check(lib.ins_mesh_create(C.byref(d), C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def from_op2(cls, coords, cell2nodes, face2cells, faceNum, bface2cells, bfaceNum):
+        """Mesh that keeps the caller's OP2 face and bface order (ins_mesh_create_op2)."""
+        coords = _f64(coords)
+        c2n = np.ascontiguousarray(cell2nodes, dtype=np.int32)
+        f2c = np.ascontiguousarray(face2cells, dtype=np.int32).reshape(-1, 2)
+        fn = np.ascontiguousarray(faceNum, dtype=np.int32).reshape(-1, 2)
+        b2c = np.ascontiguousarray(bface2cells, dtype=np.int32).ravel()
+        bn = np.ascontiguousarray(bfaceNum, dtype=np.int32).ravel()
+        ip = C.POINTER(C.c_int)
+        d = MeshDesc(coords.shape[1], coords.shape[0], coords.ctypes.data_as(C.POINTER(C.c_double)),
+                     c2n.shape[0], c2n.ctypes.data_as(ip))
+        f = MeshFacesDesc(f2c.shape[0], f2c.ctypes.data_as(ip), fn.ctypes.data_as(ip), b2c.size,
+                          b2c.ctypes.data_as(ip), bn.ctypes.data_as(ip))
+        h = C.c_void_p()
+        _check(lib.ins_mesh_create_op2(C.byref(d), C.byref(f), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_hdf5(cls, path):
+        """An OP2 HDF5 mesh file (DGMesh2D/3D(filename)), face and bface order kept."""
+        h = C.c_void_p()
+        _check(lib.ins_mesh_load_op2_hdf5(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
     def arrays(self):
         coords = np.empty((self.n_nodes, self.dim))
         c2n = np.empty((self.n_cells, self.dim + 1), dtype=np.int32)
@@ -332,6 +367,10 @@ class PoissonSolver:
         xyz = np.empty((self.dim, self.n_cells, self.Np))
         _check(lib.ins_poisson_node_coords(self._h, _ptr(xyz)))
         return xyz
+
+    def set_dat_stride(self, stride):
+        """Stride of the device SoA vectors passed afterwards (0 = self.stride)."""
+        _check(lib.ins_poisson_set_dat_stride(self._h, int(stride)))
 
     # --- PoissonMatrix / LinearSolver configuration ----------------------
     def set_bc_types(self, bc_types):

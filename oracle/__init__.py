@@ -25,6 +25,8 @@ def _load():
     vp, i64, d, ci = C.c_void_p, C.c_int64, C.c_double, C.c_int
     lib.oracle_create.argtypes = [ci, ci, i64, vp, i64, vp]
     lib.oracle_create.restype = vp
+    lib.oracle_create_faces.argtypes = [ci, ci, i64, vp, i64, vp, i64, vp, vp, i64, vp, vp]
+    lib.oracle_create_faces.restype = vp
     lib.oracle_destroy.argtypes = [vp]
     lib.oracle_destroy.restype = None
     for name in ("oracle_np", "oracle_nfp"):
@@ -108,13 +110,21 @@ def interp(dim, nf, nc):
 
 
 class Oracle:
-    def __init__(self, coords, cell2nodes, order):
+    def __init__(self, coords, cell2nodes, order, faces=None):
+        """faces: optional (face2cells, faceNum, bface2cells, bfaceNum) of the caller (an OP2
+        mesh file's face sets, periodic faces included) instead of the oracle's own search."""
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         c2n = np.ascontiguousarray(cell2nodes, dtype=np.int32)
         self.dim = coords.shape[1]
         self.order = order
-        self._h = lib.oracle_create(self.dim, order, coords.shape[0], _p(coords), c2n.shape[0],
-                                    _p(c2n))
+        if faces is None:
+            self._h = lib.oracle_create(self.dim, order, coords.shape[0], _p(coords), c2n.shape[0],
+                                        _p(c2n))
+        else:
+            f2c, fn, b2c, bn = (np.ascontiguousarray(a, dtype=np.int32) for a in faces)
+            self._h = lib.oracle_create_faces(self.dim, order, coords.shape[0], _p(coords),
+                                              c2n.shape[0], _p(c2n), f2c.shape[0], _p(f2c), _p(fn),
+                                              b2c.size, _p(b2c), _p(bn))
         if not self._h:
             raise RuntimeError("oracle_create failed")
         self.n_cells = c2n.shape[0]

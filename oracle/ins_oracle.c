@@ -457,7 +457,9 @@ static int cmp_pair(const void* a, const void* b) {
 static void vsub(const double* a, const double* b, double* o, int d) { for (int i = 0; i < d; ++i) o[i] = a[i] - b[i]; }
 
 /* For each interior face, the position in R's face mask of the node that coincides with
- * face node j of L (matched by physical coordinates, as the toolkit's fmaskR does). */
+ * face node j of L (matched by physical coordinates, as the toolkit's fmaskR does; relative
+ * to each side's face centroid, so a periodic face, whose sides differ by a translation,
+ * matches as well). */
 static int* face_map(const oracle_t* o, const refel* R) {
   const int dim = o->dim, nv = dim + 1, Nfp = R->Nfp;
   int* map = malloc(sizeof(int) * Nfp * (o->F + 1));
@@ -465,6 +467,14 @@ static int* face_map(const oracle_t* o, const refel* R) {
     const int L = o->f2c[2 * i], Rc = o->f2c[2 * i + 1], fL = o->fnum[2 * i], fR = o->fnum[2 * i + 1];
     const int* cL = o->c2n + (size_t)L * nv;
     const int* cR = o->c2n + (size_t)Rc * nv;
+    const int* fvL = dim == 2 ? FV2[fL] : FV3[fL];
+    const int* fvR = dim == 2 ? FV2[fR] : FV3[fR];
+    double ctrL[3] = {0, 0, 0}, ctrR[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k)
+      for (int x = 0; x < dim; ++x) {
+        ctrL[x] += o->coords[(size_t)cL[fvL[k]] * dim + x] / dim;
+        ctrR[x] += o->coords[(size_t)cR[fvR[k]] * dim + x] / dim;
+      }
     double scale = 0;
     for (int x = 0; x < dim; ++x) {
       const double d = o->coords[(size_t)cL[1] * dim + x] - o->coords[(size_t)cL[0] * dim + x];
@@ -483,7 +493,10 @@ static int* face_map(const oracle_t* o, const refel* R) {
         barycentric(dim, R->r[iR], R->s[iR], R->t[iR], lamR);
         for (int v = 0; v < nv; ++v) for (int x = 0; x < dim; ++x) xR[x] += lamR[v] * o->coords[(size_t)cR[v] * dim + x];
         double d2 = 0;
-        for (int x = 0; x < dim; ++x) d2 += (xL[x] - xR[x]) * (xL[x] - xR[x]);
+        for (int x = 0; x < dim; ++x) {
+          const double dx = (xL[x] - ctrL[x]) - (xR[x] - ctrR[x]);
+          d2 += dx * dx;
+        }
         if (sqrt(d2) < 1e-9 * scale) found = q;
       }
       if (found < 0) { free(map); return NULL; }
@@ -493,8 +506,30 @@ static int* face_map(const oracle_t* o, const refel* R) {
   return map;
 }
 
+static oracle_t* create(int dim, int order, int64_t n_nodes, const double* coords, int64_t n_cells,
+                        const int* cell2nodes, int64_t cF, const int* cf2c, const int* cfnum, int64_t cB,
+                        const int* cb2c, const int* cbnum);
+
 oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coords, int64_t n_cells,
                         const int* cell2nodes) {
+  return create(dim, order, n_nodes, coords, n_cells, cell2nodes, -1, NULL, NULL, -1, NULL, NULL);
+}
+
+/* The caller's face sets (the OP2 mesh file's face2cells/faceNum and bface2cells/bfaceNum,
+ * src/ins2d.cpp:192) in place of the oracle's own face search: faces may join cells whose
+ * face vertices differ by a translation (periodic faces). Cells must be positively oriented. */
+oracle_t* oracle_create_faces(int dim, int order, int64_t n_nodes, const double* coords, int64_t n_cells,
+                              const int* cell2nodes, int64_t n_faces, const int* face2cells,
+                              const int* faceNum, int64_t n_bfaces, const int* bface2cells,
+                              const int* bfaceNum) {
+  if (n_faces < 0 || n_bfaces < 0) return NULL;
+  return create(dim, order, n_nodes, coords, n_cells, cell2nodes, n_faces, face2cells, faceNum, n_bfaces,
+                bface2cells, bfaceNum);
+}
+
+static oracle_t* create(int dim, int order, int64_t n_nodes, const double* coords, int64_t n_cells,
+                        const int* cell2nodes, int64_t cF, const int* cf2c, const int* cfnum, int64_t cB,
+                        const int* cb2c, const int* cbnum) {
   if ((dim != 2 && dim != 3) || order < 1 || order > 4) return NULL;
   oracle_t* o = calloc(1, sizeof(oracle_t));
   o->dim = dim; o->N = order; o->nn = n_nodes; o->E = n_cells; o->pen_order = order;
@@ -508,7 +543,7 @@ oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coord
   /* orientation: positive measure (swap the last two vertices otherwise) */
   for (int64_t e = 0; e < E; ++e) {
     int* c = o->c2n + e * nv;
-    double a[3], b[3], cc[3], vol;
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, cc[3] = {0, 0, 0}, vol;
     vsub(o->coords + (size_t)c[1] * dim, o->coords + (size_t)c[0] * dim, a, dim);
     vsub(o->coords + (size_t)c[2] * dim, o->coords + (size_t)c[0] * dim, b, dim);
     if (dim == 2) vol = a[0] * b[1] - a[1] * b[0];
@@ -516,7 +551,10 @@ oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coord
       vsub(o->coords + (size_t)c[3] * dim, o->coords + (size_t)c[0] * dim, cc, dim);
       vol = a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) + a[2] * (b[0] * cc[1] - b[1] * cc[0]);
     }
-    if (vol < 0) { int t = c[dim - 1]; c[dim - 1] = c[dim]; c[dim] = t; }
+    if (vol < 0) {
+      if (cF >= 0) { oracle_destroy(o); return NULL; }  /* caller's faces need its vertex order */
+      int t = c[dim - 1]; c[dim - 1] = c[dim]; c[dim] = t;
+    }
   }
   /* geometry */
   const int ng = dim * dim + 1;
@@ -566,6 +604,25 @@ oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coord
     }
   }
   /* faces */
+  if (cF >= 0) {
+    if (2 * cF + cB != E * Nf) { oracle_destroy(o); return NULL; }
+    o->F = cF; o->B = cB;
+    o->f2c = malloc(sizeof(int) * 2 * (cF + 1)); o->fnum = malloc(sizeof(int) * 2 * (cF + 1));
+    o->b2c = malloc(sizeof(int) * (cB + 1)); o->bnum = malloc(sizeof(int) * (cB + 1)); o->bct = calloc(cB + 1, sizeof(int));
+    memcpy(o->f2c, cf2c, sizeof(int) * 2 * cF); memcpy(o->fnum, cfnum, sizeof(int) * 2 * cF);
+    memcpy(o->b2c, cb2c, sizeof(int) * cB); memcpy(o->bnum, cbnum, sizeof(int) * cB);
+    for (int64_t i = 0; i < 2 * cF; ++i)
+      if (o->f2c[i] < 0 || o->f2c[i] >= E || o->fnum[i] < 0 || o->fnum[i] >= Nf) { oracle_destroy(o); return NULL; }
+    o->cfb = malloc(sizeof(int) * E * Nf);
+    for (int64_t i = 0; i < E * Nf; ++i) o->cfb[i] = -1;
+    for (int64_t i = 0; i < cB; ++i) {
+      if (o->b2c[i] < 0 || o->b2c[i] >= E || o->bnum[i] < 0 || o->bnum[i] >= Nf) { oracle_destroy(o); return NULL; }
+      o->cfb[(int64_t)o->b2c[i] * Nf + o->bnum[i]] = (int)i;
+    }
+    o->mapLpos = face_map(o, &o->R);
+    if (!o->mapLpos) { oracle_destroy(o); return NULL; }
+    return o;
+  }
   facerec* recs = malloc(sizeof(facerec) * E * Nf);
   for (int64_t e = 0; e < E; ++e)
     for (int f = 0; f < Nf; ++f) {

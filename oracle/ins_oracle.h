@@ -26,6 +26,11 @@ typedef struct oracle oracle_t;
 
 oracle_t* oracle_create(int dim, int order, int64_t n_nodes, const double* coords,
                         int64_t n_cells, const int* cell2nodes);
+/* As oracle_create, with the caller's face sets (interior faces may be periodic pairs). */
+oracle_t* oracle_create_faces(int dim, int order, int64_t n_nodes, const double* coords,
+                              int64_t n_cells, const int* cell2nodes, int64_t n_faces,
+                              const int* face2cells, const int* faceNum, int64_t n_bfaces,
+                              const int* bface2cells, const int* bfaceNum);
 void oracle_destroy(oracle_t* o);
 int oracle_np(const oracle_t* o);
 int oracle_nfp(const oracle_t* o);

@@ -106,14 +106,16 @@ def simplex_measure(V):
     return math.sqrt(abs(np.linalg.det(E.T @ E))) / math.factorial(m)
 
 
-def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.0, sym_scale=1.0):
+def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.0, sym_scale=1.0,
+             periods=()):
     """Dense SIPG matrix (E*Np)^2 in the cells x Np node order of `xyz`.
 
     coords (nn, dim), c2n (E, dim+1), xyz (dim, E, Np) nodal points;
     bc_type_of(centroid) -> 0 Dirichlet / 1 Neumann (default Dirichlet);
     k (E, Np) nodal coefficient or None (k = 1); tau_scale and sym_scale multiply the
     penalty and the symmetry term (1 for the reference's form; other values let a test
-    show that a slip in either is detected)."""
+    show that a slip in either is detected); periods: translation vectors that join
+    boundary faces into periodic faces (a face and its translate are one interior face)."""
     dim = coords.shape[1]
     E, Np = xyz.shape[1], xyz.shape[2]
     nq = nq or N + 3
@@ -167,20 +169,47 @@ def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.
             n = -n
         return n
 
+    # face records: [(cell, face vertex coordinates), ...] with one or two sides
+    records = []
+    lone = []
     for key, cells in faces.items():
-        FV = coords[list(key)]
+        if len(cells) == 2:
+            records.append([(cells[0], coords[list(key)]), (cells[1], coords[list(key)])])
+        else:
+            lone.append((cells[0], coords[list(key)]))
+    if periods:
+        def ckey(FV):
+            return tuple(sorted(tuple(np.round(v, 9)) for v in FV))
+        index = {ckey(FV): i for i, (_, FV) in enumerate(lone)}
+        paired = set()
+        for i, (e, FV) in enumerate(lone):
+            if i in paired:
+                continue
+            for t in list(periods) + [-np.asarray(t) for t in periods]:
+                j = index.get(ckey(FV + np.asarray(t)))
+                if j is not None and j not in paired and j != i:
+                    records.append([(e, FV), lone[j]])
+                    paired.update((i, j))
+                    break
+        lone = [lone[i] for i in range(len(lone)) if i not in paired]
+    records += [[r] for r in lone]
+
+    for rec in records:
+        FV = rec[0][1]
+        cells = [c for c, _ in rec]
         area = simplex_measure(FV)
-        xq = FV[0] + qf @ (FV[1:] - FV[0])
-        w = wf * area * math.factorial(dim - 1)
         sJ = area / Fref
         fs = {e: sJ / (vol[e] / Kref) for e in cells}
-        fn = {e: face_nodes(e, FV) for e in cells}
+        fn = {e: face_nodes(e, F) for e, F in rec}
         kmax = {e: kk[e][fn[e]].max() for e in cells}
-        if len(cells) == 2:
+        if len(rec) == 2:
             tau = pen * max(fs[e] * kmax[e] for e in cells)
-            for e, o in (cells, cells[::-1]):
-                n = outward_normal(coords[c2n[e]], FV)
-                Le, Lo = bases[e].val(xq), bases[o].val(xq)
+            for (e, FVe), (o, FVo) in (rec, rec[::-1]):
+                t = FVo.mean(axis=0) - FVe.mean(axis=0)  # zero unless periodic
+                xq = FVe[0] + qf @ (FVe[1:] - FVe[0])
+                w = wf * area * math.factorial(dim - 1)
+                n = outward_normal(coords[c2n[e]], FVe)
+                Le, Lo = bases[e].val(xq), bases[o].val(xq + t)
                 ndL = np.tensordot(n, bases[e].grad(xq), axes=1)  # (Q, Np)
                 qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
                 qo = sum(n[d] * (Lo @ (kk[o][:, None] * G[o][d])) for d in range(dim))
@@ -189,11 +218,13 @@ def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.
                 # symmetric term: Pi_F(-1/2 k_e (u_e - u_o)) through e's face nodes
                 S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
                 Ie = np.eye(Np)[fn[e]]
-                Uo = bases[o].val(nodes[e][fn[e]])  # u_o at e's face nodes
+                Uo = bases[o].val(nodes[e][fn[e]] + t)  # u_o at e's face nodes
                 A[blk(e, e)] += -0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Ie))
                 A[blk(e, o)] += 0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Uo))
         else:
             e = cells[0]
+            xq = FV[0] + qf @ (FV[1:] - FV[0])
+            w = wf * area * math.factorial(dim - 1)
             centroid = FV.mean(axis=0)
             bt = 0 if bc_type_of is None else bc_type_of(centroid)
             if bt != 0:
