@@ -94,10 +94,14 @@ def parse():
     ap.add_argument("--apply-only", type=int, default=0, metavar="ORDER",
                     help="run only --steps FP64 operator applies of this order on the --n cube and "
                          "print their mean time (a profiling aid for counter runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batch leg and print its JSON (a profiling aid)")
+    ap.add_argument("--skip-oracle-check", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
-    ap.add_argument("--serve-rates", type=str, default="50,400,2000", help="req/s, comma list")
+    ap.add_argument("--serve-rates", type=str, default="10,25,50,100,200,500,1000,2000",
+                    help="req/s, comma list")
     ap.add_argument("--skip-serve", action="store_true")
     return ap.parse_args()
 
@@ -263,6 +267,7 @@ def mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, steps):
         wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1)
         out[prec] = {"gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
                      "iterations": [st["iterations"] for st in stats],
+                     "fp64_applies": [st["fine_applies"] for st in stats],
                      "rel_residual": max(st["rel_residual"] for st in stats),
                      "converged": all(st["converged"] for st in stats)}
     out["speedup_mixed_over_fp64"] = out["mixed"]["gdofs"] / out["fp64"]["gdofs"]
@@ -272,9 +277,11 @@ def mixed_vs_fp64_leg(ip, torch, s, b_d, x_d, dof, steps):
 
 def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
     """The cube with the AMG coarse solve (vertex space below order 1) instead of Jacobi-CG:
-    FP64 V-cycle at rel res 1e-10 and mixed at 1e-8; set-up time reported apart."""
+    FP64 and mixed at rel res 1e-8 (the like-for-like pair), FP64 at 1e-10 as well; set-up
+    time reported apart."""
     out = {}
-    for prec, rtol in (("fp64", 1e-10), ("mixed", 1e-8)):
+    for name, prec, rtol in (("fp64", "fp64", 1e-8), ("mixed", "mixed", 1e-8),
+                             ("fp64_rtol_1e-10", "fp64", 1e-10)):
         s.set_precision(ip.PRECISION_MIXED if prec == "mixed" else ip.PRECISION_FP64)
         s.set_coarse(ip.COARSE_AMG)
         s.set_tol_and_iter(rtol, 0.0, 500)
@@ -282,11 +289,13 @@ def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
         s.init()
         setup = time.perf_counter() - t0
         wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1)
-        out[prec] = {"rtol": rtol, "gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+        out[name] = {"rtol": rtol, "gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
                      "setup_s": setup, "iterations": [st["iterations"] for st in stats],
+                     "fp64_applies": [st["fine_applies"] for st in stats],
                      "rel_residual": max(st["rel_residual"] for st in stats),
                      "converged": all(st["converged"] for st in stats),
                      "amg_levels": s.amg_levels()}
+    out["speedup_mixed_over_fp64"] = out["mixed"]["gdofs"] / out["fp64"]["gdofs"]
     s.set_coarse(ip.COARSE_JACOBI_CG)
     return out
 
@@ -325,6 +334,7 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
                        f"(band 2h), pure Neumann, mean-zero, rel res 1e-8", "dof": dof}
     sols = {}
     for name, prec, coarse in (("plain_fp64", ip.PRECISION_FP64, ip.COARSE_JACOBI_CG),
+                               ("mixed_jacobi_cg", ip.PRECISION_MIXED, ip.COARSE_JACOBI_CG),
                                ("fp64_amg", ip.PRECISION_FP64, ip.COARSE_AMG),
                                ("mixed_amg", ip.PRECISION_MIXED, ip.COARSE_AMG)):
         log(f"precision form: {name}")
@@ -340,17 +350,21 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
         sols[name] = x
         out[name] = {"gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
                      "setup_s": setup, "iterations": [st["iterations"] for st in stats],
+                     "fp64_applies": [st["fine_applies"] for st in stats],
                      "rel_residual": max(st["rel_residual"] for st in stats),
                      "converged": all(st["converged"] for st in stats)}
     ref = sols["fp64_amg"]
     nref = float(np.sqrt(np.sum(ref * s.mass(ref))))
-    for name in ("plain_fp64", "mixed_amg"):
+    for name in ("plain_fp64", "mixed_jacobi_cg", "mixed_amg"):
         d = sols[name] - ref
         out[name]["rel_l2_vs_fp64_amg"] = float(np.sqrt(np.sum(d * s.mass(d)))) / nref
     out["fp32_apply"] = fp32_apply_kernels(ip, torch, s, order,
                                            peak_f32=peaks["mfma_f32_tflops"] if peaks else None)
-    out["speedup_mixed_over_plain_fp64"] = out["mixed_amg"]["gdofs"] / out["plain_fp64"]["gdofs"]
+    # like-for-like ratios: the same solver and tolerance, FP32 inner iterations vs FP64
+    out["speedup_mixed_over_fp64_jacobi_cg"] = out["mixed_jacobi_cg"]["gdofs"] / out["plain_fp64"]["gdofs"]
     out["speedup_mixed_over_fp64_amg"] = out["mixed_amg"]["gdofs"] / out["fp64_amg"]["gdofs"]
+    # the fastest mixed solver over the plain FP64 path as configured
+    out["speedup_mixed_amg_over_plain_fp64"] = out["mixed_amg"]["gdofs"] / out["plain_fp64"]["gdofs"]
     return out
 
 
@@ -445,8 +459,9 @@ def fp32_apply_kernels(ip, torch, s, order, reps=20, peak_f32=None):
     return res
 
 
-def batch_problems(ip, count, seed=20261204):
-    """2D batch set: log-uniform 500..50k triangles, N in {2,3,4}, sin(k pi x) sin(m pi y)."""
+def batch_problems(ip, count, seed=20261204, wavenumbers=None):
+    """2D batch set: log-uniform 500..50k triangles, N in {2,3,4}, sin(k pi x) sin(m pi y)
+    (the (k, m) of each problem are appended to `wavenumbers` when a list is given)."""
     rng = np.random.default_rng(seed)
     probs = []
     for i in range(count):
@@ -459,6 +474,8 @@ def batch_problems(ip, count, seed=20261204):
         xyz = mesh.node_coords(N)
         f = ((k * k + m * m) * np.pi ** 2 * np.sin(k * np.pi * xyz[0]) * np.sin(m * np.pi * xyz[1]))
         probs.append(dict(mesh=mesh, order=N, rhs=f, rhs_nodal=True, rtol=1e-8, max_iter=500))
+        if wavenumbers is not None:
+            wavenumbers.append((k, m))
     return probs
 
 
@@ -484,38 +501,42 @@ def batch_leg(ip, count):
         plain_t += time.perf_counter() - t0
         plain_its.append(s.last_stats["iterations"])
         del s
-    # batched, mixed precision: one warm-up batch of a few problems, then the whole set
-    warm = ip.Batch(2, precision=ip.PRECISION_MIXED)
-    warm.add(probs[:4])
-    warm.run()
-    del warm
-    bt = ip.Batch(2, precision=ip.PRECISION_MIXED)
-    t0 = time.perf_counter()
-    ids = bt.add(probs)
-    t1 = time.perf_counter()
-    bt.run()
-    t2 = time.perf_counter()
-    res = [bt.result(i) for i in ids]
-    setup_parts = bt.setup_seconds()
-    tile, tile_ms = bt.tuning()
-    its = [r[2]["iterations"] for r in res]
-    ok = all(r[0] == ip.OK for r in res)
-    worst = max(r[2]["rel_residual"] for r in res)
-    st = bt.stats()
-    return {
-        "problems": count, "dof": dofs,
-        "plain_fp64": {"gdofs": dofs / plain_t * 1e-9, "seconds": plain_t,
-                       "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
-                       "converged": bool(plain_ok)},
-        "batched_mixed": {"gdofs": dofs / (t2 - t1) * 1e-9, "seconds": t2 - t1,
-                          "setup_seconds": t1 - t0, "setup_parts_s": setup_parts,
-                          "tile": tile, "tile_ms": tile_ms, "solves_per_s": count / (t2 - t1),
-                          "gdofs_incl_setup": dofs / (t2 - t0) * 1e-9,
-                          "lockstep_iterations": st["iterations_run"],
-                          "mean_iterations": float(np.mean(its)), "max_rel_residual": worst,
-                          "converged": bool(ok)},
-        "speedup_batched_mixed_over_plain_fp64": plain_t / (t2 - t1),
-    }
+    out = {"problems": count, "dof": dofs,
+           "plain_fp64": {"gdofs": dofs / plain_t * 1e-9, "seconds": plain_t,
+                          "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
+                          "converged": bool(plain_ok)}}
+    # the batch engine in each precision: one warm-up batch of a few problems, then the whole set
+    for name, prec in (("batched_mixed", ip.PRECISION_MIXED), ("batched_fp64", ip.PRECISION_FP64)):
+        log(f"batch engine, {name}")
+        warm = ip.Batch(2, precision=prec)
+        warm.add(probs[:4])
+        warm.run()
+        del warm
+        bt = ip.Batch(2, precision=prec)
+        t0 = time.perf_counter()
+        ids = bt.add(probs)
+        t1 = time.perf_counter()
+        bt.run()
+        t2 = time.perf_counter()
+        res = [bt.result(i) for i in ids]
+        setup_parts = bt.setup_seconds()
+        tile, tile_ms = bt.tuning()
+        its = [r[2]["iterations"] for r in res]
+        ok = all(r[0] == ip.OK for r in res)
+        worst = max(r[2]["rel_residual"] for r in res)
+        st = bt.stats()
+        out[name] = {"gdofs": dofs / (t2 - t1) * 1e-9, "seconds": t2 - t1,
+                     "setup_seconds": t1 - t0, "setup_parts_s": setup_parts,
+                     "tile": tile, "tile_ms": tile_ms, "solves_per_s": count / (t2 - t1),
+                     "gdofs_incl_setup": dofs / (t2 - t0) * 1e-9,
+                     "lockstep_iterations": st["iterations_run"],
+                     "mean_iterations": float(np.mean(its)), "max_rel_residual": worst,
+                     "converged": bool(ok)}
+        del bt
+    out["speedup_batched_mixed_over_batched_fp64"] = (out["batched_mixed"]["gdofs"] /
+                                                      out["batched_fp64"]["gdofs"])
+    out["speedup_batched_mixed_over_plain_fp64"] = out["batched_mixed"]["gdofs"] / out["plain_fp64"]["gdofs"]
+    return out
 
 
 def density(xyz, h, contrast):
@@ -603,6 +624,128 @@ def serve_leg(ip, count, rates, budget_gib=64):
     return out
 
 
+def oracle_check_leg(ip, torch):
+    """Checker, run after every timed region: the bench's own solvers against the FP64 oracle
+    on inputs the oracle solves in seconds (same settings, smaller meshes). Relative L2 in the
+    mass norm; for manufactured solutions also both discretisation errors."""
+    import oracle as O
+
+    def mnorm(o, v):
+        return float(np.sqrt(np.sum(v * o.mass(v))))
+
+    def compare(o, x, xo, u=None):
+        r = {"rel_l2_vs_oracle": mnorm(o, x - xo) / mnorm(o, xo)}
+        if u is not None:
+            eg, eo = mnorm(o, x - u), mnorm(o, xo - u)
+            r["discretisation_error_gpu"] = eg
+            r["discretisation_error_oracle"] = eo
+            r["discretisation_error_match_2_digits"] = bool(abs(eg - eo) <= 0.005 * eo)
+        return r
+
+    out = {}
+    # configuration 1 at 12^3 x 6: FP64 to 1e-10 (the headline solver) and mixed to 1e-8
+    n = 12
+    mesh = cube(ip, n, 20261202)
+    s = ip.PoissonSolver(mesh, 3)
+    s.set_bc_types(np.zeros(s.n_bfaces, dtype=np.int32))
+    s.set_pmg([3, 2, 1], pre_it=[1] * 3, post_it=[1] * 3, cheb_orders=[3] * 3, coarse_iters=150)
+    o = O.Oracle(*mesh.arrays(), 3)
+    o.set_bc_types(np.zeros(o.n_bfaces, dtype=np.int32))
+    u, f = manufactured_sin(o.node_coords())
+    b = o.mass(f)
+    xo, rc, _, _, _ = o.solve_pmg(b, [3, 2, 1], coarse_iters=20, rtol=1e-12, max_iter=500)
+    c1 = {"workload": f"cube {n}^3x6, N=3, Dirichlet, p-MG 3->2->1 Chebyshev(3) + Jacobi-CG(150)",
+          "oracle_converged": rc == 0}
+    for name, prec, rtol in (("fp64_rtol_1e-10", ip.PRECISION_FP64, 1e-10),
+                             ("mixed_rtol_1e-8", ip.PRECISION_MIXED, 1e-8)):
+        s.set_precision(prec)
+        s.set_tol_and_iter(rtol, 0.0, 500)
+        x = np.zeros_like(b)
+        s.solve(b, x)
+        c1[name] = compare(o, x, xo, u)
+        c1[name]["iterations"] = s.last_stats["iterations"]
+    out["config1_cube12"] = c1
+    del s, o
+    # configuration 2 at 8^3 x 6: N=4, rho 1000, pure Neumann; FP64 and mixed with the AMG coarse
+    n, N = 8, 4
+    mesh = cube(ip, n, 20261206)
+    s = ip.PoissonSolver(mesh, N)
+    s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
+    s.set_nullspace(True)
+    o = O.Oracle(*mesh.arrays(), N)
+    o.set_bc_types(np.ones(o.n_bfaces, dtype=np.int32))
+    k = 1.0 / density(o.node_coords(), 1.0 / n, 1000.0)
+    s.set_factor(k)
+    o.set_factor(k)
+    s.set_pmg([4, 3, 2, 1], pre_it=[1] * 4, post_it=[1] * 4, cheb_orders=[3] * 4, coarse_iters=150)
+    s.set_coarse(ip.COARSE_AMG)
+    f = np.random.default_rng(20261206).standard_normal((s.n_cells, s.Np))
+    b = o.mass(f)
+    b -= b.mean()
+    xo, rc, _, _, _ = o.solve_pmg(b, [4, 3, 2, 1], coarse_iters=30, rtol=1e-12, max_iter=3000,
+                                  nullspace=True)
+    c2 = {"workload": f"cube {n}^3x6, N=4, rho 1000 sphere, pure Neumann, p-MG 4->1 + AMG coarse, "
+                      "rel res 1e-8", "oracle_converged": rc == 0}
+    for name, prec in (("fp64_amg", ip.PRECISION_FP64), ("mixed_amg", ip.PRECISION_MIXED)):
+        s.set_precision(prec)
+        s.set_tol_and_iter(1e-8, 0.0, 2000)
+        x = np.zeros_like(b)
+        s.solve(b, x)
+        c2[name] = compare(o, x, xo)
+        c2[name]["iterations"] = s.last_stats["iterations"]
+    out["config2_cube8"] = c2
+    del s, o
+    # batch: the four smallest problems of the batch set, batched in each precision
+    kms = []
+    allp = batch_problems(ip, 256, wavenumbers=kms)
+    pick = sorted(range(len(allp)), key=lambda i: allp[i]["mesh"].n_cells)[:4]
+    probs = [allp[i] for i in pick]
+    cb = {"workload": "the 4 smallest problems of the batch set (2D, Dirichlet, manufactured), rel res 1e-8"}
+    refs = []
+    for i, p in zip(pick, probs):
+        o = O.Oracle(*p["mesh"].arrays(), p["order"])
+        o.set_bc_types(np.zeros(o.n_bfaces, dtype=np.int32))
+        xo, rc, _, _ = o.solve_jacobi(o.mass(p["rhs"]), rtol=1e-12, max_iter=100000)
+        xyz = o.node_coords()
+        k_, m_ = kms[i]
+        refs.append((o, xo, np.sin(k_ * np.pi * xyz[0]) * np.sin(m_ * np.pi * xyz[1])))
+    for name, prec in (("batched_mixed", ip.PRECISION_MIXED), ("batched_fp64", ip.PRECISION_FP64)):
+        bt = ip.Batch(2, precision=prec)
+        ids = bt.add(probs)
+        bt.run()
+        res = []
+        for i, (o, xo, u) in zip(ids, refs):
+            _, x, _ = bt.result(i)
+            res.append(compare(o, x, xo, u))
+        cb[name] = {"max_rel_l2_vs_oracle": float(max(r["rel_l2_vs_oracle"] for r in res)),
+                    "discretisation_errors_match_2_digits": all(r["discretisation_error_match_2_digits"]
+                                                                for r in res),
+                    "discretisation_error_gpu": [r["discretisation_error_gpu"] for r in res],
+                    "discretisation_error_oracle": [r["discretisation_error_oracle"] for r in res]}
+        del bt
+    out["batch"] = cb
+    # serving: four small requests through the server (pure Neumann, rho contrast, mean-zero)
+    reqs = sorted(serve_requests(ip, 40), key=lambda r: r["mesh"].n_cells)[:4]
+    srv = ip.Server(8 << 30)
+    rids = [srv.submit(**r) for r in reqs]
+    srv.wait_all()
+    errs = []
+    for rid, r in zip(rids, reqs):
+        _, x, _, _, _ = srv.wait(rid)
+        o = O.Oracle(*r["mesh"].arrays(), r["order"])
+        o.set_bc_types(r["bc_types"])
+        if r["factor"] is not None:
+            o.set_factor(r["factor"])
+        b = o.mass(r["rhs"])
+        b -= b.mean()
+        xo, rc, _, _, _ = o.solve_pmg(b, list(range(r["order"], 0, -1)), coarse_iters=30, rtol=1e-12,
+                                      max_iter=5000, nullspace=True)
+        errs.append(mnorm(o, x - xo) / mnorm(o, xo))
+    srv.close()
+    out["serving"] = {"workload": "the 4 smallest of 40 serving requests", "max_rel_l2_vs_oracle": float(max(errs))}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -616,6 +759,9 @@ def main():
     if a.precision_only:
         res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps, peaks=peaks)
         print(json.dumps({"precision_form": res}), flush=True)
+        return
+    if a.batch_only:
+        print(json.dumps({"batch": batch_leg(ip, a.batch_problems)}), flush=True)
         return
     if a.apply_only:
         print(json.dumps({"apply_only": apply_leg(ip, torch, a.n, a.apply_only, a.steps)}), flush=True)
@@ -770,6 +916,9 @@ def main():
         out["config0"] = config0_leg(ip)
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.order, a.rtol, a.coarse_iters)
+    if not a.skip_oracle_check:
+        log("oracle check (every form at oracle-feasible sizes)")
+        out["oracle_check"] = oracle_check_leg(ip, torch)
     print(json.dumps(out), flush=True)
 
 

@@ -328,6 +328,9 @@ typedef struct {
   int tile;               /* cells per tile and block: 64, 128, 256, or 0 (default) = autotuned: the
                              first batch of the process times each size on its problems and the
                              fastest is kept for the dimension and precision */
+  int chunk;              /* lockstep iterations per graph replay (1..64, default 8): problems that
+                             converge inside a replay are frozen on the device, and the host reads
+                             the problems' state once per replay */
 } ins_batch_config;
 
 typedef struct {
@@ -342,7 +345,8 @@ typedef struct {
   int rhs_nodal;          /* 0: rhs is mass-weighted (M f); 1: rhs is nodal f, b = J M f */
 } ins_problem;
 
-/* Defaults: MIXED, Chebyshev 3, 1 pre / 1 post, 1.1, 0.1, 15, 40, graphs on, AMG coarse. */
+/* Defaults: MIXED, Chebyshev 3, 1 pre / 1 post, 1.1, 0.1, 15, 40, graphs on, AMG coarse,
+ * autotuned tile, 8 iterations per replay. */
 void ins_batch_config_default(ins_batch_config* cfg);
 /* Device bytes one problem holds while resident (admission control); n_nodes > 0 adds the
  * estimate for the AMG coarse solve of a mesh with that many vertices. */
@@ -354,8 +358,8 @@ void ins_batch_destroy(ins_batch_t b);
 /* Uploads n problems, sets them up (penalties, diagonals, Chebyshev bounds, in
  * one batched power iteration) and starts them; ids[i] names problem i. */
 int ins_batch_add(ins_batch_t b, const ins_problem* probs, int n, int* ids);
-/* Runs up to max_iters lockstep iterations, stopping early when none is running;
- * *running = problems still running afterwards. */
+/* Runs up to max_iters lockstep iterations (in replays of `chunk` iterations), stopping early
+ * when none is running; *running = problems still running afterwards. */
 int ins_batch_iterate(ins_batch_t b, int max_iters, int* running);
 /* Ids finished since the last call, at most cap per call; *n = count returned. */
 int ins_batch_take_finished(ins_batch_t b, int* ids, int cap, int* n);

@@ -123,6 +123,8 @@ BatchEngine::BatchEngine(const BatchConfig& cfg, hipStream_t stream) : cfg_(cfg)
   INSP_HIP(hipMemset(d_scal_, 0, sizeof(double) * BS_COUNT * max_slots_));
   INSP_HIP(hipDeviceSynchronize());
   INSP_HIP(hipHostMalloc((void**)&h_scal_, sizeof(double) * BS_COUNT * max_slots_, hipHostMallocDefault));
+  INSP_HIP(hipHostMalloc((void**)&h_stage_, sizeof(double) * 5 * max_slots_, hipHostMallocDefault));
+  INSP_HIP(hipMalloc(&d_stage_, sizeof(double) * 5 * max_slots_));
   probs_.resize(max_slots_);
   for (int i = max_slots_ - 1; i >= 0; --i) free_slots_.push_back(i);
   run_ = std::make_unique<View>();
@@ -132,7 +134,9 @@ BatchEngine::BatchEngine(const BatchConfig& cfg, hipStream_t stream) : cfg_(cfg)
 
 BatchEngine::~BatchEngine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  if (h_stage_) (void)hipHostFree(h_stage_);
+  if (d_stage_) (void)hipFree(d_stage_);
   for (auto& p : probs_) {
     if (p && p->arena) (void)hipFree(p->arena);
     if (p && p->amg_arena) (void)hipFree(p->amg_arena);
@@ -290,6 +294,30 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
   v.bv.ltile_cap = v.ltile_cap;
 }
 
+void BatchEngine::drop_graphs() {
+  for (auto& kv : graphs_) INSP_HIP(hipGraphExecDestroy(kv.second));
+  graphs_.clear();
+}
+
+// Resets the convergence mask of these problems to RUNNING; with limits also their
+// iteration count, tolerance and iteration limit (one copy and one scatter kernel).
+void BatchEngine::write_state(const std::vector<int>& slots, bool with_limits) {
+  if (slots.empty()) return;
+  INSP_HIP(hipStreamSynchronize(stream_));  // the staging buffer may still feed an earlier copy
+  const int n = (int)slots.size();
+  for (int j = 0; j < n; ++j) {
+    const Prob& P = *probs_[slots[j]];
+    double* r = h_stage_ + (size_t)5 * j;
+    r[0] = slots[j];
+    r[1] = BST_RUNNING;
+    r[2] = P.iters;
+    r[3] = P.tol;
+    r[4] = P.max_iter;
+  }
+  INSP_HIP(hipMemcpyAsync(d_stage_, h_stage_, sizeof(double) * 5 * n, hipMemcpyHostToDevice, stream_));
+  batch::set_state(d_scal_, n, d_stage_, with_limits, stream_);
+}
+
 void BatchEngine::read_scal(const std::vector<int>& slots) {
   int hi = 0;
   for (int s : slots) hi = std::max(hi, s + 1);
@@ -385,6 +413,7 @@ void BatchEngine::run_iteration(View& v) {
   batch::finalize(b, 1, BS_PAP, s);
   batch::fcg_update_xr(b, s);
   batch::finalize(b, 2, BS_RR, s);  // |r|^2 and r.z_old
+  batch::mask(b, s);                 // converged / broken-down problems are frozen from here
   precondition(v);
   batch::dot<double>(b, 0, BV_R, BV_Z, BV_R, BV_Z, 1, s);
   batch::finalize(b, 1, BS_RZN, s);
@@ -916,6 +945,7 @@ void BatchEngine::start(View& v, const std::vector<int>& slots, bool restart) {
   set_view(v, slots);
   const BatchView& b = v.bv;
   hipStream_t s = stream_;
+  write_state(slots, false);  // unfreeze: every kernel below works on these problems
   if (!restart) {
     batch::sum_d(b, BV_B, s);
     batch::finalize(b, 1, BS_MEAN, s);
@@ -952,6 +982,7 @@ void BatchEngine::start(View& v, const std::vector<int>& slots, bool restart) {
   }
   if (go.empty()) return;
   set_view(v, go);
+  write_state(go, true);  // tolerance, iteration count and limit of the device mask
   precondition(v);
   batch::dot<double>(v.bv, 0, BV_R, BV_Z, BV_R, BV_Z, 1, s);
   batch::finalize(v.bv, 1, BS_RZ, s);
@@ -982,8 +1013,8 @@ void BatchEngine::finish(int slot, ProbStatus st) {
   finished_.push_back(slot);
 }
 
-void BatchEngine::iterate() {
-  if (running_slots_.empty()) return;
+int BatchEngine::iterate(int k) {
+  if (running_slots_.empty() || k < 1) return 0;
   const bool refresh = graph_tile_cap_ == -2 || !run_->tiles.p;
   if (refresh) {
     const int old_t = run_->tile_cap, old_p = run_->prob_cap, old_l = run_->ltile_cap,
@@ -991,43 +1022,45 @@ void BatchEngine::iterate() {
     int old_o[5];
     std::copy(run_->ocap, run_->ocap + 5, old_o);
     set_view(*run_, running_slots_);
-    // the graph bakes in the grid sizes and the number of AMG levels it walks
+    // the graphs bake in the grid sizes and the number of AMG levels they walk
     if (run_->tile_cap != old_t || run_->prob_cap != old_p || run_->ltile_cap != old_l ||
-        run_->levels != old_lev || !std::equal(old_o, old_o + 5, run_->ocap) || !graph_) {
-      if (graph_) {
-        INSP_HIP(hipGraphExecDestroy(graph_));
-        graph_ = nullptr;
-      }
-    }
+        run_->levels != old_lev || !std::equal(old_o, old_o + 5, run_->ocap))
+      drop_graphs();
     graph_tile_cap_ = run_->tile_cap;
     graph_prob_cap_ = run_->prob_cap;
   }
   if (cfg_.use_graph) {
-    if (!graph_) {
+    auto it = graphs_.find(k);
+    if (it == graphs_.end()) {
       hipGraph_t g;
+      hipGraphExec_t exec;
       INSP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      run_iteration(*run_);
+      for (int i = 0; i < k; ++i) run_iteration(*run_);
       INSP_HIP(hipStreamEndCapture(stream_, &g));
-      INSP_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+      INSP_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
       INSP_HIP(hipGraphDestroy(g));
+      it = graphs_.emplace(k, exec).first;
     }
-    INSP_HIP(hipGraphLaunch(graph_, stream_));
+    INSP_HIP(hipGraphLaunch(it->second, stream_));
   } else {
-    run_iteration(*run_);
+    for (int i = 0; i < k; ++i) run_iteration(*run_);
   }
-  ++iters_run_;
-  read_scal(running_slots_);
+  iters_run_ += k;
+  read_scal(running_slots_);  // the only host round trip of the k iterations
   std::vector<int> keep, verify;
   for (int slot : running_slots_) {
     Prob& P = *probs_[slot];
-    ++P.iters;
-    ++P.result.fine_applies;
-    ++P.result.vcycles;
     const double* sc = h_scal_ + (size_t)slot * BS_COUNT;
-    const double pap = sc[BS_PAP], rr = sc[BS_RR];
-    if (!(pap > 0) || !std::isfinite(rr) || !std::isfinite(pap)) {
+    const int its = (int)sc[BS_ITER];
+    const int state = (int)sc[BS_STATE];
+    const int ran = its - P.iters;
+    P.iters = its;
+    P.result.fine_applies += ran;
+    // the iteration that froze a problem stops before its preconditioner
+    P.result.vcycles += ran - (state != BST_RUNNING ? 1 : 0);
+    if (state == BST_BREAKDOWN) {
       finish(slot, ProbStatus::BREAKDOWN);
-    } else if (std::sqrt(rr) <= P.tol || P.iters >= P.max_iter) {
+    } else if (state == BST_TOL || state == BST_MAXIT) {
       verify.push_back(slot);
     } else {
       keep.push_back(slot);
@@ -1038,6 +1071,7 @@ void BatchEngine::iterate() {
     graph_tile_cap_ = -2;
   }
   if (!verify.empty()) start(*work_, verify, true);
+  return k;
 }
 
 int BatchEngine::running() const { return (int)running_slots_.size(); }

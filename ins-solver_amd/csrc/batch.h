@@ -17,6 +17,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -41,8 +42,12 @@ enum BatchScal : int {
   BS_RZ = 0, BS_PAP, BS_RR, BS_RZO, BS_RZN, BS_BB, BS_MEAN,
   BS_CA, BS_CPAP, BS_CB,
   BS_TOP, BS_BOT, BS_NRM,
-  BS_COUNT = 16
+  // device-side convergence mask: a problem whose state is not RUNNING is frozen (every
+  // batched kernel skips it) until the host restarts or retires it
+  BS_STATE, BS_ITER, BS_TOL, BS_MAXIT,
+  BS_COUNT = 20
 };
+enum BatchState : int { BST_RUNNING = 0, BST_TOL = 1, BST_BREAKDOWN = 2, BST_MAXIT = 3 };
 
 constexpr int kAmgLevels = 8;   // deepest AMG hierarchy of one problem in a batch
 constexpr int kAmgMaxCoarse = 256;
@@ -129,6 +134,12 @@ template <typename T>
 void apply(const BatchView& b, const BatchTables& t, int level, int in, int tr, int out, bool dot,
            hipStream_t s);
 void finalize(const BatchView& b, int nvals, int slot, hipStream_t s);
+// After an iteration's |r|^2: count the iteration and freeze problems that reached their
+// tolerance or iteration limit or broke down (p.Ap <= 0, non-finite values).
+void mask(const BatchView& b, hipStream_t s);
+// rec holds n records (slot, state, iterations, tolerance, iteration limit): writes the state,
+// and with `limits` the other three, into each slot's scalars.
+void set_state(double* scal, int n, const double* rec, bool limits, hipStream_t s);
 // value k of the per-tile partials: sum of a_k*b_k over the tile (level rows).
 template <typename T>
 void dot(const BatchView& b, int level, int a, int bb, int c, int d, int nvals, hipStream_t s);
@@ -187,7 +198,8 @@ struct BatchConfig {
   int coarse_iters = 40;     // Jacobi-CG iterations on order 1
   bool amg = true;           // vertex-space AMG below order 1 instead of Jacobi-CG
   int amg_cheb = 2;          // Chebyshev degree on the AMG levels
-  bool use_graph = true;     // replay one captured iteration
+  bool use_graph = true;     // replay captured iterations
+  int chunk = 8;             // lockstep iterations per replay between host reads
   int tile = 0;              // cells per tile: 64, 128, 256, or 0 = autotuned
 };
 
@@ -229,8 +241,9 @@ class BatchEngine {
   // Uploads, sets up (penalty, diagonals, Chebyshev bounds) and starts a problem.
   // Returns its id. Set-up of several problems can be batched with add_many.
   std::vector<int> add_many(std::vector<ProblemSpec>&& specs);
-  // One lockstep flexible-CG iteration of every running problem.
-  void iterate();
+  // k lockstep flexible-CG iterations of every running problem without a host round trip
+  // (problems that converge on the way are frozen on the device); returns k.
+  int iterate(int k = 1);
   int running() const;
   int free_slots() const { return (int)free_slots_.size(); }
   const BatchConfig& config() const { return cfg_; }
@@ -293,8 +306,12 @@ class BatchEngine {
   std::vector<int> running_slots_, finished_;
   size_t bytes_in_use_ = 0;
   int64_t iters_run_ = 0;
-  hipGraphExec_t graph_ = nullptr;
+  std::map<int, hipGraphExec_t> graphs_;        // captured k-iteration replays, by k
   int graph_tile_cap_ = -1, graph_prob_cap_ = -1;
+  double* h_stage_ = nullptr;                  // pinned staging of per-problem scalar writes
+  double* d_stage_ = nullptr;
+  void write_state(const std::vector<int>& slots, bool with_limits);
+  void drop_graphs();
 };
 
 }  // namespace insp

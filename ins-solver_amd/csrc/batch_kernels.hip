@@ -45,6 +45,9 @@ __device__ __forceinline__ X* uni(X* p) {
   return reinterpret_cast<X*>(uni((int64_t)reinterpret_cast<uintptr_t>(p)));
 }
 
+// A problem frozen by the device-side convergence mask takes no further work.
+__device__ __forceinline__ bool frozen(const BatchProb& P) { return P.scal[BS_STATE] != 0.0; }
+
 template <int TB>
 __device__ __forceinline__ Cell locate(const BatchView& b) {
   Cell c;
@@ -57,7 +60,7 @@ __device__ __forceinline__ Cell locate(const BatchView& b) {
     const int2 tt = b.tiles[c.t];
     c.slot = uni(tt.x);
     c.e = uni(tt.y) + threadIdx.x;
-    c.valid = c.e < uni(b.P[c.slot].E);
+    c.valid = c.e < uni(b.P[c.slot].E) && !frozen(b.P[c.slot]);
   }
   return c;
 }
@@ -92,7 +95,7 @@ __device__ __forceinline__ Cell locate_order(const BatchView& b) {
     const int2 tt = b.tiles[c.t];
     c.slot = uni(tt.x);
     c.e = uni(tt.y) + threadIdx.x;
-    c.valid = c.e < uni(b.P[c.slot].E);
+    c.valid = c.e < uni(b.P[c.slot].E) && !frozen(b.P[c.slot]);
   }
   return c;
 }
@@ -152,6 +155,7 @@ __global__ void __launch_bounds__(kB) kb_finalize(BatchView b, int nvals, int sl
   const int j = blockIdx.x;
   if (j >= *b.nprobs) return;
   const int slot = b.probs[j];
+  if (frozen(b.P[slot])) return;  // keeps the values it froze with
   const int2 r = b.ranges[slot];
   for (int k = 0; k < nvals; ++k) {
     double v = 0.0;
@@ -225,7 +229,38 @@ __global__ void kb_scal_copy(BatchView b, int dst, int src) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= *b.nprobs) return;
   double* s = b.P[b.probs[j]].scal;
+  if (s[BS_STATE] != 0.0) return;
   s[dst] = s[src];
+}
+
+__global__ void kb_set_state(double* scal, int n, const double* rec, int limits) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double* r = rec + 5 * (int64_t)j;
+  double* s = scal + (int64_t)r[0] * BS_COUNT;
+  s[BS_STATE] = r[1];
+  if (limits) {
+    s[BS_ITER] = r[2];
+    s[BS_TOL] = r[3];
+    s[BS_MAXIT] = r[4];
+  }
+}
+
+// One thread per problem: count the iteration, then freeze the problem when it broke down,
+// reached its tolerance (recursive residual) or its iteration limit.
+__global__ void kb_mask(BatchView b) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *b.nprobs) return;
+  double* s = b.P[b.probs[j]].scal;
+  if (s[BS_STATE] != 0.0) return;
+  const double it = s[BS_ITER] + 1.0;
+  s[BS_ITER] = it;
+  const double pap = s[BS_PAP], rr = s[BS_RR];
+  double st = BST_RUNNING;
+  if (!(pap > 0.0) || !isfinite(rr) || !isfinite(pap)) st = BST_BREAKDOWN;
+  else if (sqrt(rr) <= s[BS_TOL]) st = BST_TOL;
+  else if (it >= s[BS_MAXIT]) st = BST_MAXIT;
+  s[BS_STATE] = st;
 }
 
 template <int TB>
@@ -308,7 +343,7 @@ __device__ __forceinline__ Row locate_row(const BatchView& b, int k) {
     const int2 tt = b.ltiles[(int64_t)k * b.ltile_cap + t];
     c.slot = tt.x;
     c.r = tt.y + threadIdx.x;
-    c.valid = c.r < b.P[c.slot].amg.lev[k].n;
+    c.valid = c.r < b.P[c.slot].amg.lev[k].n && !frozen(b.P[c.slot]);
   }
   return c;
 }
@@ -403,7 +438,7 @@ __global__ void __launch_bounds__(kB) kb_amg_dense(BatchView b, int k) {
   const int j = blockIdx.x;
   if (j >= *b.nprobs) return;
   const BatchProb& P = b.P[b.probs[j]];
-  if (P.amg.nlev == 0 || P.amg.nlev - 1 != k) return;
+  if (P.amg.nlev == 0 || P.amg.nlev - 1 != k || frozen(P)) return;
   const int n = P.amg.nc, i = threadIdx.x;
   if (i >= n) return;
   const T* M = (const T*)P.amg.cinv;
@@ -642,6 +677,17 @@ void apply(const BatchView& b, const BatchTables& t, int level, int in, int tr, 
 #undef INSP_APPLY
   }
   check("kb_apply");
+}
+
+void set_state(double* scal, int n, const double* rec, bool limits, hipStream_t s) {
+  hipLaunchKernelGGL(kb_set_state, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, scal, n, rec,
+                     (int)limits);
+  check("kb_set_state");
+}
+
+void mask(const BatchView& b, hipStream_t s) {
+  hipLaunchKernelGGL(kb_mask, dim3((unsigned)((b.prob_cap + 63) / 64)), dim3(64), 0, s, b);
+  check("kb_mask");
 }
 
 void finalize(const BatchView& b, int nvals, int slot, hipStream_t s) {

@@ -95,6 +95,8 @@ insp::BatchConfig to_batch_config(int dim, const ins_batch_config* c) {
   if (c->tile != 0 && c->tile != 64 && c->tile != 128 && c->tile != 256)
     throw insp::InvalidArgument("tile must be 64, 128, 256 or 0");
   b.tile = c->tile;
+  if (c->chunk < 0 || c->chunk > 64) throw insp::InvalidArgument("chunk must be in 1..64 (0 = default)");
+  if (c->chunk > 0) b.chunk = c->chunk;
   return b;
 }
 
@@ -785,6 +787,7 @@ void ins_batch_config_default(ins_batch_config* c) {
   c->use_graph = d.use_graph ? 1 : 0;
   c->coarse = d.amg ? INS_COARSE_AMG : INS_COARSE_JACOBI_CG;
   c->tile = d.tile;
+  c->chunk = d.chunk;
 }
 
 int ins_batch_setup_seconds(ins_batch_t b, double* t5) {
@@ -863,7 +866,9 @@ int ins_batch_iterate(ins_batch_t b, int max_iters, int* running) {
   return guarded([&] {
     auto& e = B(b);
     if (max_iters < 0) throw insp::InvalidArgument("negative iteration count");
-    for (int i = 0; i < max_iters && e.running() > 0; ++i) e.iterate();
+    // chunks of lockstep iterations, one host round trip each
+    for (int done = 0; done < max_iters && e.running() > 0;)
+      done += e.iterate(std::min(std::max(1, e.config().chunk), max_iters - done));
     if (running) *running = e.running();
     return INS_OK;
   });

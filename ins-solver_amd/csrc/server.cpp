@@ -126,7 +126,7 @@ void Server::loop() {
           active_[d][ids[k]] = a;
         }
       }
-      for (int d = 0; d < 2; ++d) eng_[d]->iterate();
+      for (int d = 0; d < 2; ++d) eng_[d]->iterate(std::max(1, eng_[d]->config().chunk));
       for (int d = 0; d < 2; ++d) {
         for (int pid : eng_[d]->take_finished()) {
           auto it = active_[d].find(pid);

@@ -100,7 +100,7 @@ class BatchConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("cheb_order", C.c_int), ("pre_it", C.c_int),
                 ("post_it", C.c_int), ("eig_safety", C.c_double), ("cheb_min_factor", C.c_double),
                 ("max_iter_eig", C.c_int), ("coarse_iters", C.c_int), ("use_graph", C.c_int),
-                ("coarse", C.c_int), ("tile", C.c_int)]
+                ("coarse", C.c_int), ("tile", C.c_int), ("chunk", C.c_int)]
 
 
 class Problem(C.Structure):

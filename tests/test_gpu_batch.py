@@ -147,6 +147,19 @@ def test_batch_graph_replay_matches_eager(ip):
         assert np.array_equal(xg, xe) and sg["iterations"] == se["iterations"]
 
 
+def test_batch_chunked_replays_match_single_iterations(ip):
+    """Replays of several lockstep iterations with the device-side convergence mask give the
+    same bits and iteration counts as one iteration per host round trip."""
+    probs = [sin_problem(ip, sq(ip, n, 70 + n), N, rtol=1e-10) for n, N in MIX2]
+    ref = run_batch(ip, 2, probs, chunk=1)[2]
+    for chunk in (3, 8, 64):
+        got = run_batch(ip, 2, probs, chunk=chunk)[2]
+        for (r1, x1, s1), (r2, x2, s2) in zip(ref, got):
+            assert r1 == r2 == ip.OK
+            assert np.array_equal(x1, x2)
+            assert s1["iterations"] == s2["iterations"]
+
+
 def test_batch_not_converged_release_and_reuse(ip):
     b = ip.Batch(2, precision=ip.PRECISION_FP64)
     m = sq(ip, 6, 1)
