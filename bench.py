@@ -88,6 +88,8 @@ def parse():
     ap.add_argument("--precision-n", type=int, default=48, help="cube size of the precision form")
     ap.add_argument("--precision-order", type=int, default=4)
     ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--precision-variants", type=str, default=",".join(PRECISION_VARIANTS),
+                    help="with --precision-only: the solvers to run (profiling)")
     ap.add_argument("--skip-viscous", action="store_true")
     ap.add_argument("--precision-only", action="store_true",
                     help="run only the precision-form leg and print its JSON (a profiling aid)")
@@ -103,6 +105,8 @@ def parse():
     ap.add_argument("--serve-rates", type=str, default="10,25,50,100,200,500,1000,2000",
                     help="req/s, comma list")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-viscous-vec", action="store_true")
+    ap.add_argument("--skip-over-int", action="store_true")
     return ap.parse_args()
 
 
@@ -300,7 +304,11 @@ def coarse_amg_leg(ip, torch, s, b_d, x_d, dof, steps):
     return out
 
 
-def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, peaks=None):
+PRECISION_VARIANTS = ("plain_fp64", "mixed_jacobi_cg", "fp64_amg", "mixed_amg")
+
+
+def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, peaks=None,
+                  variants=PRECISION_VARIANTS):
     """Variable-coefficient pure-Neumann cube: div(1/rho grad p) = f, rho = contrast inside a
     sphere of radius 0.25 at the centre (smoothed over 2h), f Gaussian projected to zero mean,
     rel res 1e-8. Runs the plain FP64 path (Jacobi-CG coarse), FP64 with the AMG coarse solve and
@@ -337,6 +345,8 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
                                ("mixed_jacobi_cg", ip.PRECISION_MIXED, ip.COARSE_JACOBI_CG),
                                ("fp64_amg", ip.PRECISION_FP64, ip.COARSE_AMG),
                                ("mixed_amg", ip.PRECISION_MIXED, ip.COARSE_AMG)):
+        if name not in variants:
+            continue
         log(f"precision form: {name}")
         s.set_precision(prec)
         s.set_coarse(coarse)
@@ -353,6 +363,8 @@ def precision_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206, pe
                      "fp64_applies": [st["fine_applies"] for st in stats],
                      "rel_residual": max(st["rel_residual"] for st in stats),
                      "converged": all(st["converged"] for st in stats)}
+    if tuple(variants) != PRECISION_VARIANTS:
+        return out  # a profiling run of some of the solvers
     ref = sols["fp64_amg"]
     nref = float(np.sqrt(np.sum(ref * s.mass(ref))))
     for name in ("plain_fp64", "mixed_jacobi_cg", "mixed_amg"):
@@ -743,6 +755,187 @@ def oracle_check_leg(ip, torch):
         errs.append(mnorm(o, x - xo) / mnorm(o, xo))
     srv.close()
     out["serving"] = {"workload": "the 4 smallest of 40 serving requests", "max_rel_l2_vs_oracle": float(max(errs))}
+    # viscous solve with slip walls at 3^3 x 6, N=3: the oracle's ViscousSolver restatement
+    n = 3
+    mesh = cube(ip, n, 20261208)
+    s = ip.ViscousVecSolver(mesh, 3)
+    bct = vec_bc_types(mesh)
+    s.set_bc_types(bct)
+    o = O.Oracle(*mesh.arrays(), 3, faces=mesh.faces())
+    mm = 1.5 * 100.0 / ((1.0 / n) / 16)
+    s.set_mm_const(mm)
+    o.set_mm_const(mm)
+    s.set_preconditioner(ip.VEC_PRE_FACTOR_INV_MASS)
+    s.set_inv_mass_factor(1.0 / mm)
+    s.set_tol_and_iter(1e-8, 0.0, 1000)
+    rng = np.random.default_rng(20261208)
+    g = [np.ascontiguousarray(rng.standard_normal((s.n_bfaces, s.Nfp))) for _ in range(3)]
+    s.set_bcs(g)
+    f = mm * rng.standard_normal((3, s.n_cells, s.Np))
+    x = np.zeros_like(f)
+    s.solve(f, x)
+    xo, rc, _, _ = o.solve_vec(np.stack(bct), f, bc=np.stack(g), rtol=1e-12, precond=2, factor=1.0 / mm)
+    out["viscous_vec_cube3"] = {"workload": "cube 3^3x6, N=3, 3 components, slip walls, FACTOR_INV_MASS PCG, rel res 1e-8",
+                                "oracle_converged": rc == 0,
+                                "rel_l2_vs_oracle": float(np.linalg.norm(x - xo) / np.linalg.norm(xo)),
+                                "iterations": s.last_stats["iterations"]}
+    del s, o
+    # over-integration at 4^3 x 6, N=4: the true residual of the GPU solution under the oracle's
+    # over-integrated operator (the oracle restates the operator, not its solver)
+    n, N = 4, 4
+    mesh = cube(ip, n, 20261206)
+    s = ip.PoissonSolver(mesh, N)
+    s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
+    s.set_nullspace(True)
+    o = O.Oracle(*mesh.arrays(), N, faces=mesh.faces())
+    o.set_bc_types(np.ones(o.n_bfaces, dtype=np.int32))
+    kn = 1.0 / density(o.node_coords(), 1.0 / n, 1000.0)
+    s.set_factor(kn)
+    o.set_factor(kn)
+    pv, ps = oi_points(ip, mesh, N)
+    kv = 1.0 / density(np.moveaxis(pv, 2, 0), 1.0 / n, 1000.0)
+    ks = 1.0 / density(np.moveaxis(ps, 2, 0), 1.0 / n, 1000.0)
+    s.set_factor_oi(kv)
+    s.set_factor_surf_oi(ks)
+    s.set_pmg([4, 3, 2, 1], pre_it=[1] * 4, post_it=[1] * 4, cheb_orders=[3] * 4, coarse_iters=150)
+    s.set_coarse(ip.COARSE_AMG)
+    b = o.mass(np.random.default_rng(20261206).standard_normal((s.n_cells, s.Np)))
+    b -= b.mean()
+    coi = {"workload": f"cube {n}^3x6, N=4, rho 1000 sphere, over-integrated, pure Neumann, AMG coarse, rel res 1e-8"}
+    rule = ip.oi_rule(3, N)
+    for name, prec in (("fp64_amg", ip.PRECISION_FP64), ("mixed_amg", ip.PRECISION_MIXED)):
+        s.set_precision(prec)
+        s.set_tol_and_iter(1e-8, 0.0, 2000)
+        x = np.zeros_like(b)
+        s.solve(b, x)
+        r = b - o.mult_oi(rule, kv, ks, x)
+        coi[name] = {"rel_residual_under_oracle_operator": float(np.linalg.norm(r) / np.linalg.norm(b)),
+                     "iterations": s.last_stats["iterations"]}
+    out["over_int_cube4"] = coi
+    return out
+
+
+FV3 = np.array([[0, 1, 2], [0, 1, 3], [1, 2, 3], [0, 2, 3]])
+
+
+def vec_bc_types(mesh):
+    """Per-component bc types of the channel set-up on the unit cube: inflow x=0 no-slip
+    (Dirichlet), outflow x=1 Neumann, slip walls on y=0, y=1, z=0, z=1."""
+    coords, c2n = mesh.arrays()
+    _, _, b2c, bnum = mesh.faces()
+    cen = coords[c2n[b2c[:, None], FV3[bnum]]].mean(axis=1)
+    t = np.full(len(b2c), 2, dtype=np.int32)
+    t[cen[:, 0] < 1e-9] = 0
+    t[cen[:, 0] > 1 - 1e-9] = 1
+    return [t.copy(), t.copy(), t.copy()]
+
+
+def viscous_vec_leg(ip, torch, n, order, steps, reynolds=100.0, g0=1.5, seed=20261208):
+    """The velocity step's viscous solve with slip walls (Matrix3Vec: ViscousMatrix3D /
+    ViscousSolver3D): three components, -lap u + (g0 Re/dt) u = f, inflow no-slip, outflow
+    Neumann, slip on the four side walls, the reference's single-phase preconditioner
+    (FACTOR_INV_MASS, factor dt/(g0 Re)), FP64, rel res 1e-8."""
+    mesh = cube(ip, n, seed)
+    s = ip.ViscousVecSolver(mesh, order)
+    s.set_bc_types(vec_bc_types(mesh))
+    dt = (1.0 / n) / (order + 1) ** 2
+    mm = g0 * reynolds / dt
+    s.set_mm_const(mm)
+    s.set_preconditioner(ip.VEC_PRE_FACTOR_INV_MASS)
+    s.set_inv_mass_factor(1.0 / mm)
+    s.set_tol_and_iter(1e-8, 0.0, 1000)
+    rng = np.random.default_rng(seed)
+    g = [np.ascontiguousarray(rng.standard_normal((s.n_bfaces, s.Nfp))) for _ in range(3)]
+    s.set_bcs(g)
+    dof = 3 * s.n_cells * s.Np
+
+    def to_soa(x):
+        soa = np.zeros((s.Np, s.stride))
+        soa[:, :s.n_cells] = x.T
+        return torch.from_numpy(soa).cuda()
+
+    b_d = [to_soa(mm * rng.standard_normal((s.n_cells, s.Np))) for _ in range(3)]
+    x_d = [torch.zeros_like(b_d[0]) for _ in range(3)]
+    bp, xp = [t.data_ptr() for t in b_d], [t.data_ptr() for t in x_d]
+    t0 = time.perf_counter()
+    s.solve(bp, xp, layout=ip.LAYOUT_DEVICE_SOA)  # warm-up (set-up included)
+    setup_and_first = time.perf_counter() - t0
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for t in x_d:
+            t.zero_()
+        s.solve(bp, xp, layout=ip.LAYOUT_DEVICE_SOA)
+        stats.append(s.last_stats)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": f"cube {n}^3x6 Kuhn tets, N={order}, 3 components, -lap u + (g0 Re/dt) u = f, "
+                        f"Re={reynolds:g}, dt={dt:.3g}, inflow no-slip / outflow Neumann / 4 slip walls, "
+                        f"FACTOR_INV_MASS PCG, FP64, rel res 1e-8",
+            "dof": dof, "gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+            "first_solve_with_setup_s": setup_and_first,
+            "iterations": [st["iterations"] for st in stats],
+            "rel_residual": max(st["rel_residual"] for st in stats),
+            "converged": all(st["converged"] for st in stats)}
+
+
+def oi_points(ip, mesh, order):
+    """Physical coordinates of the over-integration points: volume (cells, nv, 3) and faces
+    (cells, 4*ns, 3)."""
+    coords, c2n = mesh.arrays()
+    vx, vw, sb, sw = ip.oi_rule(3, order)
+    RV = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=float)
+    lam = np.linalg.solve(np.vstack([RV.T, np.ones(4)]), np.vstack([vx.T, np.ones(len(vw))])).T  # (nv, 4)
+    X = coords[c2n]  # (E, 4, 3)
+    pv = np.einsum("qv,evx->eqx", lam, X)
+    ps = np.concatenate([np.einsum("qk,ekx->eqx", sb, X[:, FV3[f]]) for f in range(4)], axis=1)
+    return pv, ps
+
+
+def over_int_leg(ip, torch, n, order, steps, contrast=1000.0, seed=20261206):
+    """The precision form's problem with the coefficient over-integrated (the reference's
+    [pressure-solve] over_int = 1: FactorPoissonMatrixFreeDiagOI3D): 1/rho sampled at the
+    cubature points of the cells and faces, the nodal 1/rho for the penalty and the coarser
+    levels; p-MG with the AMG coarse solve, FP64 and mixed precision, rel res 1e-8."""
+    mesh = cube(ip, n, seed)
+    s = ip.PoissonSolver(mesh, order)
+    s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
+    s.set_nullspace(True)
+    h = 1.0 / n
+    s.set_factor(1.0 / density(s.node_coords(), h, contrast))
+    pv, ps = oi_points(ip, mesh, order)
+    s.set_factor_oi(1.0 / density(np.moveaxis(pv, 2, 0), h, contrast))
+    s.set_factor_surf_oi(1.0 / density(np.moveaxis(ps, 2, 0), h, contrast))
+    del pv, ps
+    s.set_solver(ip.SOLVER_PMULTIGRID)
+    orders = list(range(order, 0, -1))
+    s.set_pmg(orders, pre_it=[1] * len(orders), post_it=[1] * len(orders),
+              cheb_orders=[3] * len(orders), coarse_iters=150)
+    s.set_coarse(ip.COARSE_AMG)
+    rng = np.random.default_rng(seed)
+    b = s.mass(rng.standard_normal((s.n_cells, s.Np)))
+    b -= b.sum() / b.size
+    dof = s.n_cells * s.Np
+    soa = np.zeros((s.Np, s.stride))
+    soa[:, :s.n_cells] = b.T
+    b_d = torch.from_numpy(soa).cuda()
+    x_d = torch.zeros_like(b_d)
+    out = {"workload": f"cube {n}^3x6 Kuhn tets, N={order}, rho {contrast:g} in a sphere r=0.25 (band 2h), "
+                       f"coefficient over-integrated ({ip.oi_rule(3, order)[1].size} volume, "
+                       f"{ip.oi_rule(3, order)[3].size} per-face points), pure Neumann, AMG coarse, rel res 1e-8",
+           "dof": dof}
+    for name, prec in (("fp64_amg", ip.PRECISION_FP64), ("mixed_amg", ip.PRECISION_MIXED)):
+        log(f"over-integration: {name}")
+        s.set_precision(prec)
+        s.set_tol_and_iter(1e-8, 0.0, 1000)
+        s.init()
+        wall, lat, stats, _ = timed_solves(s, ip, torch, b_d, x_d, steps, 1, tag="oi_" + name)
+        out[name] = {"gdofs": dof * steps / wall * 1e-9, "ms_per_solve": wall / steps * 1e3,
+                     "iterations": [st["iterations"] for st in stats],
+                     "rel_residual": max(st["rel_residual"] for st in stats),
+                     "converged": all(st["converged"] for st in stats)}
+    out["speedup_mixed_over_fp64_amg"] = out["mixed_amg"]["gdofs"] / out["fp64_amg"]["gdofs"]
     return out
 
 
@@ -757,7 +950,8 @@ def main():
     peaks = ip.device_peaks()
     log(f"device peaks: {peaks}")
     if a.precision_only:
-        res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps, peaks=peaks)
+        res = precision_leg(ip, torch, a.precision_n, a.precision_order, a.steps, peaks=peaks,
+                            variants=tuple(v for v in a.precision_variants.split(",") if v))
         print(json.dumps({"precision_form": res}), flush=True)
         return
     if a.batch_only:
@@ -898,6 +1092,14 @@ def main():
     if not a.skip_viscous:
         log("viscous (Helmholtz) leg")
         out["viscous"] = viscous_leg(ip, torch, a.n, a.order, 3)
+        torch.cuda.empty_cache()
+    if not a.skip_viscous_vec:
+        log("viscous solve with slip walls (vector) leg")
+        out["viscous_vec"] = viscous_vec_leg(ip, torch, a.n, a.order, 3)
+        torch.cuda.empty_cache()
+    if not a.skip_over_int:
+        log("over-integration leg")
+        out["over_int"] = over_int_leg(ip, torch, a.precision_n, a.precision_order, 2)
         torch.cuda.empty_cache()
     if not a.skip_precision:
         log("precision form leg")

@@ -178,6 +178,27 @@ int ins_poisson_set_mm_factor(ins_poisson_t h, const double* mm, int layout);
 /* Constant mass factor, A u + mm J M u: MMPoissonMatrixFree2D::set_factor(DG_FP)
  * (src/solvers/2d/ins_solver.cpp:518-520, kernel vmf_2d_mult_mm_geof.h). 0 = none. */
 int ins_poisson_set_mm_const(ins_poisson_t h, double mm);
+/* Over-integration: FactorPoissonMatrixFreeDiagOI{2D,3D}::mat_free_set_factor_oi and
+ * mat_free_set_factor_surf_oi (src/solvers/2d/mp_ins_solver.cpp:597-598, selected by
+ * [pressure-solve] over_int, :98-100). The coefficient at the cubature points of the cells
+ * (cells x n_vol, in `layout`) and of their faces (cells x (Nfaces*n_surf), face-major, each
+ * face's points in the order of ins_oi_rule); while both are set, the finest order integrates
+ * the operator with them. The nodal factor of ins_poisson_set_factor still sets the penalty,
+ * the boundary terms and the coarser p-multigrid levels, as the reference's caller also calls
+ * set_factor (:599). NULL clears one part. */
+int ins_poisson_set_factor_oi(ins_poisson_t h, const double* factor_vol, int layout);
+int ins_poisson_set_factor_surf_oi(ins_poisson_t h, const double* factor_surf, int layout);
+/* The cubature rules of the over-integration at order N (the library's own rules: the caller
+ * evaluates its coefficient at these points in place of DGConstants::CUB2D_INTERP /
+ * CUBSURF2D_INTERP, src/solvers/3d/ls_solver.cpp:202-232): n_vol volume points (reference
+ * coordinates, n_vol x dim, weights summing to the reference volume) and n_surf points per face
+ * (barycentrics of the face's vertices, n_surf x dim, weights summing to 2). Arrays may be NULL
+ * to query the counts. */
+int ins_oi_rule(int dim, int order, int* n_vol, int* n_surf, double* vol_rst, double* vol_w,
+                double* surf_bary, double* surf_w);
+/* Interpolation from the nodes of order N to those points: vol n_vol x Np and surf
+ * (Nfaces*n_surf) x Np, row-major (either may be NULL). */
+int ins_oi_interp(int dim, int order, double* vol, double* surf);
 /* Scaling of the inverse-mass preconditioner (INS_SOLVER_INV_MASS), z = f (J M)^-1 r:
  * PETScInvMassSolver::setFactor / ViscousSolver2D::set_inv_mass_factor
  * (src/solvers/2d/ins_solver.cpp:521, slip_matrix/viscous_solver.cpp:49-51,347-357). */
@@ -394,6 +415,64 @@ int ins_server_wait(ins_server_t s, int64_t id, double* x, ins_solve_stats* stat
                     double* latency_s, double* queue_s);
 int ins_server_wait_all(ins_server_t s);
 int ins_server_stats(ins_server_t s, int64_t* completed, int64_t* peak_bytes);
+
+/* ---- Vector (velocity) viscous solve with slip walls ----
+ * The slip-matrix family: ViscousMatrix{2D,3D} / FactorViscousMatrix{2D,3D} behind
+ * Matrix2Vec / Matrix3Vec (include/slip_matrix/2d/matrix_2_vec.h:8-14), solved by
+ * ViscousSolver{2D,3D} (include/slip_matrix/2d/viscous_solver.h), as the velocity step builds
+ * them (src/solvers/2d/ins_solver.cpp:100-106, src/solvers/2d/mp_ins_solver.cpp:116-125).
+ * The vector has dim components; every argument of type `const double* const*` (or
+ * `double* const*`) is an array of dim component pointers, each a cells x Np dat in `layout`.
+ * Each component is the scalar operator of ins_poisson_* (SIPG plus the mass term); bc types
+ * are given per component: 0 Dirichlet, 1 Neumann, 2 slip. A boundary face whose every
+ * component has type 2 is a slip wall: the normal velocity is held at zero and the tangential
+ * part is free (vmf_2d_mult_avg_jump.h:7-27, vmf_3d_mult_avg_jump.h:8-40); type 2 on only some
+ * components contributes nothing to them. */
+typedef struct ins_viscous_vec* ins_viscous_vec_t;
+enum { INS_BC_SLIP = 2 };
+/* ViscousSolver2D::Preconditioners (include/slip_matrix/2d/viscous_solver.h:13-15). */
+enum { INS_VEC_PRE_NONE = 0, INS_VEC_PRE_INV_MASS = 1, INS_VEC_PRE_FACTOR_INV_MASS = 2,
+       INS_VEC_PRE_RECP_FACTOR_DAT_INV_MASS = 3, INS_VEC_PRE_JACOBI = 4,
+       INS_VEC_PRE_BLOCK_JACOBI = 5 };
+/* new ViscousMatrix2D(mesh) / FactorViscousMatrix2D(mesh) + new ViscousSolver2D(mesh)
+ * (src/solvers/2d/ins_solver.cpp:100-106). */
+int ins_viscous_vec_create(ins_mesh_t mesh, int order, ins_viscous_vec_t* out);
+void ins_viscous_vec_destroy(ins_viscous_vec_t h);
+int ins_viscous_vec_info(ins_viscous_vec_t h, int* ncomp, int* np, int* nfp, int64_t* n_cells,
+                         int64_t* stride, int64_t* n_bfaces);
+/* Stride of the caller's device SoA dats (as ins_poisson_set_dat_stride). */
+int ins_viscous_vec_set_dat_stride(ins_viscous_vec_t h, int64_t stride);
+/* Matrix2Vec::set_bc_types(u_bc_ty, v_bc_ty) (matrix_2_vec.h:11): dim arrays of n_bfaces. */
+int ins_viscous_vec_set_bc_types(ins_viscous_vec_t h, const int* const* bc_types, int64_t n_bfaces);
+/* FactorViscousMatrix2D::set_factor (NULL = constant 1) and set_mm_factor
+ * (include/slip_matrix/2d/factor_viscous_matrix.h:20-21); ViscousMatrix2D::set_factor, the
+ * constant mass factor (include/slip_matrix/2d/viscous_matrix.h:20). */
+int ins_viscous_vec_set_factor(ins_viscous_vec_t h, const double* factor, int layout);
+int ins_viscous_vec_set_mm_factor(ins_viscous_vec_t h, const double* mm, int layout);
+int ins_viscous_vec_set_mm_const(ins_viscous_vec_t h, double mm);
+/* ViscousSolver2D::set_bcs(u_bcs, v_bcs): dim arrays of bfaces x Nfp (AoS), or NULL = none. */
+int ins_viscous_vec_set_bcs(ins_viscous_vec_t h, const double* const* bc_data, int layout);
+/* ViscousSolver2D::set_tol_and_iter (defaults rtol 1e-8, atol 1e-9, 5000 iterations,
+ * src/solvers/2d/slip_matrix/viscous_solver.cpp:21-25), set_preconditioner, set_inv_mass_factor,
+ * set_inv_mass_recp_factor. */
+int ins_viscous_vec_set_tol_and_iter(ins_viscous_vec_t h, double rtol, double atol, int max_iter);
+int ins_viscous_vec_set_preconditioner(ins_viscous_vec_t h, int preconditioner);
+int ins_viscous_vec_set_inv_mass_factor(ins_viscous_vec_t h, double f);
+int ins_viscous_vec_set_inv_mass_recp_factor(ins_viscous_vec_t h, const double* f, int layout);
+/* Matrix2Vec::mult(u_in, v_in, u_out, v_out) and apply_bc(u_rhs, v_rhs, u_bc, v_bc) (in place
+ * on rhs, with the data of ins_viscous_vec_set_bcs). */
+int ins_viscous_vec_mult(ins_viscous_vec_t h, const double* const* in, double* const* out, int layout);
+int ins_viscous_vec_apply_bc(ins_viscous_vec_t h, double* const* rhs, int layout);
+/* Diagonal of the Jacobi preconditioner (FactorViscousMatrix2D::calc_diag, u_diag / v_diag). */
+int ins_viscous_vec_diag(ins_viscous_vec_t h, double* const* diag, int layout);
+/* Inverse diagonal blocks of the block-Jacobi preconditioner (calc_inv_block_diag), host
+ * dim x cells x Np x Np row-major; needs INS_VEC_PRE_BLOCK_JACOBI. */
+int ins_viscous_vec_inv_block_diag(ins_viscous_vec_t h, double* out);
+/* ViscousSolver2D::solve(u_rhs, v_rhs, u_ans, v_ans) (viscous_solver.cpp:59-75): boundary terms
+ * added to a copy of the rhs (the caller's rhs is not modified), ans IN/OUT. 0 converged,
+ * 1 not converged, <0 errors (INS_ERR_BREAKDOWN on p.Ap <= 0). */
+int ins_viscous_vec_solve(ins_viscous_vec_t h, const double* const* rhs, double* const* ans, int layout,
+                          ins_solve_stats* stats);
 
 const char* ins_poisson_last_error(void);
 

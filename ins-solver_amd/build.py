@@ -17,11 +17,11 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "ins_poisson", "libins_poisson.so")
 ARCH = os.environ.get("INS_POISSON_ARCH", "gfx950")
 
-SOURCES = ["refelem.cpp", "mesh.cpp", "hdf5_reader.cpp", "solver.cpp", "batch.cpp", "server.cpp", "amg.cpp",
+SOURCES = ["refelem.cpp", "mesh.cpp", "hdf5_reader.cpp", "solver.cpp", "viscous_vec.cpp", "oi.cpp", "batch.cpp", "server.cpp", "amg.cpp",
            "capi.cpp", "sipg_kernels.hip", "vec_kernels.hip", "batch_kernels.hip",
            "amg_kernels.hip", "block_kernels.hip", "rhs_kernels.hip", "sipg_mfma.hip",
            "probe_kernels.hip", "mass_kernels.hip",
-           "proj_kernels.hip"]
+           "proj_kernels.hip", "sipg_oi.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # packed FP32 instructions are disabled for all device code

@@ -12,7 +12,10 @@
 #include "batch.h"
 #include "hdf5_reader.h"
 #include "mesh.h"
+#include "oi.h"
+#include "refelem.h"
 #include "server.h"
+#include "viscous_vec.h"
 #include "solver.h"
 
 struct ins_mesh {
@@ -20,6 +23,9 @@ struct ins_mesh {
 };
 struct ins_poisson {
   std::unique_ptr<insp::PoissonSolver> s;
+};
+struct ins_viscous_vec {
+  std::unique_ptr<insp::VecViscousSolver> s;
 };
 struct ins_batch {
   std::unique_ptr<insp::BatchEngine> e;
@@ -64,6 +70,21 @@ insp::Layout to_layout(int layout) {
 insp::PoissonSolver& S(ins_poisson_t h) {
   if (!h || !h->s) throw insp::InvalidArgument("null solver handle");
   return *h->s;
+}
+insp::VecViscousSolver& V(ins_viscous_vec_t h) {
+  if (!h || !h->s) throw insp::InvalidArgument("null solver handle");
+  return *h->s;
+}
+void fill_stats(const insp::SolveStats& st, ins_solve_stats* stats) {
+  if (!stats) return;
+  stats->iterations = st.iterations;
+  stats->converged = st.converged;
+  stats->rel_residual = st.rel_residual;
+  stats->abs_residual = st.abs_residual;
+  stats->bnorm = st.bnorm;
+  stats->seconds = st.seconds;
+  stats->fine_applies = st.fine_applies;
+  stats->smoother_applies = st.smoother_applies;
 }
 const insp::HostMesh& M(ins_mesh_t m) {
   if (!m) throw insp::InvalidArgument("null mesh handle");
@@ -413,6 +434,50 @@ int ins_poisson_set_bc_types(ins_poisson_t h, const int* bc_types, int64_t n_bfa
 int ins_poisson_set_factor(ins_poisson_t h, const double* factor, int layout) {
   return guarded([&] {
     S(h).set_factor(factor, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_factor_oi(ins_poisson_t h, const double* factor_vol, int layout) {
+  return guarded([&] {
+    S(h).set_factor_oi(factor_vol, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_poisson_set_factor_surf_oi(ins_poisson_t h, const double* factor_surf, int layout) {
+  return guarded([&] {
+    S(h).set_factor_surf_oi(factor_surf, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_oi_rule(int dim, int order, int* n_vol, int* n_surf, double* vol_rst, double* vol_w,
+                double* surf_bary, double* surf_w) {
+  return guarded([&] {
+    if (dim != 2 && dim != 3) throw insp::InvalidArgument("dimension must be 2 or 3");
+    if (order < 1 || order > insp::kMaxOrder) throw insp::InvalidArgument("order must be in 1..4");
+    const insp::OiRule q = insp::make_oi_rule(dim, order);
+    if (n_vol) *n_vol = q.nv;
+    if (n_surf) *n_surf = q.ns;
+    if (vol_rst) std::copy(q.vx.begin(), q.vx.end(), vol_rst);
+    if (vol_w) std::copy(q.vw.begin(), q.vw.end(), vol_w);
+    if (surf_bary) std::copy(q.sb.begin(), q.sb.end(), surf_bary);
+    if (surf_w) std::copy(q.sw.begin(), q.sw.end(), surf_w);
+    return INS_OK;
+  });
+}
+
+int ins_oi_interp(int dim, int order, double* vol, double* surf) {
+  return guarded([&] {
+    if (dim != 2 && dim != 3) throw insp::InvalidArgument("dimension must be 2 or 3");
+    if (order < 1 || order > insp::kMaxOrder) throw insp::InvalidArgument("order must be in 1..4");
+    const insp::OiRule q = insp::make_oi_rule(dim, order);
+    const insp::RefElement re = insp::make_ref_element(dim, order);
+    std::vector<double> Iv, Is, xv, xs;
+    insp::oi_interp(re, q, Iv, Is, xv, xs);
+    if (vol) std::copy(Iv.begin(), Iv.end(), vol);
+    if (surf) std::copy(Is.begin(), Is.end(), surf);
     return INS_OK;
   });
 }
@@ -973,6 +1038,163 @@ int ins_server_stats(ins_server_t s, int64_t* completed, int64_t* peak_bytes) {
     if (completed) *completed = sv.completed();
     if (peak_bytes) *peak_bytes = (int64_t)sv.peak_bytes();
     return INS_OK;
+  });
+}
+
+int ins_viscous_vec_create(ins_mesh_t mesh, int order, ins_viscous_vec_t* out) {
+  return guarded([&] {
+    if (!out) throw insp::InvalidArgument("null output handle");
+    const auto& m = M(mesh);
+    auto* h = new ins_viscous_vec;
+    try {
+      h->s = std::make_unique<insp::VecViscousSolver>(m, order);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return INS_OK;
+  });
+}
+
+void ins_viscous_vec_destroy(ins_viscous_vec_t h) { delete h; }
+
+int ins_viscous_vec_info(ins_viscous_vec_t h, int* ncomp, int* np, int* nfp, int64_t* n_cells,
+                         int64_t* stride, int64_t* n_bfaces) {
+  return guarded([&] {
+    auto& v = V(h);
+    auto& s = v.base();
+    if (ncomp) *ncomp = v.ncomp();
+    if (np) *np = s.np();
+    if (nfp) *nfp = s.nfp();
+    if (n_cells) *n_cells = s.n_cells();
+    if (stride) *stride = s.stride();
+    if (n_bfaces) *n_bfaces = s.mesh().n_bfaces;
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_dat_stride(ins_viscous_vec_t h, int64_t stride) {
+  return guarded([&] {
+    V(h).base().set_dat_stride(stride);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_bc_types(ins_viscous_vec_t h, const int* const* bc_types, int64_t n_bfaces) {
+  return guarded([&] {
+    V(h).set_bc_types(bc_types, n_bfaces);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_factor(ins_viscous_vec_t h, const double* factor, int layout) {
+  return guarded([&] {
+    auto& v = V(h);
+    v.base().set_factor(factor, to_layout(layout));
+    v.touch();
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_mm_factor(ins_viscous_vec_t h, const double* mm, int layout) {
+  return guarded([&] {
+    auto& v = V(h);
+    v.base().set_mm_factor(mm, to_layout(layout));
+    v.touch();
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_mm_const(ins_viscous_vec_t h, double mm) {
+  return guarded([&] {
+    auto& v = V(h);
+    v.base().set_mm_const(mm);
+    v.touch();
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_bcs(ins_viscous_vec_t h, const double* const* bc_data, int layout) {
+  return guarded([&] {
+    const auto l = to_layout(layout);
+    if (l == insp::Layout::DEVICE_SOA) throw insp::InvalidArgument("bc data must be AoS");
+    V(h).set_bcs(bc_data, l);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_tol_and_iter(ins_viscous_vec_t h, double rtol, double atol, int max_iter) {
+  return guarded([&] {
+    V(h).set_tol_and_iter(rtol, atol, max_iter);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_preconditioner(ins_viscous_vec_t h, int preconditioner) {
+  return guarded([&] {
+    if (preconditioner < INS_VEC_PRE_NONE || preconditioner > INS_VEC_PRE_BLOCK_JACOBI)
+      throw insp::InvalidArgument("unknown preconditioner");
+    V(h).set_preconditioner(static_cast<insp::VecPrecond>(preconditioner));
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_inv_mass_factor(ins_viscous_vec_t h, double f) {
+  return guarded([&] {
+    if (!std::isfinite(f)) throw insp::InvalidArgument("inverse-mass factor must be finite");
+    V(h).base().set_inv_mass_factor(f);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_set_inv_mass_recp_factor(ins_viscous_vec_t h, const double* f, int layout) {
+  return guarded([&] {
+    V(h).base().set_inv_mass_recp_factor(f, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_mult(ins_viscous_vec_t h, const double* const* in, double* const* out, int layout) {
+  return guarded([&] {
+    if (!in || !out) throw insp::InvalidArgument("null vector");
+    V(h).mult(in, out, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_apply_bc(ins_viscous_vec_t h, double* const* rhs, int layout) {
+  return guarded([&] {
+    if (!rhs) throw insp::InvalidArgument("null vector");
+    V(h).apply_bc(rhs, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_diag(ins_viscous_vec_t h, double* const* diag, int layout) {
+  return guarded([&] {
+    if (!diag) throw insp::InvalidArgument("null vector");
+    V(h).diag(diag, to_layout(layout));
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_inv_block_diag(ins_viscous_vec_t h, double* out) {
+  return guarded([&] {
+    if (!out) throw insp::InvalidArgument("null output");
+    V(h).inv_block_diag(out);
+    return INS_OK;
+  });
+}
+
+int ins_viscous_vec_solve(ins_viscous_vec_t h, const double* const* rhs, double* const* ans, int layout,
+                          ins_solve_stats* stats) {
+  return guarded([&] {
+    if (!rhs || !ans) throw insp::InvalidArgument("null rhs or ans");
+    insp::SolveStats st;
+    const int rc = V(h).solve(rhs, ans, to_layout(layout), &st);
+    fill_stats(st, stats);
+    return rc;
   });
 }
 

@@ -23,6 +23,24 @@ struct MeshView {
   const int* nbc = nullptr;
 };
 
+// Boundary faces of the vector (velocity) operator with the slip condition (bc type 2 on
+// every component: vmf_2d_mult_avg_jump.h:7-27, vmf_3d_mult_avg_jump.h:8-40): component c
+// sees the Dirichlet ghost of its share of the normal part, u_c -> n_c (n.U) and
+// T_c -> n_c sum_d n_d T_d, so the normal velocity is held at zero and the tangential part
+// is free. The components of U live cstride apart, their face traces tstride apart.
+struct NoSlip {
+  static constexpr bool enabled = false;
+};
+template <typename T>
+struct SlipFaces {
+  static constexpr bool enabled = true;
+  const T* u0;        // component 0 of the vector
+  const T* tr0;       // face traces of component 0 (orders >= 2)
+  int64_t cstride, tstride;
+  const double* bn;   // [n_bfaces][DIM] outward unit normals
+  int comp;
+};
+
 struct LevelView {
   int N;
   const void* C;       // operator constants in the kernel precision
@@ -50,6 +68,32 @@ template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
                        hipStream_t s);
+// Element pass of component slip.comp of the vector (velocity) operator, FP64: the scalar
+// operator of that component, except that faces of bc type 2 are slip faces, which read the
+// other components' values and face traces of the cell (the traces of every component from
+// launch_trace first, at orders >= 2). With `partial`, u.(A u) partials as launch_apply.
+void launch_apply_slip(const MeshView& m, const LevelView& l, const double* u, const double* tr,
+                       const double* kfac, double* out, const SlipFaces<double>& slip, hipStream_t s,
+                       double* partial = nullptr);
+// Over-integrated variable-coefficient operator (sipg_oi.hip; rules and tables in oi.h): the
+// coefficient at the volume cubature points (kv, [nv][stride]) and the face points (ks,
+// [Nf*ns][stride]); the trace pass writes u and sJ k n.grad u at every face point (tU, tT).
+template <typename T>
+struct OiView {
+  const T* C;
+  const int* qmap;
+  const T* kv;
+  const T* ks;
+  T* tU;
+  T* tT;
+};
+template <typename T>
+void launch_apply_oi(const MeshView& m, const LevelView& l, const OiView<T>& o, const T* u, T* out,
+                     hipStream_t s, double* partial = nullptr);
+void launch_diag_oi(const MeshView& m, const LevelView& l, const OiView<double>& o, double* diag,
+                    hipStream_t s);
+// The kernels' compile-time rule sizes agree with the host's rule and tables.
+bool oi_shape_matches(int dim, int N, int nv, int ns, int table_size);
 // Element pass on the matrix cores (sipg_mfma.hip): 3D, orders 2-4. launch_apply
 // takes it whenever the level view carries fragments.
 bool mfma_apply_supported(int dim, int N);
@@ -60,9 +104,11 @@ std::vector<double> build_mfma_fragments(int dim, int N, const std::vector<doubl
 template <typename T>
 void launch_apply_mfma(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, T* out, hipStream_t s, double* partial);
-// Exact diagonal of the operator (double).
+// Exact diagonal of the operator (double). With bnormal ([n_bfaces][dim] outward unit
+// normals): the diagonal of component comp of the vector operator, whose bc type 2 faces are
+// slip faces.
 void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
-                 hipStream_t s);
+                 hipStream_t s, const double* bnormal = nullptr, int comp = 0);
 // Penalty tau*sJ per cell face (double), tau = 2(P+1)(P+2) max(fscale k) with P = pen_order.
 void launch_tau(const MeshView& m, const LevelView& l, const double* kfac, int pen_order,
                 double* tauS, hipStream_t s);

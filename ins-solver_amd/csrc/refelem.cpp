@@ -514,6 +514,21 @@ RefElement make_ref_element(int dim, int N) {
   return E;
 }
 
+void basis_at(const RefElement& re, const std::vector<double>& pts, std::vector<double>& I,
+              std::vector<double>* G) {
+  const int dim = re.dim, n = (int)(pts.size() / dim), Np = re.Np;
+  std::vector<double> r(n), s(n), t(n, 0.0), V, Vd[3];
+  for (int i = 0; i < n; ++i) {
+    r[i] = pts[(size_t)i * dim];
+    s[i] = pts[(size_t)i * dim + 1];
+    if (dim == 3) t[i] = pts[(size_t)i * dim + 2];
+  }
+  vandermonde(dim, re.N, r, s, t, V, G ? Vd : nullptr);
+  I = mat_mul(V, re.invV, n, Np, Np);
+  if (G)
+    for (int d = 0; d < dim; ++d) G[d] = mat_mul(Vd[d], re.invV, n, Np, Np);
+}
+
 std::vector<double> interp_matrix(const RefElement& to, const RefElement& from) {
   if (to.dim != from.dim) throw std::runtime_error("interp_matrix: dimension mismatch");
   std::vector<double> tt = to.dim == 3 ? to.t : std::vector<double>(to.Np, 0.0);

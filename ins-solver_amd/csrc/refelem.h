@@ -51,6 +51,11 @@ RefElement make_ref_element(int dim, int N);
 // simplex: returns to.Np x from.Np.
 std::vector<double> interp_matrix(const RefElement& to, const RefElement& from);
 
+// Values (n x Np) and, when G is given, reference gradients (G[d], n x Np) of the nodal basis
+// at n points (reference coordinates, n x dim).
+void basis_at(const RefElement& re, const std::vector<double>& pts, std::vector<double>& I,
+              std::vector<double>* G);
+
 // Small dense helpers (row-major).
 std::vector<double> mat_inverse(const std::vector<double>& A, int n);
 std::vector<double> mat_mul(const std::vector<double>& A, const std::vector<double>& B,

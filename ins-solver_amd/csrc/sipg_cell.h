@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dg_layout.h"
+#include "kernels.h"
 #include "reduce.h"
 
 namespace insp {
@@ -117,13 +118,14 @@ struct ChebOut {
 
 // One cell of the element pass; returns u.(A u) over the cell. PACK: nbr holds the packed
 // words (neighbour << 5) | code, -(bface+1) on the boundary, and code is not read.
-template <int DIM, int N, typename T, int VAR, bool PACK = false, class Out = T*>
+template <int DIM, int N, typename T, int VAR, bool PACK = false, class Out = T*,
+          class Slip = NoSlip>
 __device__ __forceinline__ double
 apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restrict__ tr,
            const double* __restrict__ Kt, const double* __restrict__ tauS,
            const int* __restrict__ nbr, const int* __restrict__ code,
            const int* __restrict__ bctype, const T* __restrict__ kfac, cptr<T> __restrict__ C,
-           cptr<int> __restrict__ tab, Out out) {
+           cptr<int> __restrict__ tab, Out out, const Slip& slip = Slip{}) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, N);
@@ -179,13 +181,14 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
     } else {
       nb = nbr[f * stride + e];
     }
-    int mode = 0;  // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms)
+    int mode = 0;  // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms), 3 slip
     int tbase = 0;
     if (nb >= 0) {
       if constexpr (!PACK) c = code[f * stride + e];
       tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
     } else {
-      mode = bctype[-nb - 1] == 0 ? 1 : 2;
+      const int bt = bctype[-nb - 1];
+      mode = bt == 0 ? 1 : (Slip::enabled && bt == 2 ? 3 : 2);
     }
     if (mode == 2) continue;
     const T ts = (T)tauS[f * stride + e];
@@ -199,8 +202,21 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
     for (int i = 0; i < Nfp; ++i) sv[i] = 0;
     for (int j = 0; j < Nfp; ++j) {
       const int p = tab[L.tab_fmask + f * Nfp + j];
-      const T um = u[p * stride + e];
-      const T Tm = tr[(int64_t)(f * Nfp + j) * stride + e];
+      T um = u[p * stride + e];
+      T Tm = tr[(int64_t)(f * Nfp + j) * stride + e];
+      if constexpr (Slip::enabled) {
+        if (mode == 3) {
+          const double* n = slip.bn + (int64_t)(-nb - 1) * DIM;
+          T su = 0, st = 0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            su = fma((T)n[d], slip.u0[d * slip.cstride + p * stride + e], su);
+            st = fma((T)n[d], slip.tr0[d * slip.tstride + (int64_t)(f * Nfp + j) * stride + e], st);
+          }
+          um = (T)n[slip.comp] * su;
+          Tm = (T)n[slip.comp] * st;
+        }
+      }
       T up, Tp;
       if (mode == 0) {
         const int pk = tab[tbase + j];
@@ -272,13 +288,13 @@ static_assert(face_alpha_is_rows(2) && face_alpha_is_rows(3), "face coefficient 
 
 // PACK: nbr holds one word per face, (neighbour << 5) | code inside the mesh and the
 // boundary index -(bface+1) on the boundary, and code is not read.
-template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*>
+template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
               const int* __restrict__ code, const int* __restrict__ bctype,
               const T* __restrict__ kfac, cptr<T> __restrict__ C, cptr<int> __restrict__ tab,
-              Out out) {
+              Out out, const Slip& slip = Slip{}) {
   constexpr OpLayout L = op_layout(DIM, 1);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   constexpr FmaskTable FM = make_fmask(DIM, 1);
@@ -371,15 +387,48 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
             tn0 = fma(-(T)Kt[(int64_t)kidx(DIM, rr, s) * stride + nb], gn[s], tn0);
         }
       } else {
-        mode = bctype[-nb - 1] == 0 ? 1 : 2;
+        const int bt = bctype[-nb - 1];
+        mode = bt == 0 ? 1 : (Slip::enabled && bt == 2 ? 3 : 2);
       }
       if (mode == 2) continue;
+      // slip: n_c (n.U) at the face nodes and n_c sum_d n_d (beta.grad u_d), the gradients of
+      // the components recomputed from their nodal values (constant on a P1 cell)
+      T nc = 0, sm0 = 0;
+      const double* bnv = nullptr;
+      if constexpr (Slip::enabled) {
+        if (mode == 3) {
+          bnv = slip.bn + (int64_t)(-nb - 1) * DIM;
+          nc = (T)bnv[slip.comp];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            T t = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < DIM; ++s2) {
+              T a = 0;
+#pragma unroll
+              for (int m = 0; m < Np; ++m)
+                a = fma(C[L.off_D + (s2 * Np) * Np + m], slip.u0[d * slip.cstride + m * stride + e], a);
+              t = fma(beta[s2], a, t);
+            }
+            sm0 = fma((T)bnv[d], t, sm0);
+          }
+        }
+      }
       const T ts = (T)tauS[f * stride + e];
 #pragma unroll
       for (int j = 0; j < Nfp; ++j) {
         const int p = FM.v[f * Nfp + j];
-        const T um = uu[p];
-        const T Tm = kk[p] * tm0;
+        T um = uu[p];
+        T Tm = kk[p] * tm0;
+        if constexpr (Slip::enabled) {
+          if (mode == 3) {
+            T su = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) su = fma((T)bnv[d], slip.u0[d * slip.cstride + p * stride + e], su);
+            um = nc * su;
+            Tm = kk[p] * nc * sm0;
+          }
+        }
         T up, Tp;
         if (mode == 0) {
           const int node = tab[tbase + j] >> 6;

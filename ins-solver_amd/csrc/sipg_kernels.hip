@@ -120,12 +120,39 @@ k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double
                                            ChebOut<T>{f, inv, d, x_next, c1, c2});
 }
 
+// Element pass of one component of the vector (velocity) operator with slip faces (FP64;
+// orders >= 2 read the face traces, order 1 recomputes them). kfac may be null (k = 1).
+template <int DIM, int N>
+__global__ void __launch_bounds__(256)
+k_apply_slip(int64_t E, int64_t stride, const double* __restrict__ u, const double* __restrict__ tr,
+             const double* __restrict__ Kt, const double* __restrict__ tauS,
+             const int* __restrict__ nbr, const int* __restrict__ code,
+             const int* __restrict__ bctype, const double* __restrict__ kfac,
+             const double* __restrict__ C, const int* __restrict__ tab, double* __restrict__ out,
+             double* __restrict__ partial, SlipFaces<double> slip) {
+  const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
+  double dotv = 0.0;
+  if (e < E) {
+    if constexpr (N == 1)
+      dotv = apply_p1_cell<DIM, double, 2, false, double*, SlipFaces<double>>(
+          e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab), out, slip);
+    else
+      dotv = apply_cell<DIM, N, double, 2, false, double*, SlipFaces<double>>(
+          e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab), out, slip);
+  }
+  if (partial) {
+    const double t = block_sum1<256>(dotv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
 template <int DIM, int N, bool VAR>
 __global__ void __launch_bounds__(256)
 k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* __restrict__ tauS,
        const int* __restrict__ nbr, const int* __restrict__ bctype,
        const double* __restrict__ kfac, const double* __restrict__ C,
-       const int* __restrict__ tab, double* __restrict__ diag) {
+       const int* __restrict__ tab, double* __restrict__ diag,
+       const double* __restrict__ bn, int comp) {
   constexpr OpLayout L = op_layout(DIM, N);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
@@ -133,7 +160,7 @@ k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* _
   double K[L.NK];
 #pragma unroll
   for (int k = 0; k < L.NK; ++k) K[k] = Kt[k * stride + e];
-  double beta[Nf][DIM], ts[Nf];
+  double beta[Nf][DIM], ts[Nf], wf[Nf];
   int mode[Nf];
 #pragma unroll
   for (int f = 0; f < Nf; ++f) {
@@ -145,7 +172,20 @@ k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* _
       beta[f][s] = b;
     }
     const int nb = nbr[f * stride + e];
-    mode[f] = nb >= 0 ? 0 : (bctype[-nb - 1] == 0 ? 1 : 2);
+    // slip faces of the vector operator (bn given): the Dirichlet terms of component comp,
+    // scaled by n_comp^2 (the part of the projected jump and trace that is its own)
+    wf[f] = 1.0;
+    if (nb >= 0) {
+      mode[f] = 0;
+    } else {
+      const int bt = bctype[-nb - 1];
+      mode[f] = bt == 0 ? 1 : 2;
+      if (bn && bt == 2) {
+        const double nc = bn[(int64_t)(-nb - 1) * DIM + comp];
+        mode[f] = 1;
+        wf[f] = nc * nc;
+      }
+    }
     ts[f] = tauS[f * stride + e];
   }
   for (int i = 0; i < Np; ++i) {
@@ -177,8 +217,8 @@ k_diag(int64_t E, int64_t stride, const double* __restrict__ Kt, const double* _
 #pragma unroll
         for (int s = 0; s < DIM; ++s) tm += beta[f][s] * C[L.off_D + (s * Np + p) * Np + i];
         tm *= kp;
-        const double F = scale * (ts[f] * um - tm);
-        const double sig = -scale * kp * um;
+        const double F = wf[f] * scale * (ts[f] * um - tm);
+        const double sig = -wf[f] * scale * kp * um;
         if (li >= 0) d += C[L.off_MfT + (f * Nfp + j) * Nfp + li] * F;
 #pragma unroll
         for (int r = 0; r < DIM; ++r)
@@ -416,6 +456,17 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
 
 int apply_blocks(int64_t E) { return (int)((E + 255) / 256); }
 
+void launch_apply_slip(const MeshView& m, const LevelView& l, const double* u, const double* tr,
+                       const double* kfac, double* out, const SlipFaces<double>& slip, hipStream_t s,
+                       double* partial) {
+  if (m.E == 0) return;
+  const double* C = l.C64;
+  INSP_DISPATCH_N(m.dim, l.N,
+                  hipLaunchKernelGGL((k_apply_slip<D_, N_>), grid_for(m.E), dim3(256), 0, s, m.E,
+                                     m.stride, u, tr, m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac, C,
+                                     l.tab, out, partial, slip));
+}
+
 template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
@@ -469,18 +520,18 @@ template void launch_apply<float>(const MeshView&, const LevelView&, const float
                                   const float*, float*, hipStream_t, double*);
 
 void launch_diag(const MeshView& m, const LevelView& l, const double* kfac, double* diag,
-                 hipStream_t s) {
+                 hipStream_t s, const double* bnormal, int comp) {
   if (m.E == 0) return;
   if (kfac)
     INSP_DISPATCH_N(m.dim, l.N,
                     hipLaunchKernelGGL((k_diag<D_, N_, true>), grid_for(m.E), dim3(256), 0, s, m.E,
                                        m.stride, m.Kt, l.tauS, m.nbr, m.bctype, kfac, l.C64,
-                                       l.tab, diag));
+                                       l.tab, diag, bnormal, comp));
   else
     INSP_DISPATCH_N(m.dim, l.N,
                     hipLaunchKernelGGL((k_diag<D_, N_, false>), grid_for(m.E), dim3(256), 0, s, m.E,
                                        m.stride, m.Kt, l.tauS, m.nbr, m.bctype, kfac, l.C64,
-                                       l.tab, diag));
+                                       l.tab, diag, bnormal, comp));
 }
 
 void launch_tau(const MeshView& m, const LevelView& l, const double* kfac, int pen_order,

@@ -159,6 +159,16 @@ MeshView PoissonSolver::mesh_view() const {
   return m;
 }
 
+const double* PoissonSolver::fine_factor() const {
+  return var_factor_ ? levels_[0]->w64.k.p : nullptr;
+}
+const double* PoissonSolver::fine_mm() const {
+  return var_mm_ ? levels_[0]->w64.mm.p : nullptr;
+}
+int PoissonSolver::fine_face_nodes() const {
+  return levels_[0]->L.Nf * levels_[0]->L.Nfp;
+}
+
 bool PoissonSolver::matrix_pass(int l) const {
   return l >= 0 && l < (int)levels_.size() && !levels_[l]->w32.frag.empty();
 }
@@ -193,6 +203,61 @@ void PoissonSolver::set_factor(const double* k, Layout layout) {
     var_factor_ = true;
   }
   initialised_ = false;
+}
+
+const OiRule& PoissonSolver::oi_rule() {
+  if (!oi_rule_) oi_rule_ = std::make_unique<OiRule>(make_oi_rule(mesh_.dim, N_));
+  return *oi_rule_;
+}
+
+void PoissonSolver::ensure_oi_tables() {
+  if (!oi_C64_.empty()) return;
+  const OiRule& q = oi_rule();
+  const std::vector<double> C = make_oi_tables(fine_re_, q);
+  if (!oi_shape_matches(mesh_.dim, N_, q.nv, q.ns, (int)C.size()))
+    throw std::logic_error("over-integration rule and kernels disagree");
+  oi_C64_.upload(C.data(), C.size());
+  std::vector<float> Cf(C.begin(), C.end());
+  oi_C32_.upload(Cf.data(), Cf.size());
+  oi_qmap_.upload(q.qmap.data(), q.qmap.size());
+}
+
+// One part of the coefficient at the points (volume: nv rows, faces: Nf*ns rows), kept in both
+// precisions; null clears it. Over-integration is on when both parts are set.
+void PoissonSolver::set_oi_part(const double* k, Layout layout, bool surface) {
+  DBuf<double>& d = surface ? oi_ks64_ : oi_kv64_;
+  DBuf<float>& f = surface ? oi_ks32_ : oi_kv32_;
+  if (!k) {
+    d.release();
+    f.release();
+  } else {
+    ensure_oi_tables();
+    const OiRule& q = oi_rule();
+    const int rows = surface ? (mesh_.dim + 1) * q.ns : q.nv;
+    d.alloc((size_t)rows * stride_);
+    f.alloc((size_t)rows * stride_);
+    import_vec(k, d.p, layout, rows);
+    convert<double, float>((int64_t)rows * stride_, d.p, f.p, stream_);
+    INSP_HIP(hipStreamSynchronize(stream_));
+  }
+  oi_ = !oi_kv64_.empty() && !oi_ks64_.empty();
+  initialised_ = false;
+}
+
+void PoissonSolver::set_factor_oi(const double* kv, Layout layout) { set_oi_part(kv, layout, false); }
+void PoissonSolver::set_factor_surf_oi(const double* ks, Layout layout) { set_oi_part(ks, layout, true); }
+
+template <>
+OiView<double> PoissonSolver::oi_view<double>() {
+  const int64_t n = (int64_t)(mesh_.dim + 1) * oi_rule().ns * stride_;
+  if (oi_tr64_.empty()) oi_tr64_.alloc((size_t)2 * n);
+  return OiView<double>{oi_C64_.p, oi_qmap_.p, oi_kv64_.p, oi_ks64_.p, oi_tr64_.p, oi_tr64_.p + n};
+}
+template <>
+OiView<float> PoissonSolver::oi_view<float>() {
+  const int64_t n = (int64_t)(mesh_.dim + 1) * oi_rule().ns * stride_;
+  if (oi_tr32_.empty()) oi_tr32_.alloc((size_t)2 * n);
+  return OiView<float>{oi_C32_.p, oi_qmap_.p, oi_kv32_.p, oi_ks32_.p, oi_tr32_.p, oi_tr32_.p + n};
 }
 
 void PoissonSolver::set_mm_factor(const double* mm, Layout layout) {
@@ -459,7 +524,10 @@ void PoissonSolver::compute_diagonals() {
   for (size_t l = 0; l < levels_.size(); ++l) {
     Level& lv = *levels_[l];
     const int Np = lv.L.Np;
-    launch_diag(mv, level_view((int)l, false), var_factor_ ? lv.w64.k.p : nullptr, lv.diag.p, stream_);
+    if (oi_ && l == 0)
+      launch_diag_oi(mv, level_view(0, false), oi_view<double>(), lv.diag.p, stream_);
+    else
+      launch_diag(mv, level_view((int)l, false), var_factor_ ? lv.w64.k.p : nullptr, lv.diag.p, stream_);
     if (has_mass_term())
       launch_mass_diag(mv, level_view((int)l, false), Np, var_mm_ ? lv.w64.mm.p : nullptr, mm_const_,
                        lv.diag.p, stream_);
@@ -575,15 +643,19 @@ hipEvent_t PoissonSolver::ev_take() {
 }
 
 template <typename T>
-void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
+void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial, T* tr) {
   Level& lv = *levels_[l];
   Work<T>& w = lv.work<T>();
-  if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
+  const bool own_tr = tr == nullptr;  // a caller-given trace buffer: no allocation, no events
+  if (own_tr) {
+    if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
+    tr = w.tr.p;
+  }
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
   const T* k = var_factor_ ? w.k.p : nullptr;
-  const bool prof = profiling_ && l == 0;
-  const bool prof_coarse = profiling_ && l > 0 && l + 1 == (int)levels_.size() &&
+  const bool prof = own_tr && profiling_ && l == 0;
+  const bool prof_coarse = own_tr && profiling_ && l > 0 && l + 1 == (int)levels_.size() &&
                            (coarse_sample_++ & 3) == 0;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (prof) {
@@ -593,9 +665,14 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial) {
     e0 = ev_take(); e2 = ev_take();
     INSP_HIP(hipEventRecord(e0, stream_));
   }
-  launch_trace<T>(mv, v, u, k, w.tr.p, stream_);
-  if (prof) INSP_HIP(hipEventRecord(e1, stream_));
-  launch_apply<T>(mv, v, u, w.tr.p, k, out, stream_, partial);
+  if (oi_ && l == 0) {
+    if (prof) INSP_HIP(hipEventRecord(e1, stream_));
+    launch_apply_oi<T>(mv, v, oi_view<T>(), u, out, stream_, partial);
+  } else {
+    launch_trace<T>(mv, v, u, k, tr, stream_);
+    if (prof) INSP_HIP(hipEventRecord(e1, stream_));
+    launch_apply<T>(mv, v, u, tr, k, out, stream_, partial);
+  }
   if (has_mass_term())
     launch_mass_add<T>(mv, v, lv.L.Np, var_mm_ ? w.mm.p : nullptr, (T)mm_const_, u, out, stream_,
                        partial);
@@ -664,7 +741,7 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   // no mass term (the next iterate goes to x2, then x and x2 trade places), else apply + stream.
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
-  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() &&
+  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && !(oi_ && l == 0) &&
                     !(v.frag && mfma_apply_supported(mv.dim, v.N));
   auto step = [&](T c1, T c2) {
     ++smoother_applies_;
@@ -706,9 +783,15 @@ void PoissonSolver::coarse_cg(int l) {
   dot<T>(n, w.b.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
   int a = S_CG_A, b = S_CG_B;
   for (int it = 0; it < pmg_.coarse_iters; ++it) {
-    apply_level<T>(l, w.p.p, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
+    if (std::is_same<T, float>::value && matrix_pass(l) && !(oi_ && l == 0)) {
+      // the matrix-core element pass has no fused dot
+      apply_level<T>(l, w.p.p, w.Ap.p);
+      dot<T>(n, w.p.p, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
+    } else {
+      apply_level<T>(l, w.p.p, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
+      finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
+    }
     ++smoother_applies_;
-    finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
     cg_update_z<T>(n, w.z.p, w.Ap.p, w.inv.p, scal_.p, a, S_CG_PAP, b, rs, stream_);
     cg_update_xp<T>(n, w.x.p, w.p.p, w.z.p, scal_.p, a, S_CG_PAP, b, it + 1 == pmg_.coarse_iters,
                     stream_);
@@ -1049,8 +1132,14 @@ int PoissonSolver::fcg_mixed(double tol, int* iters, double* rr_out, int64_t* fi
   copy<float>(n, z32_.p, p32_.p, stream_);
   int status = 1;
   while (it < max_iter_) {
-    apply_level<float>(0, p32_.p, Ap32_.p, partial_.p);  // p.Ap fused into the element pass
-    finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_PAP, stream_);
+    if (matrix_pass(0) && !oi_) {
+      // the matrix-core element pass has no fused dot
+      apply_level<float>(0, p32_.p, Ap32_.p);
+      dot<float>(n, p32_.p, Ap32_.p, nullptr, nullptr, 1, scal_.p, S_PAP, rs, stream_);
+    } else {
+      apply_level<float>(0, p32_.p, Ap32_.p, partial_.p);  // p.Ap fused into the element pass
+      finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_PAP, stream_);
+    }
     cg_update_xr_mixed(n, x_.p, r, p32_.p, Ap32_.p, scal_.p, S_RZ, S_PAP, S_RR, rs, stream_);
     ++it;
     read_scalar(0);

@@ -17,6 +17,7 @@
 #include "dg_layout.h"
 #include "kernels.h"
 #include "mesh.h"
+#include "oi.h"
 #include "refelem.h"
 #include "vec_kernels.h"
 
@@ -126,6 +127,7 @@ struct CoarseTiming {
 };
 
 class PoissonSolver {
+ friend class VecViscousSolver;  // the vector (velocity) form shares the operator
  public:
   PoissonSolver(HostMesh mesh, int order, hipStream_t stream = nullptr);
   ~PoissonSolver();
@@ -145,6 +147,15 @@ class PoissonSolver {
 
   void set_bc_types(const int* types, int64_t n);
   void set_factor(const double* k, Layout layout);  // nullptr = constant coefficient 1
+  // Over-integration (FactorPoissonMatrixFreeDiagOI::mat_free_set_factor_oi /
+  // mat_free_set_factor_surf_oi): the coefficient at the cubature points of the finest order's
+  // rule (oi.h), cells x nv and cells x Nf*ns; on while both are set (null clears one). The
+  // nodal factor of set_factor still sets the penalty, the boundary terms and the coarser
+  // p-multigrid levels.
+  void set_factor_oi(const double* kv, Layout layout);
+  void set_factor_surf_oi(const double* ks, Layout layout);
+  bool over_integrated() const { return oi_; }
+  const OiRule& oi_rule();
   // Mass term of the viscous operator (FactorMMPoissonMatrixFree::set_mm_factor, per node;
   // MMPoissonMatrixFree::set_factor, constant). nullptr / 0 = none.
   void set_mm_factor(const double* mm, Layout layout);
@@ -218,7 +229,11 @@ class PoissonSolver {
   void compute_diagonals();
   void estimate_eigenvalues();
   template <typename T>
-  void apply_level(int l, const T* u, T* out, double* partial = nullptr);
+  void apply_level(int l, const T* u, T* out, double* partial = nullptr, T* tr = nullptr);
+  // finest level: coefficient and mass-term factor (null when constant), face nodes per cell
+  const double* fine_factor() const;
+  const double* fine_mm() const;
+  int fine_face_nodes() const;
   template <typename T>
   void vcycle(int l);
   template <typename T>
@@ -258,6 +273,16 @@ class PoissonSolver {
   bool have_bc_ = false;
   bool var_factor_ = false;
   DBuf<double> factor_;  // fine-level factor, SoA
+  // over-integration of the finest order: rule, tables, coefficient at the points, traces
+  bool oi_ = false;
+  std::unique_ptr<OiRule> oi_rule_;
+  DBuf<double> oi_C64_, oi_kv64_, oi_ks64_, oi_tr64_;
+  DBuf<float> oi_C32_, oi_kv32_, oi_ks32_, oi_tr32_;
+  DBuf<int> oi_qmap_;
+  template <typename T>
+  OiView<T> oi_view();
+  void ensure_oi_tables();
+  void set_oi_part(const double* k, Layout layout, bool surface);
   bool var_mm_ = false;  // per-node mass-term factor (fine level in mm_, per level in Work::mm)
   DBuf<double> mm_;
   double mm_const_ = 0;  // constant mass-term factor, 0 = none

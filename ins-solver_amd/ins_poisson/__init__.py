@@ -67,6 +67,14 @@ EXPORTED = [
     "ins_batch_release", "ins_batch_stats", "ins_batch_tuning", "ins_batch_setup_seconds", "ins_server_create", "ins_server_destroy",
     "ins_server_submit", "ins_server_poll", "ins_server_wait", "ins_server_wait_all",
     "ins_server_stats", "ins_poisson_last_error",
+    "ins_poisson_set_factor_oi", "ins_poisson_set_factor_surf_oi", "ins_oi_rule", "ins_oi_interp",
+    "ins_viscous_vec_create", "ins_viscous_vec_destroy", "ins_viscous_vec_info",
+    "ins_viscous_vec_set_dat_stride", "ins_viscous_vec_set_bc_types", "ins_viscous_vec_set_factor",
+    "ins_viscous_vec_set_mm_factor", "ins_viscous_vec_set_mm_const", "ins_viscous_vec_set_bcs",
+    "ins_viscous_vec_set_tol_and_iter", "ins_viscous_vec_set_preconditioner",
+    "ins_viscous_vec_set_inv_mass_factor", "ins_viscous_vec_set_inv_mass_recp_factor",
+    "ins_viscous_vec_mult", "ins_viscous_vec_apply_bc", "ins_viscous_vec_diag",
+    "ins_viscous_vec_inv_block_diag", "ins_viscous_vec_solve",
 ]
 
 
@@ -186,6 +194,28 @@ def _load():
         "ins_server_wait": [vp, i64, vp, C.POINTER(SolveStats), dp, dp],
         "ins_server_wait_all": [vp],
         "ins_server_stats": [vp, C.POINTER(i64), C.POINTER(i64)],
+        "ins_poisson_set_factor_oi": [vp, vp, C.c_int],
+        "ins_poisson_set_factor_surf_oi": [vp, vp, C.c_int],
+        "ins_oi_rule": [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, vp, vp, vp],
+        "ins_oi_interp": [C.c_int, C.c_int, vp, vp],
+        "ins_viscous_vec_create": [vp, C.c_int, C.POINTER(vp)],
+        "ins_viscous_vec_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
+        "ins_viscous_vec_set_dat_stride": [vp, i64],
+        "ins_viscous_vec_set_bc_types": [vp, vp, i64],
+        "ins_viscous_vec_set_factor": [vp, vp, C.c_int],
+        "ins_viscous_vec_set_mm_factor": [vp, vp, C.c_int],
+        "ins_viscous_vec_set_mm_const": [vp, dbl],
+        "ins_viscous_vec_set_bcs": [vp, vp, C.c_int],
+        "ins_viscous_vec_set_tol_and_iter": [vp, dbl, dbl, C.c_int],
+        "ins_viscous_vec_set_preconditioner": [vp, C.c_int],
+        "ins_viscous_vec_set_inv_mass_factor": [vp, dbl],
+        "ins_viscous_vec_set_inv_mass_recp_factor": [vp, vp, C.c_int],
+        "ins_viscous_vec_mult": [vp, vp, vp, C.c_int],
+        "ins_viscous_vec_apply_bc": [vp, vp, C.c_int],
+        "ins_viscous_vec_diag": [vp, vp, C.c_int],
+        "ins_viscous_vec_inv_block_diag": [vp, vp],
+        "ins_viscous_vec_solve": [vp, vp, vp, C.c_int, C.POINTER(SolveStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -195,7 +225,7 @@ def _load():
     lib.ins_mesh_destroy.restype = None
     lib.ins_poisson_destroy.argtypes = [vp]
     lib.ins_poisson_destroy.restype = None
-    for name in ("ins_batch_destroy", "ins_server_destroy"):
+    for name in ("ins_batch_destroy", "ins_server_destroy", "ins_viscous_vec_destroy"):
         getattr(lib, name).argtypes = [vp]
         getattr(lib, name).restype = None
     lib.ins_batch_config_default.argtypes = [C.POINTER(BatchConfig)]
@@ -403,6 +433,16 @@ class PoissonSolver:
                                                 _ptr(out[2]) if self.dim == 3 else None,
                                                 LAYOUT_HOST_AOS))
         return out[:self.dim]
+
+    def set_factor_oi(self, kv, layout=LAYOUT_HOST_AOS):
+        """Coefficient at the volume cubature points (cells, n_vol); None clears it."""
+        kv = None if kv is None else (_f64(kv) if layout == LAYOUT_HOST_AOS else kv)
+        _check(lib.ins_poisson_set_factor_oi(self._h, _ptr(kv), layout))
+
+    def set_factor_surf_oi(self, ks, layout=LAYOUT_HOST_AOS):
+        """Coefficient at the face cubature points (cells, Nfaces * n_surf); None clears it."""
+        ks = None if ks is None else (_f64(ks) if layout == LAYOUT_HOST_AOS else ks)
+        _check(lib.ins_poisson_set_factor_surf_oi(self._h, _ptr(ks), layout))
 
     def set_mm_factor(self, mm, layout=LAYOUT_HOST_AOS):
         """Per-node mass-term factor of the viscous operator (cells x Np); None clears."""
@@ -745,3 +785,153 @@ class Server:
         c, p = C.c_int64(), C.c_int64()
         _check(lib.ins_server_stats(self._h, C.byref(c), C.byref(p)))
         return {"completed": c.value, "peak_bytes": p.value}
+
+
+VEC_PRE_NONE = 0
+VEC_PRE_INV_MASS = 1
+VEC_PRE_FACTOR_INV_MASS = 2
+VEC_PRE_RECP_FACTOR_DAT_INV_MASS = 3
+VEC_PRE_JACOBI = 4
+VEC_PRE_BLOCK_JACOBI = 5
+BC_SLIP = 2
+
+
+def _ptr_array(arrs):
+    """C array of component pointers: numpy arrays (host) or integer device addresses."""
+    vals = [a if isinstance(a, int) else (a.ctypes.data if a is not None else 0) for a in arrs]
+    return (C.c_void_p * len(vals))(*vals)
+
+
+class ViscousVecSolver:
+    """Matrix2Vec / Matrix3Vec (ViscousMatrix, FactorViscousMatrix) + ViscousSolver on one GPU:
+    the velocity step's viscous solve, dim components, slip walls (bc type 2 on every
+    component). Host arrays are (dim, cells, Np)."""
+
+    def __init__(self, mesh, order):
+        self.mesh = mesh
+        h = C.c_void_p()
+        _check(lib.ins_viscous_vec_create(mesh._h, order, C.byref(h)))
+        self._h = h
+        self.order = order
+        nc, np_, nfp, ncell, st, nb = (C.c_int(), C.c_int(), C.c_int(), C.c_int64(), C.c_int64(),
+                                      C.c_int64())
+        _check(lib.ins_viscous_vec_info(h, C.byref(nc), C.byref(np_), C.byref(nfp), C.byref(ncell),
+                                        C.byref(st), C.byref(nb)))
+        self.ncomp, self.Np, self.Nfp = nc.value, np_.value, nfp.value
+        self.n_cells, self.stride, self.n_bfaces = ncell.value, st.value, nb.value
+        self.dim = mesh.dim
+        self.last_stats = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ins_viscous_vec_destroy(self._h)
+            self._h = None
+
+    def _host(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.ncomp, self.n_cells, self.Np)
+        return [np.ascontiguousarray(x[c]) for c in range(self.ncomp)]
+
+    def set_dat_stride(self, stride):
+        _check(lib.ins_viscous_vec_set_dat_stride(self._h, int(stride)))
+
+    def set_bc_types(self, bct):
+        t = [np.ascontiguousarray(b, dtype=np.int32) for b in bct]
+        assert len(t) == self.ncomp and all(b.size == self.n_bfaces for b in t)
+        self._bct = t
+        _check(lib.ins_viscous_vec_set_bc_types(self._h, _ptr_array(t), self.n_bfaces))
+
+    def set_factor(self, k, layout=LAYOUT_HOST_AOS):
+        k = None if k is None else (_f64(k) if layout == LAYOUT_HOST_AOS else k)
+        _check(lib.ins_viscous_vec_set_factor(self._h, _ptr(k), layout))
+
+    def set_mm_factor(self, mm, layout=LAYOUT_HOST_AOS):
+        mm = None if mm is None else (_f64(mm) if layout == LAYOUT_HOST_AOS else mm)
+        _check(lib.ins_viscous_vec_set_mm_factor(self._h, _ptr(mm), layout))
+
+    def set_mm_const(self, mm):
+        _check(lib.ins_viscous_vec_set_mm_const(self._h, float(mm)))
+
+    def set_bcs(self, bc):
+        if bc is None:
+            _check(lib.ins_viscous_vec_set_bcs(self._h, None, LAYOUT_HOST_AOS))
+            return
+        b = [np.ascontiguousarray(x, dtype=np.float64) for x in bc]
+        self._bc = b
+        _check(lib.ins_viscous_vec_set_bcs(self._h, _ptr_array(b), LAYOUT_HOST_AOS))
+
+    def set_tol_and_iter(self, rtol, atol, max_iter):
+        _check(lib.ins_viscous_vec_set_tol_and_iter(self._h, rtol, atol, max_iter))
+
+    def set_preconditioner(self, p):
+        _check(lib.ins_viscous_vec_set_preconditioner(self._h, p))
+
+    def set_inv_mass_factor(self, f):
+        _check(lib.ins_viscous_vec_set_inv_mass_factor(self._h, float(f)))
+
+    def set_inv_mass_recp_factor(self, f, layout=LAYOUT_HOST_AOS):
+        f = None if f is None else (_f64(f) if layout == LAYOUT_HOST_AOS else f)
+        _check(lib.ins_viscous_vec_set_inv_mass_recp_factor(self._h, _ptr(f), layout))
+
+    def mult(self, x, layout=LAYOUT_HOST_AOS):
+        """A x; host (dim, cells, Np) arrays, or (in_ptrs, out_ptrs) device component lists."""
+        if layout == LAYOUT_HOST_AOS:
+            xi = self._host(x)
+            out = [np.empty_like(a) for a in xi]
+            _check(lib.ins_viscous_vec_mult(self._h, _ptr_array(xi), _ptr_array(out), layout))
+            return np.stack(out)
+        _check(lib.ins_viscous_vec_mult(self._h, _ptr_array(x[0]), _ptr_array(x[1]), layout))
+        return None
+
+    def diag(self):
+        out = [np.empty((self.n_cells, self.Np)) for _ in range(self.ncomp)]
+        _check(lib.ins_viscous_vec_diag(self._h, _ptr_array(out), LAYOUT_HOST_AOS))
+        return np.stack(out)
+
+    def inv_block_diag(self):
+        out = np.empty((self.ncomp, self.n_cells, self.Np, self.Np))
+        _check(lib.ins_viscous_vec_inv_block_diag(self._h, _ptr(out)))
+        return out
+
+    def apply_bc(self, rhs):
+        """rhs plus the boundary terms (a new array; rhs is not modified)."""
+        r = [a.copy() for a in self._host(rhs)]
+        _check(lib.ins_viscous_vec_apply_bc(self._h, _ptr_array(r), LAYOUT_HOST_AOS))
+        return np.stack(r)
+
+    def solve(self, rhs, ans, layout=LAYOUT_HOST_AOS):
+        """Solve; host: rhs (dim, cells, Np), ans a C-contiguous float64 (dim, cells, Np) array
+        updated in place; device: lists of component addresses. Returns converged."""
+        st = SolveStats()
+        if layout == LAYOUT_HOST_AOS:
+            if not (isinstance(ans, np.ndarray) and ans.dtype == np.float64 and ans.flags.c_contiguous
+                    and ans.size == self.ncomp * self.n_cells * self.Np):
+                raise TypeError("ans must be a C-contiguous float64 (dim, cells, Np) array")
+            a3 = ans.reshape(self.ncomp, self.n_cells, self.Np)
+            r = self._host(rhs)
+            rc = lib.ins_viscous_vec_solve(self._h, _ptr_array(r), _ptr_array([a3[c] for c in range(self.ncomp)]),
+                                           layout, C.byref(st))
+        else:
+            rc = lib.ins_viscous_vec_solve(self._h, _ptr_array(rhs), _ptr_array(ans), layout, C.byref(st))
+        self.last_stats = st.as_dict()
+        _check(rc, allow=(OK, NOT_CONVERGED))
+        return rc == OK
+
+
+def oi_rule(dim, order):
+    """Over-integration rules: (vol_rst (n_vol, dim), vol_w, surf_bary (n_surf, dim), surf_w)."""
+    nv, ns = C.c_int(), C.c_int()
+    _check(lib.ins_oi_rule(dim, order, C.byref(nv), C.byref(ns), None, None, None, None))
+    vx, vw = np.zeros((nv.value, dim)), np.zeros(nv.value)
+    sb, sw = np.zeros((ns.value, dim)), np.zeros(ns.value)
+    _check(lib.ins_oi_rule(dim, order, None, None, _ptr(vx), _ptr(vw), _ptr(sb), _ptr(sw)))
+    return vx, vw, sb, sw
+
+
+def oi_interp(dim, order):
+    """Nodes -> over-integration points: (n_vol, Np), (Nfaces * n_surf, Np)."""
+    vx, _, sb, _ = oi_rule(dim, order)
+    Np = (order + 1) * (order + 2) // 2 if dim == 2 else (order + 1) * (order + 2) * (order + 3) // 6
+    Iv = np.zeros((vx.shape[0], Np))
+    Is = np.zeros(((dim + 1) * sb.shape[0], Np))
+    _check(lib.ins_oi_interp(dim, order, _ptr(Iv), _ptr(Is)))
+    return Iv, Is

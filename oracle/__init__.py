@@ -68,6 +68,17 @@ def _load():
     lib.oracle_set_mm_const.restype = None
     lib.oracle_solve_inv_mass.argtypes = [vp, vp, vp, d, d, ci, d, vp, C.POINTER(ci), C.POINTER(d)]
     lib.oracle_solve_inv_mass.restype = ci
+    lib.oracle_mult_oi.argtypes = [vp, ci, vp, vp, ci, vp, vp, vp, vp, vp, vp]
+    lib.oracle_mult_oi.restype = None
+    lib.oracle_mult_vec.argtypes = [vp, ci, vp, vp, vp]
+    lib.oracle_mult_vec.restype = None
+    lib.oracle_diag_vec.argtypes = [vp, ci, vp, ci, vp]
+    lib.oracle_diag_vec.restype = None
+    lib.oracle_apply_bc_vec.argtypes = [vp, ci, vp, vp, vp]
+    lib.oracle_apply_bc_vec.restype = None
+    lib.oracle_solve_vec.argtypes = [vp, ci, vp, vp, vp, vp, d, d, ci, ci, d, vp, C.POINTER(ci),
+                                     C.POINTER(d)]
+    lib.oracle_solve_vec.restype = ci
     return lib
 
 
@@ -230,6 +241,74 @@ class Oracle:
         """sqrt(e^T (J M) e): the L2 norm of the nodal interpolant."""
         e = self._vec(e)
         return float(np.sqrt(e.ravel() @ self.mass(e).ravel()))
+
+    def mult_oi(self, rule, kv, ks, x):
+        """Over-integrated operator: rule = (vol_rst, vol_w, surf_bary, surf_w) (the library's
+        rules), kv (cells, n_vol), ks (cells, Nfaces * n_surf)."""
+        vx, vw, sb, sw = (np.ascontiguousarray(a, dtype=np.float64) for a in rule)
+        kv = np.ascontiguousarray(kv, dtype=np.float64)
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        assert kv.size == self.n_cells * vw.size and ks.size == self.n_cells * (self.dim + 1) * sw.size
+        x = self._vec(x)
+        y = np.zeros_like(x)
+        lib.oracle_mult_oi(self._h, vw.size, _p(vx), _p(vw), sw.size, _p(sb), _p(sw), _p(kv), _p(ks),
+                           _p(x), _p(y))
+        return y.reshape(self.n_cells, self.Np)
+
+    # ---- vector (velocity) operator with slip walls: dim components, (dim, cells, Np) arrays,
+    # bc types (dim, n_bfaces): 0 Dirichlet, 1 Neumann, 2 slip ----
+    def _vvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.dim * self.n_cells * self.Np
+        return x
+
+    def _vbct(self, bct):
+        t = np.ascontiguousarray(bct, dtype=np.int32)
+        assert t.size == self.dim * self.n_bfaces
+        return t
+
+    def mult_vec(self, bct, x):
+        x = self._vvec(x)
+        y = np.zeros_like(x)
+        lib.oracle_mult_vec(self._h, self.dim, _p(self._vbct(bct)), _p(x), _p(y))
+        return y.reshape(self.dim, self.n_cells, self.Np)
+
+    def diag_vec(self, bct, consistent=True):
+        d = np.zeros((self.dim, self.n_cells, self.Np))
+        lib.oracle_diag_vec(self._h, self.dim, _p(self._vbct(bct)), int(consistent), _p(d))
+        return d
+
+    def apply_bc_vec(self, bct, bc, rhs):
+        bc = np.ascontiguousarray(bc, dtype=np.float64)
+        assert bc.size == self.dim * self.n_bfaces * self.Nfp
+        r = self._vvec(rhs).copy()
+        lib.oracle_apply_bc_vec(self._h, self.dim, _p(self._vbct(bct)), _p(bc), _p(r))
+        return r.reshape(self.dim, self.n_cells, self.Np)
+
+    def solve_vec(self, bct, rhs, bc=None, x0=None, rtol=1e-10, atol=0.0, max_iter=5000, precond=4,
+                  factor=1.0, recp_factor=None):
+        """ViscousSolver: boundary terms of bc added to the rhs, then PCG (precond 0 none,
+        1 inverse mass, 2 inverse mass times factor, 3 inverse mass / recp_factor, 4 Jacobi)."""
+        n = self.dim * self.n_cells * self.Np
+        x = np.zeros(n) if x0 is None else self._vvec(x0).copy()
+        bcv = None if bc is None else np.ascontiguousarray(bc, dtype=np.float64)
+        rf = None if recp_factor is None else self._vec(recp_factor)
+        it, rr = C.c_int(), C.c_double()
+        rc = lib.oracle_solve_vec(self._h, self.dim, _p(self._vbct(bct)), _p(bcv), _p(self._vvec(rhs)), _p(x),
+                                  rtol, atol, max_iter, int(precond), float(factor), _p(rf),
+                                  C.byref(it), C.byref(rr))
+        return x.reshape(self.dim, self.n_cells, self.Np), rc, it.value, rr.value
+
+    def dense_vec(self, bct):
+        """Dense vector operator by unit vectors (tiny meshes only)."""
+        n = self.dim * self.n_cells * self.Np
+        A = np.zeros((n, n))
+        e = np.zeros(n)
+        for j in range(n):
+            e[:] = 0
+            e[j] = 1
+            A[:, j] = self.mult_vec(bct, e).ravel()
+        return A
 
     def dense(self):
         """Dense operator by applying mult to unit vectors (tiny meshes only)."""

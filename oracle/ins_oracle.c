@@ -222,6 +222,8 @@ typedef struct {
   double Mface[4 * MAXNFP * MAXNFP];
   double MF[4][MAXNP * MAXNP]; /* face mass as Np x Np (zero off the face) */
   double EMAT[MAXNP * 4 * MAXNFP];
+  double Vi[MAXNP * MAXNP];  /* inverse monomial Vandermonde: phi_j = sum_k m_k Vi[k][j] */
+  int ex[MAXNP][3];          /* monomial exponents in x = (1 + r) / 2 */
 } refel;
 
 static const int FV2[3][2] = {{0, 1}, {1, 2}, {2, 0}};
@@ -262,6 +264,8 @@ static int build_refel(int dim, int N, refel* R) {
       V[i * Np + k] = pow(x[0], ex[k][0]) * pow(x[1], ex[k][1]) * (dim == 3 ? pow(x[2], ex[k][2]) : 1.0);
   }
   if (inv_dense(V, Np, Vi)) return -1;
+  memcpy(R->Vi, Vi, sizeof(double) * Np * Np);
+  memcpy(R->ex, ex, sizeof(int) * 3 * Np);
   for (int d = 0; d < dim; ++d) {
     for (int i = 0; i < Np; ++i) {
       const double x[3] = {(1 + R->r[i]) / 2, (1 + R->s[i]) / 2, (1 + R->t[i]) / 2};
@@ -752,15 +756,31 @@ static void calc_tau(const oracle_t* o, const refel* R, const int* map, const do
 }
 
 /* FactorViscousMatrix2D::mult with mass factor 0, one component
- * (src/solvers/2d/slip_matrix/factor_viscous_matrix.cpp:87-209; 3D twin). */
-static void apply_op(const oracle_t* o, const refel* R, const int* map, const double* k,
-                     const double* tau, const double* in, double* out) {
+ * (src/solvers/2d/slip_matrix/factor_viscous_matrix.cpp:87-209; 3D twin), in three phases:
+ * gradients and the interior face buffers, the boundary faces, then flux and assembly. */
+typedef struct {
+  double *q, *jump, *avg;  /* q[x][e][i], jump[e][f*Nfp+j], avg[x][e][f*Nfp+j] */
+} op_bufs;
+
+static op_bufs op_bufs_new(const oracle_t* o, const refel* R) {
+  const int NF = R->Nf * R->Nfp;
+  op_bufs b;
+  b.q = malloc(sizeof(double) * o->E * R->Np * o->dim);
+  b.jump = calloc((size_t)o->E * NF, sizeof(double));
+  b.avg = calloc((size_t)o->E * NF * o->dim, sizeof(double));
+  return b;
+}
+
+static void op_bufs_free(op_bufs* b) { free(b->q); free(b->jump); free(b->avg); }
+
+static void op_grad_faces(const oracle_t* o, const refel* R, const int* map, const double* k,
+                          const double* in, op_bufs* B) {
   const int dim = o->dim, Np = R->Np, Nfp = R->Nfp, Nf = R->Nf, NF = Nf * Nfp;
   const int64_t E = o->E;
   const int ng = dim * dim + 1;
-  double* q = malloc(sizeof(double) * E * Np * dim);   /* q[x][e][i] */
-  double* jump = calloc((size_t)E * NF, sizeof(double));
-  double* avg = calloc((size_t)E * NF * dim, sizeof(double));
+  double* q = B->q;
+  double* jump = B->jump;
+  double* avg = B->avg;
   /* op2_gemv DR/DS(/DT) + fvmf_2d_grad: q_x = k (rx ur + sx us + tx ut) */
   OMP_FOR
   for (int64_t e = 0; e < E; ++e) {
@@ -800,7 +820,15 @@ static void apply_op(const oracle_t* o, const refel* R, const int* map, const do
       }
     }
   }
-  /* vmf_2d_mult_avg_jump.h:29-39 (Dirichlet ghost), Neumann: nothing */
+}
+
+/* vmf_2d_mult_avg_jump.h:29-39 (Dirichlet ghost), Neumann: nothing */
+static void op_bc_scalar(const oracle_t* o, const refel* R, const double* in, op_bufs* B) {
+  const int dim = o->dim, Np = R->Np, Nfp = R->Nfp, NF = R->Nf * Nfp;
+  const int64_t E = o->E;
+  const double* q = B->q;
+  double* jump = B->jump;
+  double* avg = B->avg;
   for (int64_t b = 0; b < o->B; ++b) {
     if (o->bct[b] != 0) continue;
     const int c = o->b2c[b], f = o->bnum[b];
@@ -810,6 +838,16 @@ static void apply_op(const oracle_t* o, const refel* R, const int* map, const do
       for (int x = 0; x < dim; ++x) avg[((size_t)x * E + c) * NF + f * Nfp + j] += q[((size_t)x * E + c) * Np + i];
     }
   }
+}
+
+static void op_flux_assemble(const oracle_t* o, const refel* R, const double* k, const double* tau,
+                             op_bufs* B, double* out) {
+  const int dim = o->dim, Np = R->Np, Nfp = R->Nfp, Nf = R->Nf, NF = Nf * Nfp;
+  const int64_t E = o->E;
+  const int ng = dim * dim + 1;
+  const double* q = B->q;
+  double* jump = B->jump;
+  double* avg = B->avg;
   /* fvmf_2d_mult_flux.h:5-16 */
   OMP_FOR
   for (int64_t e = 0; e < E; ++e) {
@@ -860,7 +898,15 @@ static void apply_op(const oracle_t* o, const refel* R, const int* map, const do
       oe[i] += a;
     }
   }
-  free(q); free(jump); free(avg);
+}
+
+static void apply_op(const oracle_t* o, const refel* R, const int* map, const double* k,
+                     const double* tau, const double* in, double* out) {
+  op_bufs B = op_bufs_new(o, R);
+  op_grad_faces(o, R, map, k, in, &B);
+  op_bc_scalar(o, R, in, &B);
+  op_flux_assemble(o, R, k, tau, &B, out);
+  op_bufs_free(&B);
 }
 
 /* Mass term of the viscous (Helmholtz) operator on the finest order:
@@ -909,8 +955,18 @@ static void normal_dmat(const oracle_t* o, const refel* R, int64_t e, int f, con
 }
 
 /* fvmf_2d_op1_diag.h + fvmf_2d_op2_diag.h + fvmf_2d_bop_diag.h (3D twins). */
+/* bct: bc type per bface (NULL: the oracle's own); nrm_comp >= 0: the diagonal of that
+ * component of the vector operator, whose faces of type 2 are slip faces weighted by
+ * n_comp^2 (fvmf_2d_bop_diag.h:24-60). */
+static void calc_diag_bc(const oracle_t* o, const refel* R, const double* k, const double* tau,
+                         int consistent, const int* bct, int nrm_comp, double* diag);
 static void calc_diag(const oracle_t* o, const refel* R, const double* k, const double* tau,
                       int consistent, double* diag) {
+  calc_diag_bc(o, R, k, tau, consistent, NULL, -1, diag);
+}
+static void calc_diag_bc(const oracle_t* o, const refel* R, const double* k, const double* tau,
+                         int consistent, const int* bct, int nrm_comp, double* diag) {
+  if (!bct) bct = o->bct;
   const int dim = o->dim, Np = R->Np, Nf = R->Nf, ng = dim * dim + 1;
   const int64_t E = o->E;
   OMP_FOR
@@ -944,8 +1000,13 @@ static void calc_diag(const oracle_t* o, const refel* R, const double* k, const 
       /* find the face kind for this cell face */
       const int bf = o->cfb[e * Nf + f];
       const int is_b = bf >= 0;
-      if (is_b && o->bct[bf] != 0) continue;
-      scale = is_b ? 1.0 : 0.5;
+      double w = 1.0;
+      if (is_b && bct[bf] != 0) {
+        if (!(nrm_comp >= 0 && bct[bf] == 2)) continue;
+        const double nc = o->nrm[(e * Nf + f) * dim + nrm_comp];
+        w = nc * nc;
+      }
+      scale = (is_b ? 1.0 : 0.5) * w;
       double Dn[MAXNP * MAXNP];
       normal_dmat(o, R, e, f, k, Dn);
       const double* MF = R->MF[f];
@@ -987,8 +1048,10 @@ void oracle_diag(oracle_t* o, int consistent, double* out) {
   free(tau);
 }
 
+static void apply_bc_bct(oracle_t* o, const int* bct, const double* bc, double* rhs);
 /* fvmf_2d_apply_bc.h:24-76 (vmf_2d_apply_bc.h when the factor is 1) */
-void oracle_apply_bc(oracle_t* o, const double* bc, double* rhs) {
+void oracle_apply_bc(oracle_t* o, const double* bc, double* rhs) { apply_bc_bct(o, o->bct, bc, rhs); }
+static void apply_bc_bct(oracle_t* o, const int* bct, const double* bc, double* rhs) {
   const refel* R = &o->R;
   const int Np = R->Np, Nfp = R->Nfp, Nf = R->Nf, N = R->N;
   for (int64_t b = 0; b < o->B; ++b) {
@@ -997,7 +1060,7 @@ void oracle_apply_bc(oracle_t* o, const double* bc, double* rhs) {
     const double sJ = o->sJ[c * Nf + f];
     const double* g = bc + b * Nfp;
     double* rh = rhs + (size_t)c * Np;
-    if (o->bct[b] == 0) {
+    if (bct[b] == 0) {
       double Dn[MAXNP * MAXNP];
       normal_dmat(o, R, c, f, o->k, Dn);
       double gtau = o->k ? o->k[(size_t)c * Np + fm[0]] : 1.0;
@@ -1014,7 +1077,7 @@ void oracle_apply_bc(oracle_t* o, const double* bc, double* rhs) {
       }
       for (int i = 0; i < Np; ++i)
         for (int j = 0; j < Np; ++j) rh[i] += -Dn[j * Np + i] * tmp[j];
-    } else if (o->bct[b] == 1) {
+    } else if (bct[b] == 1) {
       for (int i = 0; i < Np; ++i)
         for (int j = 0; j < Nfp; ++j) rh[i] += sJ * R->MF[f][i * Np + fm[j]] * g[j];
     }
@@ -1236,6 +1299,384 @@ int oracle_solve_inv_mass(oracle_t* o, const double* rhs, double* x, double rtol
                      iters, relres);
   free(tau); free(Minv);
   return rc;
+}
+
+/* ---- Vector (velocity) operator with slip walls ----
+ * Matrix2Vec / Matrix3Vec: ViscousMatrix2D::mult (src/solvers/2d/slip_matrix/viscous_matrix.cpp:67-178)
+ * and FactorViscousMatrix2D::mult (factor_viscous_matrix.cpp:87-209), 3D twins in
+ * src/solvers/3d/slip_matrix. Vectors are nc x cells x Np; bct is nc x bfaces. */
+
+/* vmf_2d_mult_avg_jump.h:1-51 / vmf_3d_mult_avg_jump.h: a face whose every component has type 2
+ * adds the projections n_a (n.U) (twice, to the jump) and n_a (n.dU/dx) to the averages of
+ * each component a; otherwise each component of type 0 takes the Dirichlet ghost and the
+ * others nothing. */
+static void op_bc_vec(const oracle_t* o, const refel* R, int nc, const int* bct, const double* in,
+                      op_bufs* B) {
+  const int dim = o->dim, Np = R->Np, Nfp = R->Nfp, Nf = R->Nf, NF = Nf * Nfp;
+  const int64_t E = o->E, NB = o->B;
+  for (int64_t b = 0; b < NB; ++b) {
+    const int c = o->b2c[b], f = o->bnum[b];
+    int all2 = 1;
+    for (int a = 0; a < nc; ++a) all2 = all2 && bct[a * NB + b] == 2;
+    if (all2) {
+      const double* n = o->nrm + ((size_t)c * Nf + f) * dim;
+      for (int j = 0; j < Nfp; ++j) {
+        const int i = R->fmask[f * Nfp + j];
+        const size_t ind = (size_t)c * NF + f * Nfp + j;
+        double nu = 0, nq[3] = {0, 0, 0};
+        for (int d = 0; d < nc; ++d) {
+          nu += n[d] * in[((size_t)d * E + c) * Np + i];
+          for (int x = 0; x < dim; ++x) nq[x] += n[d] * B[d].q[((size_t)x * E + c) * Np + i];
+        }
+        for (int a = 0; a < nc; ++a) {
+          B[a].jump[ind] += 2.0 * n[a] * nu;
+          for (int x = 0; x < dim; ++x) B[a].avg[(size_t)x * E * NF + ind] += n[a] * nq[x];
+        }
+      }
+      continue;
+    }
+    for (int a = 0; a < nc; ++a) {
+      if (bct[a * NB + b] != 0) continue;
+      for (int j = 0; j < Nfp; ++j) {
+        const int i = R->fmask[f * Nfp + j];
+        const size_t ind = (size_t)c * NF + f * Nfp + j;
+        B[a].jump[ind] += 2.0 * in[((size_t)a * E + c) * Np + i];
+        for (int x = 0; x < dim; ++x) B[a].avg[(size_t)x * E * NF + ind] += B[a].q[((size_t)x * E + c) * Np + i];
+      }
+    }
+  }
+}
+
+static void apply_vec(const oracle_t* o, const double* tau, int nc, const int* bct, const double* in,
+                      double* out) {
+  const refel* R = &o->R;
+  const size_t cs = (size_t)o->E * R->Np;
+  op_bufs B[3];
+  for (int a = 0; a < nc; ++a) {
+    B[a] = op_bufs_new(o, R);
+    op_grad_faces(o, R, o->mapLpos, o->k, in + a * cs, &B[a]);
+  }
+  op_bc_vec(o, R, nc, bct, in, B);
+  for (int a = 0; a < nc; ++a) {
+    op_flux_assemble(o, R, o->k, tau, &B[a], out + a * cs);
+    mass_term(o, in + a * cs, out + a * cs);
+    op_bufs_free(&B[a]);
+  }
+}
+
+/* Per-component bc types as the scalar kernels see them: 2 stays only where every component
+ * is 2 (a slip wall); a lone 2 becomes 3 (no terms, vmf_2d_apply_bc.h:70-74,122-126). */
+static int* vec_types(const oracle_t* o, int nc, const int* bct) {
+  const int64_t NB = o->B;
+  int* t = malloc(sizeof(int) * (nc * NB + 1));
+  for (int64_t b = 0; b < NB; ++b) {
+    int all2 = 1;
+    for (int a = 0; a < nc; ++a) all2 = all2 && bct[a * NB + b] == 2;
+    for (int a = 0; a < nc; ++a) {
+      const int v = bct[a * NB + b];
+      t[a * NB + b] = v == 2 ? (all2 ? 2 : 3) : v;
+    }
+  }
+  return t;
+}
+
+void oracle_mult_vec(oracle_t* o, int nc, const int* bct, const double* in, double* out) {
+  double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
+  apply_vec(o, tau, nc, bct, in, out);
+  free(tau);
+}
+
+/* FactorViscousMatrix2D::calc_diag (factor_viscous_matrix.cpp:211-256): the scalar diagonal of
+ * each component with its own bc types, slip faces weighted by n_a^2 (fvmf_2d_bop_diag.h:24-60). */
+void oracle_diag_vec(oracle_t* o, int nc, const int* bct, int consistent, double* out) {
+  double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
+  int* t = vec_types(o, nc, bct);
+  const size_t cs = (size_t)o->E * o->R.Np;
+  for (int a = 0; a < nc; ++a) {
+    calc_diag_bc(o, &o->R, o->k, tau, consistent, t + a * o->B, a, out + a * cs);
+    mass_diag(o, out + a * cs);
+  }
+  free(t);
+  free(tau);
+}
+
+/* ViscousMatrix2D::apply_bc (viscous_matrix.cpp:26-43 with vmf_2d_apply_bc.h): each component's
+ * Dirichlet or Neumann terms; slip faces add nothing. bc is nc x bfaces x Nfp. */
+void oracle_apply_bc_vec(oracle_t* o, int nc, const int* bct, const double* bc, double* rhs) {
+  int* t = vec_types(o, nc, bct);
+  const size_t cs = (size_t)o->E * o->R.Np, bs = (size_t)o->B * o->R.Nfp;
+  for (int a = 0; a < nc; ++a) apply_bc_bct(o, t + a * o->B, bc + a * bs, rhs + a * cs);
+  free(t);
+}
+
+/* ViscousSolver2D::preconditioned_conjugate_gradient (viscous_solver.cpp:158-266; with no
+ * preconditioner z = r, conjugate_gradient :85-156) over all components at once, with the
+ * flexible beta and the true residual as the stopping test, as the scalar solves here. */
+typedef struct { const oracle_t* o; const double* tau; int nc; const int* bct; } vapply_ctx;
+typedef struct { int kind; int nc; im_ctx im; const double* inv; int64_t n; } vpre_ctx;
+static void vpre(void* vc, const double* r, double* z) {
+  const vpre_ctx* c = vc;
+  const size_t cs = (size_t)c->im.o->E * c->im.o->R.Np;
+  if (c->kind == 0) {
+    memcpy(z, r, sizeof(double) * c->n);
+  } else if (c->kind == 4) {
+    for (int64_t i = 0; i < c->n; ++i) z[i] = c->inv[i] * r[i];
+  } else {
+    for (int a = 0; a < c->nc; ++a) im_precond((void*)&c->im, r + a * cs, z + a * cs);
+  }
+}
+
+int oracle_solve_vec(oracle_t* o, int nc, const int* bct, const double* bc, const double* rhs, double* x,
+                     double rtol, double atol, int max_iter, int precond, double inv_mass_factor,
+                     const double* recp_factor, int* iters, double* relres) {
+  const int Np = o->R.Np;
+  const int64_t n = (int64_t)nc * o->E * Np;
+  double* tau = ref_tau(o, &o->R, o->mapLpos, o->k);
+  double* b = malloc(sizeof(double) * n);
+  memcpy(b, rhs, sizeof(double) * n);
+  if (bc) oracle_apply_bc_vec(o, nc, bct, bc, b);
+  double* Minv = malloc(sizeof(double) * Np * Np);
+  if (inv_dense(o->R.M, Np, Minv) != 0) { free(tau); free(b); free(Minv); return -1; }
+  double* inv = NULL;
+  if (precond == 4) {
+    inv = malloc(sizeof(double) * n);
+    oracle_diag_vec(o, nc, bct, 1, inv);
+    for (int64_t i = 0; i < n; ++i) inv[i] = 1.0 / inv[i];
+  }
+  vpre_ctx pc = {precond, nc, {o, Minv, precond == 2 ? inv_mass_factor : 1.0, precond == 3 ? recp_factor : NULL},
+                 inv, n};
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n), *zo = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  const double bn = sqrt(vdot(n, b, b));
+  const double tol = fmax(rtol * bn, atol);
+  int it = 0, status = 1;
+  double rr = 0;
+  for (int restart = 0; restart < 50; ++restart) {
+    apply_vec(o, tau, nc, bct, x, Ap);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - Ap[i];
+    rr = vdot(n, r, r);
+    if (!isfinite(rr)) { status = -3; break; }
+    if (sqrt(rr) <= tol) { status = 0; break; }
+    if (it >= max_iter) break;
+    vpre(&pc, r, z);
+    double rz = vdot(n, r, z);
+    memcpy(p, z, sizeof(double) * n);
+    int bd = 0;
+    while (it < max_iter) {
+      apply_vec(o, tau, nc, bct, p, Ap);
+      const double pap = vdot(n, p, Ap);
+      if (!(pap > 0)) { bd = 1; break; }
+      const double alpha = rz / pap;
+      for (int64_t i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+      ++it;
+      rr = vdot(n, r, r);
+      if (!isfinite(rr)) { bd = 1; break; }
+      if (sqrt(rr) <= tol) break;
+      double* t = zo; zo = z; z = t;
+      vpre(&pc, r, z);
+      const double rzn = vdot(n, r, z);
+      double bnum = 0;
+      for (int64_t i = 0; i < n; ++i) bnum += r[i] * (z[i] - zo[i]);
+      const double beta = rz != 0 ? bnum / rz : 0;
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+      rz = rzn;
+    }
+    if (bd) { status = -3; break; }
+  }
+  if (iters) *iters = it;
+  if (relres) *relres = bn > 0 ? sqrt(rr) / bn : 0;
+  free(b); free(r); free(z); free(zo); free(p); free(Ap); free(Minv); free(inv); free(tau);
+  return status;
+}
+
+/* ---- Over-integrated variable-coefficient operator ----
+ * FactorPoissonMatrixFreeDiagOI{2D,3D} (toolkit, not vendored; selected at
+ * src/solvers/2d/mp_ins_solver.cpp:98-100, factors set at :580-600): the SIPG form of the
+ * nodal operator with every integral taken by cubature, the coefficient given at the points:
+ *   volume   sum_q w_q J k_q grad u . grad phi_i
+ *   faces    sum_q w_q sJ [ (tau/2 [u] - {k grad u}.n) phi_i - k^-_q [u]/2 n.grad phi_i ]
+ * tau from the nodal factor (fvmf_2d_calc_tau_faces.h:9); Dirichlet ghost as
+ * vmf_2d_mult_avg_jump.h:32-38. The points are taken from the caller (the library's rules) and
+ * matched across faces by their physical coordinates. */
+
+/* nodal basis (values, reference gradients) of R at a reference point */
+static void basis_eval(const refel* R, const double* xr, double* phi, double (*dphi)[MAXNP]) {
+  const int dim = R->dim, Np = R->Np;
+  double m[MAXNP], dm[3][MAXNP];
+  const double x[3] = {(1 + xr[0]) / 2, (1 + xr[1]) / 2, dim == 3 ? (1 + xr[2]) / 2 : 0};
+  for (int k = 0; k < Np; ++k) {
+    const int* e = R->ex[k];
+    m[k] = pow(x[0], e[0]) * pow(x[1], e[1]) * (dim == 3 ? pow(x[2], e[2]) : 1.0);
+    for (int d = 0; d < dim; ++d) {
+      int f[3] = {e[0], e[1], e[2]};
+      double v = 0;
+      if (f[d] > 0) {
+        v = 0.5 * f[d];
+        f[d] -= 1;
+        v *= pow(x[0], f[0]) * pow(x[1], f[1]) * (dim == 3 ? pow(x[2], f[2]) : 1.0);
+      }
+      dm[d][k] = v;
+    }
+  }
+  for (int j = 0; j < Np; ++j) {
+    double a = 0;
+    for (int k = 0; k < Np; ++k) a += m[k] * R->Vi[k * Np + j];
+    phi[j] = a;
+    for (int d = 0; d < dim; ++d) {
+      double b = 0;
+      for (int k = 0; k < Np; ++k) b += dm[d][k] * R->Vi[k * Np + j];
+      dphi[d][j] = b;
+    }
+  }
+}
+
+static const double RV2[3][2] = {{-1, -1}, {1, -1}, {-1, 1}};
+static const double RV3[4][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+
+/* reference and physical coordinates of face point q of face f of cell e */
+static void face_point(const oracle_t* o, int64_t e, int f, const double* b, double* xr, double* xp) {
+  const int dim = o->dim;
+  for (int x = 0; x < dim; ++x) { xr[x] = 0; xp[x] = 0; }
+  for (int k = 0; k < dim; ++k) {
+    const int v = dim == 2 ? FV2[f][k] : FV3[f][k];
+    const int node = o->c2n[e * (dim + 1) + v];
+    for (int x = 0; x < dim; ++x) {
+      xr[x] += b[k] * (dim == 2 ? RV2[v][x] : RV3[v][x]);
+      xp[x] += b[k] * o->coords[(size_t)node * dim + x];
+    }
+  }
+}
+
+void oracle_mult_oi(oracle_t* o, int nv, const double* vrst, const double* vw, int ns, const double* sbary,
+                    const double* sw, const double* kv, const double* ks, const double* in, double* out) {
+  const refel* R = &o->R;
+  const int dim = o->dim, Np = R->Np, Nf = R->Nf, ng = dim * dim + 1;
+  const int64_t E = o->E;
+  double* tau = ref_tau(o, R, o->mapLpos, o->k);
+  /* physical gradient of u and of every basis function at a reference point of cell e */
+  #define OI_GRADS(e, xr, phi, gphys, gu)                                        \
+    do {                                                                         \
+      double dref_[3][MAXNP];                                                    \
+      basis_eval(R, xr, phi, dref_);                                             \
+      const double* g_ = o->geof + (e) * ng;                                     \
+      for (int x = 0; x < dim; ++x) {                                            \
+        (gu)[x] = 0;                                                             \
+        for (int j = 0; j < Np; ++j) {                                           \
+          double a = 0;                                                          \
+          for (int r = 0; r < dim; ++r) a += g_[r * dim + x] * dref_[r][j];      \
+          (gphys)[x][j] = a;                                                     \
+          (gu)[x] += a * in[(e) * Np + j];                                       \
+        }                                                                        \
+      }                                                                          \
+    } while (0)
+  /* volume */
+  OMP_FOR
+  for (int64_t e = 0; e < E; ++e) {
+    const double J = o->geof[e * ng + dim * dim];
+    double* oe = out + e * Np;
+    for (int i = 0; i < Np; ++i) oe[i] = 0;
+    for (int q = 0; q < nv; ++q) {
+      double phi[MAXNP], gp[3][MAXNP], gu[3];
+      OI_GRADS(e, vrst + (size_t)q * dim, phi, gp, gu);
+      const double c = vw[q] * J * kv[e * nv + q];
+      for (int i = 0; i < Np; ++i) {
+        double a = 0;
+        for (int x = 0; x < dim; ++x) a += gp[x][i] * gu[x];
+        oe[i] += c * a;
+      }
+    }
+  }
+  /* faces: each cell face in turn, the neighbour's value at the physically matching point */
+  OMP_FOR
+  for (int64_t e = 0; e < E; ++e) {
+    double* oe = out + e * Np;
+    for (int f = 0; f < Nf; ++f) {
+      const int bf = o->cfb[e * Nf + f];
+      int64_t en = -1;
+      int fnn = -1;
+      if (bf < 0) {
+        /* interior: find the face record */
+        for (int64_t i = 0; i < o->F && en < 0; ++i) {
+          if (o->f2c[2 * i] == e && o->fnum[2 * i] == f) { en = o->f2c[2 * i + 1]; fnn = o->fnum[2 * i + 1]; }
+          else if (o->f2c[2 * i + 1] == e && o->fnum[2 * i + 1] == f) { en = o->f2c[2 * i]; fnn = o->fnum[2 * i]; }
+        }
+        if (en < 0) continue;
+      } else if (o->bct[bf] != 0) {
+        continue; /* Neumann: nothing */
+      }
+      const double* n = o->nrm + (e * Nf + f) * dim;
+      const double sJ = o->sJ[e * Nf + f], t = tau[e * Nf + f];
+      /* the neighbour's points (periodic faces: matched up to the translation of the face) */
+      double shift[3] = {0, 0, 0};
+      if (en >= 0) {
+        double cm[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) {
+          const int vm = dim == 2 ? FV2[f][k] : FV3[f][k], vn = dim == 2 ? FV2[fnn][k] : FV3[fnn][k];
+          for (int x = 0; x < dim; ++x) {
+            cm[x] += o->coords[(size_t)o->c2n[e * (dim + 1) + vm] * dim + x] / dim;
+            cn[x] += o->coords[(size_t)o->c2n[en * (dim + 1) + vn] * dim + x] / dim;
+          }
+        }
+        for (int x = 0; x < dim; ++x) shift[x] = cn[x] - cm[x];
+      }
+      for (int q = 0; q < ns; ++q) {
+        double xr[3], xp[3], phi[MAXNP], gp[3][MAXNP], gu[3];
+        face_point(o, e, f, sbary + (size_t)q * dim, xr, xp);
+        OI_GRADS(e, xr, phi, gp, gu);
+        double um = 0;
+        for (int j = 0; j < Np; ++j) um += phi[j] * in[e * Np + j];
+        const double km = ks[(e * Nf + f) * ns + q];
+        double dnum = 0;
+        for (int x = 0; x < dim; ++x) dnum += n[x] * gu[x];
+        double jump, avg;  /* [u] and {k grad u}.n */
+        if (en >= 0) {
+          int qn = -1;
+          double best = 1e300;
+          for (int p = 0; p < ns; ++p) {
+            double yr[3], yp[3], d2 = 0;
+            face_point(o, en, fnn, sbary + (size_t)p * dim, yr, yp);
+            for (int x = 0; x < dim; ++x) d2 += (yp[x] - shift[x] - xp[x]) * (yp[x] - shift[x] - xp[x]);
+            if (d2 < best) { best = d2; qn = p; }
+          }
+          double yr[3], yp[3], phn[MAXNP], gun[3];
+          face_point(o, en, fnn, sbary + (size_t)qn * dim, yr, yp);
+          {
+            const int64_t ee = en;
+            double dref_[3][MAXNP];
+            basis_eval(R, yr, phn, dref_);
+            const double* g_ = o->geof + ee * ng;
+            for (int x = 0; x < dim; ++x) {
+              gun[x] = 0;
+              for (int j = 0; j < Np; ++j) {
+                double a = 0;
+                for (int r = 0; r < dim; ++r) a += g_[r * dim + x] * dref_[r][j];
+                gun[x] += a * in[ee * Np + j];
+              }
+            }
+          }
+          double up = 0;
+          for (int j = 0; j < Np; ++j) up += phn[j] * in[en * Np + j];
+          const double kp = ks[(en * Nf + fnn) * ns + qn];
+          double dnup = 0;
+          for (int x = 0; x < dim; ++x) dnup += n[x] * gun[x];
+          jump = um - up;
+          avg = 0.5 * (km * dnum + kp * dnup);
+        } else {
+          jump = 2 * um;
+          avg = km * dnum;
+        }
+        const double w = sw[q] * sJ;
+        for (int i = 0; i < Np; ++i) {
+          double dphi_n = 0;
+          for (int x = 0; x < dim; ++x) dphi_n += n[x] * gp[x][i];
+          oe[i] += w * ((0.5 * t * jump - avg) * phi[i] - 0.5 * km * jump * dphi_n);
+        }
+      }
+    }
+  }
+  #undef OI_GRADS
+  mass_term(o, in, out);
+  free(tau);
 }
 
 /* ---- p-multigrid ---- */

@@ -68,6 +68,27 @@ void oracle_grad_central(oracle_t* o, const double* p, double* gx, double* gy, d
 /* Mass-weighted squared L2 norm, sum_e 1^T J M (e o e) (l2_error_vortex.cpp:25-44). */
 double oracle_l2sq(oracle_t* o, const double* e);
 
+/* Vector (velocity) operator with slip walls (Matrix2Vec / Matrix3Vec: ViscousMatrix2D,
+ * FactorViscousMatrix2D and their 3D twins). nc = dim components; vectors nc x cells x Np;
+ * bct nc x bfaces: 0 Dirichlet, 1 Neumann, 2 slip (a wall where every component is 2). */
+void oracle_mult_vec(oracle_t* o, int nc, const int* bct, const double* in, double* out);
+void oracle_diag_vec(oracle_t* o, int nc, const int* bct, int consistent, double* out);
+void oracle_apply_bc_vec(oracle_t* o, int nc, const int* bct, const double* bc, double* rhs);
+/* ViscousSolver2D::solve: rhs plus boundary terms of bc (nc x bfaces x Nfp, or NULL), then PCG
+ * with precond 0 none, 1 inverse mass, 2 inverse mass times inv_mass_factor, 3 inverse mass
+ * divided by recp_factor (cells x Np), 4 Jacobi. x IN/OUT; returns as oracle_solve_jacobi. */
+int oracle_solve_vec(oracle_t* o, int nc, const int* bct, const double* bc, const double* rhs, double* x,
+                     double rtol, double atol, int max_iter, int precond, double inv_mass_factor,
+                     const double* recp_factor, int* iters, double* relres);
+
+/* Over-integrated variable-coefficient operator (FactorPoissonMatrixFreeDiagOI): out = A in with
+ * every integral taken by the given rules (nv volume points in reference coordinates and
+ * weights; ns points per face as barycentrics of the face's vertices and weights summing to 2)
+ * and the coefficient at the points: kv cells x nv, ks cells x Nf x ns. The penalty takes the
+ * nodal factor of oracle_set_factor; the mass term is added as in oracle_mult. */
+void oracle_mult_oi(oracle_t* o, int nv, const double* vrst, const double* vw, int ns, const double* sbary,
+                    const double* sw, const double* kv, const double* ks, const double* in, double* out);
+
 /* Flexible PCG with Jacobi preconditioner (viscous_solver.cpp:158-266); converged
  * when |b - A x| <= max(rtol |b|, atol) for the true residual. x is IN/OUT.
  * Returns 0 converged, 1 not converged, -3 breakdown. */

@@ -107,7 +107,7 @@ def simplex_measure(V):
 
 
 def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.0, sym_scale=1.0,
-             periods=()):
+             periods=(), bface_blocks=None, k_fn=None):
     """Dense SIPG matrix (E*Np)^2 in the cells x Np node order of `xyz`.
 
     coords (nn, dim), c2n (E, dim+1), xyz (dim, E, Np) nodal points;
@@ -115,7 +115,12 @@ def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.
     k (E, Np) nodal coefficient or None (k = 1); tau_scale and sym_scale multiply the
     penalty and the symmetry term (1 for the reference's form; other values let a test
     show that a slip in either is detected); periods: translation vectors that join
-    boundary faces into periodic faces (a face and its translate are one interior face)."""
+    boundary faces into periodic faces (a face and its translate are one interior face);
+    bface_blocks: a list that receives (cell, outward unit normal, face vertices, Np x Np block
+    of the Dirichlet face terms) for every boundary face, whatever its type;
+    k_fn: a coefficient k(x) (points (Q, dim) -> (Q,)) integrated exactly, in place of the nodal
+    interpolants Pi(k grad u) and Pi_F(k [u]) (the over-integrated form); the penalty still
+    takes the nodal k."""
     dim = coords.shape[1]
     E, Np = xyz.shape[1], xyz.shape[2]
     nq = nq or N + 3
@@ -144,7 +149,10 @@ def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.
         L = bases[e].val(xq)
         dL = bases[e].grad(xq)
         for d in range(dim):
-            A[blk(e, e)] += dL[d].T @ (w[:, None] * (L @ (kk[e][:, None] * G[e][d])))
+            if k_fn is None:
+                A[blk(e, e)] += dL[d].T @ (w[:, None] * (L @ (kk[e][:, None] * G[e][d])))
+            else:
+                A[blk(e, e)] += dL[d].T @ ((w * k_fn(xq))[:, None] * dL[d])
 
     faces = {}
     for e in range(E):
@@ -211,30 +219,107 @@ def assemble(coords, c2n, xyz, N, bc_type_of=None, k=None, nq=None, tau_scale=1.
                 n = outward_normal(coords[c2n[e]], FVe)
                 Le, Lo = bases[e].val(xq), bases[o].val(xq + t)
                 ndL = np.tensordot(n, bases[e].grad(xq), axes=1)  # (Q, Np)
-                qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
-                qo = sum(n[d] * (Lo @ (kk[o][:, None] * G[o][d])) for d in range(dim))
+                if k_fn is None:
+                    qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
+                    qo = sum(n[d] * (Lo @ (kk[o][:, None] * G[o][d])) for d in range(dim))
+                else:
+                    kq = k_fn(xq)[:, None]
+                    qe = kq * ndL
+                    qo = kq * np.tensordot(n, bases[o].grad(xq + t), axes=1)
                 A[blk(e, e)] += Le.T @ (w[:, None] * (0.5 * tau * Le - 0.5 * qe))
                 A[blk(e, o)] += Le.T @ (w[:, None] * (-0.5 * tau * Lo - 0.5 * qo))
-                # symmetric term: Pi_F(-1/2 k_e (u_e - u_o)) through e's face nodes
-                S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
-                Ie = np.eye(Np)[fn[e]]
-                Uo = bases[o].val(nodes[e][fn[e]] + t)  # u_o at e's face nodes
-                A[blk(e, e)] += -0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Ie))
-                A[blk(e, o)] += 0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Uo))
+                if k_fn is None:
+                    # symmetric term: Pi_F(-1/2 k_e (u_e - u_o)) through e's face nodes
+                    S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
+                    Ie = np.eye(Np)[fn[e]]
+                    Uo = bases[o].val(nodes[e][fn[e]] + t)  # u_o at e's face nodes
+                    A[blk(e, e)] += -0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Ie))
+                    A[blk(e, o)] += 0.5 * sym_scale * ndL.T @ (w[:, None] * (S @ Uo))
+                else:
+                    kw = (w * k_fn(xq))[:, None]
+                    A[blk(e, e)] += -0.5 * sym_scale * ndL.T @ (kw * Le)
+                    A[blk(e, o)] += 0.5 * sym_scale * ndL.T @ (kw * Lo)
         else:
             e = cells[0]
             xq = FV[0] + qf @ (FV[1:] - FV[0])
             w = wf * area * math.factorial(dim - 1)
             centroid = FV.mean(axis=0)
             bt = 0 if bc_type_of is None else bc_type_of(centroid)
-            if bt != 0:
+            if bt != 0 and bface_blocks is None:
                 continue
             tau = pen * fs[e] * kmax[e]
             n = outward_normal(coords[c2n[e]], FV)
             Le = bases[e].val(xq)
             ndL = np.tensordot(n, bases[e].grad(xq), axes=1)
-            qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
-            A[blk(e, e)] += Le.T @ (w[:, None] * (tau * Le - qe))
-            S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
-            A[blk(e, e)] += -sym_scale * ndL.T @ (w[:, None] * (S @ np.eye(Np)[fn[e]]))
+            if k_fn is None:
+                qe = sum(n[d] * (Le @ (kk[e][:, None] * G[e][d])) for d in range(dim))
+                S = Le[:, fn[e]] * kk[e][fn[e]][None, :]
+                Db = (Le.T @ (w[:, None] * (tau * Le - qe))
+                      - sym_scale * ndL.T @ (w[:, None] * (S @ np.eye(Np)[fn[e]])))
+            else:
+                kq = k_fn(xq)[:, None]
+                Db = (Le.T @ (w[:, None] * (tau * Le - kq * ndL))
+                      - sym_scale * ndL.T @ ((w[:, None] * kq) * Le))
+            if bface_blocks is not None:
+                bface_blocks.append((e, n, FV, Db))
+            if bt == 0:
+                A[blk(e, e)] += Db
+    return A
+
+
+def assemble_vec(coords, c2n, xyz, N, bc_types_of, k=None, mm=None, nq=None):
+    """Dense matrix of the vector (velocity) operator with slip walls, (d E Np)^2 with the
+    components as d blocks of E*Np, from the textbook scalar form above:
+
+      component blocks   a_c(u, v) with the Dirichlet terms on faces where that component is 0
+      slip walls         sum_{a,b} n_a n_b D_F(u_b, v_a) on faces where every component is 2
+                         (D_F: the Dirichlet face form; the wall holds n.u = 0 weakly and
+                         leaves the tangential part free: the normal projection n n^T of the
+                         reference's slip ghost, vmf_2d_mult_avg_jump.h:7-27)
+      mass term          mm(x) u.v integrated exactly (mm a nodal field, or a constant)
+
+    bc_types_of(centroid) -> tuple of d bc types of that boundary face."""
+    dim = coords.shape[1]
+    E, Np = xyz.shape[1], xyz.shape[2]
+    blocks = []
+    A0 = assemble(coords, c2n, xyz, N, bc_type_of=lambda c: 1, k=k, nq=nq, bface_blocks=blocks)
+    n = E * Np
+    A = np.zeros((dim * n, dim * n))
+    for a in range(dim):
+        A[a * n:(a + 1) * n, a * n:(a + 1) * n] = A0
+    for e, nrm, FV, Db in blocks:
+        t = bc_types_of(FV.mean(axis=0))
+        sl = slice(e * Np, (e + 1) * Np)
+        if all(x == 2 for x in t):
+            for a in range(dim):
+                for b in range(dim):
+                    A[a * n + e * Np:a * n + (e + 1) * Np, b * n + e * Np:b * n + (e + 1) * Np] += nrm[a] * nrm[b] * Db
+            continue
+        for a in range(dim):
+            if t[a] == 0:
+                A[a * n:(a + 1) * n, a * n:(a + 1) * n][sl, sl] += Db
+    if mm is not None:
+        Mm = mass_matrix(coords, c2n, xyz, N, mm, nq=nq)
+        for a in range(dim):
+            A[a * n:(a + 1) * n, a * n:(a + 1) * n] += Mm
+    return A
+
+
+def mass_matrix(coords, c2n, xyz, N, mm, nq=None):
+    """Block-diagonal matrix of int_K Pi(mm u) v: the nodal interpolant of the product, as the
+    reference forms mm * u node by node (fvmf_2d_mult_mm_geof.h); mm a nodal (E, Np) field or a
+    constant. By quadrature."""
+    dim = coords.shape[1]
+    E, Np = xyz.shape[1], xyz.shape[2]
+    nq = nq or N + 3
+    qv, wv = unit_simplex_rule(dim, nq)
+    A = np.zeros((E * Np, E * Np))
+    for e in range(E):
+        V = coords[c2n[e]]
+        vol = simplex_measure(V)
+        xq = V[0] + qv @ (V[1:] - V[0])
+        w = wv * vol * math.factorial(dim)
+        L = CellBasis(xyz[:, e, :].T, N).val(xq)
+        Me = L.T @ (w[:, None] * L)
+        A[e * Np:(e + 1) * Np, e * Np:(e + 1) * Np] = Me * (float(mm) if np.isscalar(mm) else np.asarray(mm)[e][None, :])
     return A

@@ -41,13 +41,14 @@ def case(ip, oracle):
     return mesh, k, b, xo
 
 
-def solve(ip, mesh, k, b, precision, coarse, epass):
+def solve(ip, mesh, k, b, precision, coarse, epass, orders=(4, 3, 2, 1)):
     s = ip.PoissonSolver(mesh, ORDER)
     s.set_bc_types(np.ones(s.n_bfaces, dtype=np.int32))
     s.set_nullspace(True)
     s.set_factor(k)
     s.set_solver(ip.SOLVER_PMULTIGRID)
-    s.set_pmg([4, 3, 2, 1], pre_it=[1] * 4, post_it=[1] * 4, cheb_orders=[3] * 4, coarse_iters=30)
+    nl = len(orders)
+    s.set_pmg(list(orders), pre_it=[1] * nl, post_it=[1] * nl, cheb_orders=[3] * nl, coarse_iters=30)
     s.set_coarse(coarse)
     s.set_precision(precision)
     s.set_element_pass(epass)
@@ -62,13 +63,19 @@ def test_config2_mixed_matrix_cores_matches_oracle(ip, case):
     mesh, k, b, xo = case
     x_auto, st = solve(ip, mesh, k, b, ip.PRECISION_MIXED, ip.COARSE_AMG, ip.ELEMENT_PASS_AUTO)
     assert relL2(x_auto, xo) < 1e-8, (relL2(x_auto, xo), st)
-    # AUTO takes the matrix cores at order 4: the same solve as forcing them, and not the
-    # same bits as the vector FMA element pass
+    # AUTO takes the matrix cores at order 4: not the same bits as the vector FMA element pass;
+    # forcing the matrix cores on the order 3 and 2 levels too, or the vector pass everywhere,
+    # solves to the same tolerance
     x_mat, _ = solve(ip, mesh, k, b, ip.PRECISION_MIXED, ip.COARSE_AMG, ip.ELEMENT_PASS_MATRIX)
     x_vec, _ = solve(ip, mesh, k, b, ip.PRECISION_MIXED, ip.COARSE_AMG, ip.ELEMENT_PASS_VECTOR)
-    assert np.array_equal(x_auto, x_mat)
     assert not np.array_equal(x_auto, x_vec)
+    assert relL2(x_mat, xo) < 1e-8
     assert relL2(x_vec, xo) < 1e-8
+    # with order 4 the only level above order 1, AUTO and MATRIX run the same kernels: same bits
+    x_a41, _ = solve(ip, mesh, k, b, ip.PRECISION_MIXED, ip.COARSE_AMG, ip.ELEMENT_PASS_AUTO, (4, 1))
+    x_m41, _ = solve(ip, mesh, k, b, ip.PRECISION_MIXED, ip.COARSE_AMG, ip.ELEMENT_PASS_MATRIX, (4, 1))
+    assert np.array_equal(x_a41, x_m41)
+    assert relL2(x_a41, xo) < 1e-8
 
 
 @pytest.mark.parametrize("coarse", ["amg", "jacobi_cg"])
