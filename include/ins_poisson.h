@@ -261,6 +261,15 @@ int ins_poisson_solve(ins_poisson_t h, const double* rhs, double* ans, int layou
 /* PoissonMatrix::mult(in, out) on the finest order (FP64), and its FP32 kernel variant. */
 int ins_poisson_mult(ins_poisson_t h, const double* in, double* out, int layout);
 int ins_poisson_mult_fp32(ins_poisson_t h, const double* in, double* out, int layout);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the FP64 apply of
+ * ins_poisson_mult, calling no allocator. Its face traces (and, for layouts other than the
+ * solver's own device SoA, staging copies of in and out) live in a static per-GPU scratch array
+ * of INS_NOALLOC_SCRATCH_BYTES; a mesh needing more returns INS_ERR_INVALID, as does a handle not
+ * yet initialised (ins_poisson_init) or a host layout. Calls that overlap on one GPU share the
+ * scratch and race. */
+#define INS_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int ins_poisson_mult_noalloc(ins_poisson_t h, const double* in, double* out, int layout);
 /* Diagonal used by Jacobi (FactorViscousMatrix2D::calc_diag, factor_viscous_matrix.cpp:211-256). */
 int ins_poisson_diag(ins_poisson_t h, double* diag, int layout);
 /* Inverses of the per-cell Np x Np diagonal blocks used by INS_SOLVER_BLOCK_JACOBI

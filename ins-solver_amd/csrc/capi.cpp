@@ -11,6 +11,7 @@
 
 #include "batch.h"
 #include "hdf5_reader.h"
+#include "kernels.h"
 #include "mesh.h"
 #include "oi.h"
 #include "refelem.h"
@@ -701,6 +702,15 @@ int ins_poisson_mult(ins_poisson_t h, const double* in, double* out, int layout)
 int ins_poisson_set_dat_stride(ins_poisson_t h, int64_t stride) {
   return guarded([&] {
     S(h).set_dat_stride(stride);
+    return INS_OK;
+  });
+}
+
+int ins_poisson_mult_noalloc(ins_poisson_t h, const double* in, double* out, int layout) {
+  static_assert(INS_NOALLOC_SCRATCH_BYTES == (long long)insp::kNoAllocScratchBytes,
+                "scratch size of the header and the kernels");
+  return guarded([&] {
+    S(h).mult_noalloc(in, out, to_layout(layout));
     return INS_OK;
   });
 }

@@ -155,6 +155,11 @@ void launch_grad_central(int dim, int N, int64_t E, int64_t stride, const double
                          const double* p, double dt, const double* const u[3], double* const out[3],
                          hipStream_t s);
 
+// Scratch of the allocation-free apply (sipg_kernels.hip): a static device array of
+// kNoAllocScratchBytes per GPU. Returns the current device's copy, looked up once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+double* noalloc_scratch();
+
 // Achievable peaks of the current device (probe_kernels.hip): streaming copy bandwidth
 // (GB/s, read + write) and FP32 / FP64 16x16x4 matrix-core throughput (TFLOP/s).
 void measure_device_peaks(double* copy_gbs, double* mfma_f32_tflops, double* mfma_f64_tflops);

@@ -22,6 +22,7 @@
 // through the scalar cache; the cell's nodal values stay in VGPRs.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -626,5 +627,35 @@ template void launch_elem_matvec<double>(int, int, int, int64_t, int64_t, const 
                                          const double*, double*, bool, hipStream_t);
 template void launch_elem_matvec<float>(int, int, int, int64_t, int64_t, const float*,
                                         const float*, float*, bool, hipStream_t);
+
+// Scratch memory of the allocation-free apply (ins_poisson_mult_noalloc): the face traces of
+// one FP64 apply, then staging copies of its input and output when the caller's dat stride is
+// not the solver's. Every GPU the library runs on holds its own copy until the process ends;
+// calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+double* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    throw std::runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    // the array's name is a host-side stand-in; the device copy's address comes from the runtime
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
 
 }  // namespace insp

@@ -628,6 +628,7 @@ void PoissonSolver::init() {
     minv_.release();
   }
   INSP_HIP(hipStreamSynchronize(stream_));
+  (void)noalloc_scratch();  // this device's copy of the allocation-free apply's scratch
   initialised_ = true;
 }
 
@@ -1328,6 +1329,33 @@ void PoissonSolver::mult(const double* in, double* out, Layout layout, bool fp32
     INSP_HIP(hipStreamSynchronize(stream_));
   }
   export_vec(Ap_.p, out, layout, np());
+  INSP_HIP(hipStreamSynchronize(stream_));
+}
+
+void PoissonSolver::mult_noalloc(const double* in, double* out, Layout layout) {
+  if (!initialised_)
+    throw InvalidArgument("the allocation-free apply needs an initialised solver (ins_poisson_init)");
+  if (layout == Layout::HOST_AOS) throw InvalidArgument("the allocation-free apply takes device vectors");
+  if (!in || !out) throw InvalidArgument("null vector");
+  const Level& l0 = *levels_[0];
+  const size_t n_tr = (size_t)l0.L.Nf * l0.L.Nfp * stride_;  // FP64 face traces
+  const size_t n_vec = (size_t)np() * stride_;
+  // the solver's own SoA layout runs on the caller's vectors; any other is staged in the scratch
+  const bool direct = layout == Layout::DEVICE_SOA && dat_stride() == stride_ && in != out;
+  const size_t need = (n_tr + (direct ? 0 : 2 * n_vec)) * sizeof(double);
+  if (need > kNoAllocScratchBytes)
+    throw InvalidArgument("mesh too large for the allocation-free apply: needs " + std::to_string(need) +
+                          " bytes of scratch, the limit is " + std::to_string(kNoAllocScratchBytes));
+  double* scr = noalloc_scratch();
+  if (direct) {
+    apply_level<double>(0, in, out, nullptr, scr);
+  } else {
+    double* u = scr + n_tr;
+    double* y = u + n_vec;
+    import_vec(in, u, layout, np());
+    apply_level<double>(0, u, y, nullptr, scr);
+    export_vec(y, out, layout, np());
+  }
   INSP_HIP(hipStreamSynchronize(stream_));
 }
 

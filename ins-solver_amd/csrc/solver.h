@@ -190,6 +190,9 @@ class PoissonSolver {
 
   // out = A in on the finest order (FP64 or FP32 kernels).
   void mult(const double* in, double* out, Layout layout, bool fp32 = false);
+  // FP64 apply that allocates nothing: face traces (and staging copies for other layouts) in a
+  // static per-GPU scratch array; needs init() done and device vectors.
+  void mult_noalloc(const double* in, double* out, Layout layout);
   void diag(double* out, Layout layout);
   // Inverses of the per-cell diagonal blocks (block-Jacobi solver type), host cells x Np x Np.
   void inv_block_diag(double* out);

@@ -58,7 +58,7 @@ EXPORTED = [
     "ins_poisson_set_precision", "ins_poisson_set_element_pass", "ins_poisson_set_mm_factor",
     "ins_poisson_set_mm_const", "ins_poisson_set_inv_mass_factor", "ins_poisson_set_inv_mass_recp_factor", "ins_poisson_set_pmg", "ins_poisson_set_coarse",
     "ins_poisson_coarse_matrix", "ins_poisson_amg_levels", "ins_poisson_init", "ins_poisson_solve",
-    "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
+    "ins_poisson_mult", "ins_poisson_mult_fp32", "ins_poisson_mult_noalloc", "ins_poisson_diag", "ins_poisson_inv_block_diag", "ins_poisson_apply_bc",
     "ins_poisson_mass", "ins_poisson_pressure_rhs", "ins_poisson_set_history_size",
     "ins_poisson_history_push", "ins_poisson_initial_guess", "ins_poisson_lambda_max", "ins_poisson_set_profiling",
     "ins_poisson_take_timing", "ins_poisson_take_coarse_timing", "ins_device_peaks", "ins_mesh_renumber", "ins_poisson_grad_central", "ins_poisson_project_velocity", "ins_ref_element", "ins_ref_interp",
@@ -159,6 +159,7 @@ def _load():
         "ins_poisson_solve": [vp, vp, vp, C.c_int, C.POINTER(SolveStats)],
         "ins_poisson_mult": [vp, vp, vp, C.c_int],
         "ins_poisson_mult_fp32": [vp, vp, vp, C.c_int],
+        "ins_poisson_mult_noalloc": [vp, vp, vp, C.c_int],
         "ins_poisson_diag": [vp, vp, C.c_int],
         "ins_poisson_inv_block_diag": [vp, vp, C.c_int],
         "ins_poisson_apply_bc": [vp, vp, C.c_int],
@@ -551,6 +552,12 @@ class PoissonSolver:
 
     def mult_fp32(self, x, layout=LAYOUT_HOST_AOS):
         return self._unary(lib.ins_poisson_mult_fp32, x, layout)
+
+    def mult_noalloc(self, x, layout=LAYOUT_DEVICE_SOA):
+        """FP64 apply that allocates no device memory; x = (in_ptr, out_ptr) device vectors."""
+        if layout == LAYOUT_HOST_AOS:
+            raise ValueError("the allocation-free apply takes device vectors")
+        _check(lib.ins_poisson_mult_noalloc(self._h, _ptr(x[0]), _ptr(x[1]), layout))
 
     def mass(self, x, layout=LAYOUT_HOST_AOS):
         return self._unary(lib.ins_poisson_mass, x, layout)
