@@ -100,6 +100,8 @@ def parse():
                     help="run only the batch leg and print its JSON (a profiling aid)")
     ap.add_argument("--skip-oracle-check", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="with --batch-only: leave out the one-at-a-time FP64 path (engine profiles)")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-rates", type=str, default="10,25,50,100,200,500,1000,2000",
@@ -491,13 +493,14 @@ def batch_problems(ip, count, seed=20261204, wavenumbers=None):
     return probs
 
 
-def batch_leg(ip, count):
-    """Same problems solved one at a time on the plain FP64 path, then batched (mixed V-cycle)."""
+def batch_leg(ip, count, plain=True):
+    """Same problems solved one at a time on the plain FP64 path, then batched (mixed V-cycle).
+    plain=False leaves the one-at-a-time path out (profiles of the batch engine alone)."""
     probs = batch_problems(ip, count)
     dofs = sum(p["mesh"].n_cells * ip.np_of(2, p["order"]) for p in probs)
     # plain FP64 path, one problem at a time (set-up untimed, solve timed)
     plain_t, plain_its, plain_ok = 0.0, [], True
-    for i, p in enumerate(probs):
+    for i, p in enumerate(probs if plain else []):
         if i % 64 == 0:
             log(f"plain FP64 problem {i}/{len(probs)}")
         s = ip.PoissonSolver(p["mesh"], p["order"])
@@ -513,10 +516,11 @@ def batch_leg(ip, count):
         plain_t += time.perf_counter() - t0
         plain_its.append(s.last_stats["iterations"])
         del s
-    out = {"problems": count, "dof": dofs,
-           "plain_fp64": {"gdofs": dofs / plain_t * 1e-9, "seconds": plain_t,
-                          "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
-                          "converged": bool(plain_ok)}}
+    out = {"problems": count, "dof": dofs}
+    if plain:
+        out["plain_fp64"] = {"gdofs": dofs / plain_t * 1e-9, "seconds": plain_t,
+                             "solves_per_s": count / plain_t, "mean_iterations": float(np.mean(plain_its)),
+                             "converged": bool(plain_ok)}
     # the batch engine in each precision: one warm-up batch of a few problems, then the whole set
     for name, prec in (("batched_mixed", ip.PRECISION_MIXED), ("batched_fp64", ip.PRECISION_FP64)):
         log(f"batch engine, {name}")
@@ -547,7 +551,8 @@ def batch_leg(ip, count):
         del bt
     out["speedup_batched_mixed_over_batched_fp64"] = (out["batched_mixed"]["gdofs"] /
                                                       out["batched_fp64"]["gdofs"])
-    out["speedup_batched_mixed_over_plain_fp64"] = out["batched_mixed"]["gdofs"] / out["plain_fp64"]["gdofs"]
+    if plain:
+        out["speedup_batched_mixed_over_plain_fp64"] = out["batched_mixed"]["gdofs"] / out["plain_fp64"]["gdofs"]
     return out
 
 
@@ -955,7 +960,7 @@ def main():
         print(json.dumps({"precision_form": res}), flush=True)
         return
     if a.batch_only:
-        print(json.dumps({"batch": batch_leg(ip, a.batch_problems)}), flush=True)
+        print(json.dumps({"batch": batch_leg(ip, a.batch_problems, plain=not a.batch_no_plain)}), flush=True)
         return
     if a.apply_only:
         print(json.dumps({"apply_only": apply_leg(ip, torch, a.n, a.apply_only, a.steps)}), flush=True)

@@ -1014,6 +1014,13 @@ void BatchEngine::finish(int slot, ProbStatus st) {
 }
 
 int BatchEngine::iterate(int k) {
+  const int n = iterate_launch(k);
+  if (n) iterate_collect();
+  return n;
+}
+
+int BatchEngine::iterate_launch(int k) {
+  launched_ = 0;
   if (running_slots_.empty() || k < 1) return 0;
   const bool refresh = graph_tile_cap_ == -2 || !run_->tiles.p;
   if (refresh) {
@@ -1046,7 +1053,19 @@ int BatchEngine::iterate(int k) {
     for (int i = 0; i < k; ++i) run_iteration(*run_);
   }
   iters_run_ += k;
-  read_scal(running_slots_);  // the only host round trip of the k iterations
+  // the problems' state comes back with the iterations: the only host round trip of the k
+  // iterations, waited for in iterate_collect
+  int hi = 0;
+  for (int s : running_slots_) hi = std::max(hi, s + 1);
+  INSP_HIP(hipMemcpyAsync(h_scal_, d_scal_, sizeof(double) * BS_COUNT * hi, hipMemcpyDeviceToHost, stream_));
+  launched_ = k;
+  return k;
+}
+
+void BatchEngine::iterate_collect() {
+  if (!launched_) return;
+  launched_ = 0;
+  INSP_HIP(hipStreamSynchronize(stream_));
   std::vector<int> keep, verify;
   for (int slot : running_slots_) {
     Prob& P = *probs_[slot];
@@ -1071,7 +1090,6 @@ int BatchEngine::iterate(int k) {
     graph_tile_cap_ = -2;
   }
   if (!verify.empty()) start(*work_, verify, true);
-  return k;
 }
 
 int BatchEngine::running() const { return (int)running_slots_.size(); }

@@ -244,6 +244,12 @@ class BatchEngine {
   // k lockstep flexible-CG iterations of every running problem without a host round trip
   // (problems that converge on the way are frozen on the device); returns k.
   int iterate(int k = 1);
+  // iterate in two halves, so that the work of several engines overlaps on the device:
+  // iterate_launch enqueues k iterations and the read-back of the problems' state on the
+  // engine's stream and returns at once (the number of iterations launched, 0 when no problem
+  // runs); iterate_collect waits for them and retires the problems that finished.
+  int iterate_launch(int k = 1);
+  void iterate_collect();
   int running() const;
   int free_slots() const { return (int)free_slots_.size(); }
   const BatchConfig& config() const { return cfg_; }
@@ -294,6 +300,7 @@ class BatchEngine {
   void autotune(const std::vector<int>& slots);
   hipStream_t stream_;
   bool own_stream_ = false;
+  int launched_ = 0;  // iterations launched and not yet collected
   BatchTables tables_{};
   std::vector<void*> table_bufs_;
   std::vector<std::unique_ptr<Prob>> probs_;   // by slot

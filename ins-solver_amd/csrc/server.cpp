@@ -126,7 +126,9 @@ void Server::loop() {
           active_[d][ids[k]] = a;
         }
       }
-      for (int d = 0; d < 2; ++d) eng_[d]->iterate(std::max(1, eng_[d]->config().chunk));
+      // both engines' iterations in flight together (one stream each), then collected
+      for (int d = 0; d < 2; ++d) eng_[d]->iterate_launch(std::max(1, eng_[d]->config().chunk));
+      for (int d = 0; d < 2; ++d) eng_[d]->iterate_collect();
       for (int d = 0; d < 2; ++d) {
         for (int pid : eng_[d]->take_finished()) {
           auto it = active_[d].find(pid);

@@ -101,3 +101,17 @@ def test_power_iteration_matches_dense_eigenvalue(ip, oracle, dim, N, var):
     est = o.solve_pmg(b, [N, 1], max_iter_eig=3000, max_iter=1)[4][0]
     assert abs(est - lmax) < 1e-6 * lmax
     assert 0.5 * lmax < est15 <= lmax * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_indefinite_factor_breaks_down(ip, oracle, dim):
+    """A density that changes sign (an indefinite operator) ends the flexible CG with a
+    breakdown (p.Ap <= 0), status -3, as the GPU solve reports INS_ERR_BREAKDOWN."""
+    mesh = ip.Mesh.square(6, jitter=0.1, seed=2) if dim == 2 else ip.Mesh.cube(3, 0.1, 2)
+    o = oracle.Oracle(*mesh.arrays(), 2)
+    o.set_bc_types(np.zeros(o.n_bfaces, dtype=np.int32))
+    xyz = o.node_coords()
+    o.set_factor(np.where(xyz[0] < 0.5, 1.0, -1.0))
+    b = o.mass(np.ones((o.n_cells, o.Np)))
+    x, rc, it, rr = o.solve_jacobi(b, rtol=1e-10, max_iter=500)
+    assert rc == -3
