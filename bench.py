@@ -837,8 +837,8 @@ def vec_bc_types(mesh):
 
 def viscous_vec_leg(ip, torch, n, order, steps, reynolds=100.0, g0=1.5, seed=20261208):
     """The velocity step's viscous solve with slip walls (Matrix3Vec: ViscousMatrix3D /
-    ViscousSolver3D): three components, -lap u + (g0 Re/dt) u = f, inflow no-slip, outflow
-    Neumann, slip on the four side walls, the reference's single-phase preconditioner
+    ViscousSolver3D): three components, -lap u + (g0 Re/dt) u = f with a smooth f, inflow no-slip,
+    outflow Neumann, slip on the four side walls, the reference's single-phase preconditioner
     (FACTOR_INV_MASS, factor dt/(g0 Re)), FP64, rel res 1e-8."""
     mesh = cube(ip, n, seed)
     s = ip.ViscousVecSolver(mesh, order)
@@ -849,17 +849,21 @@ def viscous_vec_leg(ip, torch, n, order, steps, reynolds=100.0, g0=1.5, seed=202
     s.set_preconditioner(ip.VEC_PRE_FACTOR_INV_MASS)
     s.set_inv_mass_factor(1.0 / mm)
     s.set_tol_and_iter(1e-8, 0.0, 1000)
-    rng = np.random.default_rng(seed)
-    g = [np.ascontiguousarray(rng.standard_normal((s.n_bfaces, s.Nfp))) for _ in range(3)]
+    # a smooth forcing (the velocity step's rhs is mass times a smooth field) and homogeneous data
+    g = [np.zeros((s.n_bfaces, s.Nfp)) for _ in range(3)]
     s.set_bcs(g)
     dof = 3 * s.n_cells * s.Np
+    xyz = ip.PoissonSolver(mesh, order).node_coords()
+    smooth, _ = manufactured_sin(xyz)
 
     def to_soa(x):
         soa = np.zeros((s.Np, s.stride))
         soa[:, :s.n_cells] = x.T
         return torch.from_numpy(soa).cuda()
 
-    b_d = [to_soa(mm * rng.standard_normal((s.n_cells, s.Np))) for _ in range(3)]
+    mass = ip.PoissonSolver(mesh, order)
+    b_d = [to_soa(mass.mass(mm * (1.0 + 0.3 * c) * smooth)) for c in range(3)]
+    del mass
     x_d = [torch.zeros_like(b_d[0]) for _ in range(3)]
     bp, xp = [t.data_ptr() for t in b_d], [t.data_ptr() for t in x_d]
     t0 = time.perf_counter()
