@@ -60,10 +60,10 @@ template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
                   T* out, hipStream_t s, double* partial = nullptr);
 int apply_blocks(int64_t E);
-// Element pass (order 1 included) with one Chebyshev step of the smoother in place of the
-// store: res = inv (f - A u), d = c1 d + c2 res (d not read when c1 = 0), x_next = u + d.
-// x_next must differ from u. Levels with the matrix-core element pass take its fused variant
-// (FP32); returns false, launching nothing, only for an FP64 level with fragments (none exist).
+// Element pass (vector kernels, order 1 included) with one Chebyshev step of the smoother in
+// place of the store: res = inv (f - A u), d = c1 d + c2 res (d not read when c1 = 0),
+// x_next = u + d. x_next must differ from u. Returns false, launching nothing, where the level
+// takes the matrix-core element pass.
 template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
@@ -104,11 +104,6 @@ std::vector<double> build_mfma_fragments(int dim, int N, const std::vector<doubl
 template <typename T>
 void launch_apply_mfma(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, T* out, hipStream_t s, double* partial);
-// The matrix-core element pass with one Chebyshev step of the smoother in place of the store
-// (as launch_apply_cheb); x_next must differ from u.
-void launch_apply_mfma_cheb(const MeshView& m, const LevelView& l, const float* u, const float* tr,
-                            const float* kfac, const float* f, const float* inv, float* d, float* x_next,
-                            float c1, float c2, hipStream_t s);
 // Exact diagonal of the operator (double). With bnormal ([n_bfaces][dim] outward unit
 // normals): the diagonal of component comp of the vector operator, whose bc type 2 faces are
 // slip faces.

@@ -286,8 +286,26 @@ constexpr bool face_alpha_is_rows(int dim) {
 }
 static_assert(face_alpha_is_rows(2) && face_alpha_is_rows(3), "face coefficient rows");
 
+// Entry i (0 <= i < n) of a small register array, selected by a chain of compares. The chain
+// starts from zero rather than from v[0], so that no select has two array entries as its
+// operands: the compiler would turn such a select into a load through a selected address and
+// move the array to scratch memory.
+template <int n, typename T>
+__device__ __forceinline__ T pick(const T (&v)[n], int i) {
+  T r = (T)0;
+#pragma unroll
+  for (int k = 0; k < n; ++k) r = i == k ? v[k] : r;
+  return r;
+}
+
 // PACK: nbr holds one word per face, (neighbour << 5) | code inside the mesh and the
 // boundary index -(bface+1) on the boundary, and code is not read.
+//
+// Loads are issued in two rounds, so that a wave waits for memory twice per cell instead of
+// several times per face: first everything addressed by the cell index (neighbour words,
+// nodal values, metric, penalties), then everything addressed by a neighbour index (the
+// neighbour's nodal values and whole metric, and the node map of the shared face). A
+// boundary face reads the cell's own entries in the second round and does not use them.
 template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
@@ -296,168 +314,181 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
               const T* __restrict__ kfac, cptr<T> __restrict__ C, cptr<int> __restrict__ tab,
               Out out, const Slip& slip = Slip{}) {
   constexpr OpLayout L = op_layout(DIM, 1);
-  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf;
+  constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf, NK = L.NK;
   constexpr FmaskTable FM = make_fmask(DIM, 1);
-  double dotv = 0.0;
-  {
-    T uu[Np], acc[Np], kk[Np];
+  // round 1
+  int nb[Nf], c[Nf];
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) {
+    if constexpr (PACK) {
+      const int w = nbr[f * stride + e];
+      nb[f] = w >= 0 ? w >> 5 : w;
+      c[f] = w >= 0 ? (w & 31) : 0;
+    } else {
+      nb[f] = nbr[f * stride + e];
+      c[f] = nb[f] >= 0 ? code[f * stride + e] : 0;
+    }
+  }
+  T uu[Np], acc[Np], kk[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    uu[m] = u[m * stride + e];
+    acc[m] = 0;
+    kk[m] = kval<VAR>(kfac, m * stride + e);
+  }
+  T K[NK];
+#pragma unroll
+  for (int k = 0; k < NK; ++k) K[k] = (T)Kt[k * stride + e];
+  T ts[Nf];
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) ts[f] = (T)tauS[f * stride + e];
+  // round 2
+  T un[Nf][Np], kn[Nf][NK], fn_k[Nf][Np];
+  int nd[Nf][Nfp], bt[Nf];
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) {
+    const bool in = nb[f] >= 0;
+    const int64_t q = in ? (int64_t)nb[f] : e;
+    const int cf = in ? c[f] : 0;
 #pragma unroll
     for (int m = 0; m < Np; ++m) {
-      uu[m] = u[m * stride + e];
-      acc[m] = 0;
-      kk[m] = kval<VAR>(kfac, m * stride + e);
+      un[f][m] = u[m * stride + q];
+      fn_k[f][m] = kval<VAR>(kfac, m * stride + q);
     }
-    T K[L.NK];
 #pragma unroll
-    for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
-    T g[DIM];
+    for (int k = 0; k < NK; ++k) kn[f][k] = (T)Kt[k * stride + q];
+#pragma unroll
+    for (int j = 0; j < Nfp; ++j) nd[f][j] = tab[L.tab_nbr + (f * Nf * L.nperm + cf) * Nfp + j] >> 6;
+    bt[f] = in ? 0 : bctype[-nb[f] - 1];
+  }
+
+  double dotv = 0.0;
+  T g[DIM];
+#pragma unroll
+  for (int s = 0; s < DIM; ++s) {
+    T a = 0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], uu[m], a);
+    g[s] = a;
+  }
+  T h[DIM];
+#pragma unroll
+  for (int r = 0; r < DIM; ++r) {
+    T a = 0;
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
+    h[r] = a;
+  }
+#pragma unroll
+  for (int j = 0; j < Np; ++j)
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+      for (int i = 0; i < Np; ++i)
+        acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h[r] * kk[j], acc[i]);
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) {
+    T beta[DIM];
+    T tm0 = 0;
 #pragma unroll
     for (int s = 0; s < DIM; ++s) {
-      T a = 0;
+      T b = 0;
 #pragma unroll
-      for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], uu[m], a);
-      g[s] = a;
+      for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
+      beta[s] = b;
+      tm0 = fma(b, g[s], tm0);
     }
-    T h[DIM];
-#pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-      T a = 0;
-#pragma unroll
-      for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
-      h[r] = a;
-    }
-#pragma unroll
-    for (int j = 0; j < Np; ++j)
-#pragma unroll
-      for (int r = 0; r < DIM; ++r)
-#pragma unroll
-        for (int i = 0; i < Np; ++i)
-          acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h[r] * kk[j], acc[i]);
-#pragma unroll
-    for (int f = 0; f < Nf; ++f) {
-      T beta[DIM];
-      T tm0 = 0;
+    // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms), 3 slip
+    const int mode = nb[f] >= 0 ? 0 : (bt[f] == 0 ? 1 : (Slip::enabled && bt[f] == 2 ? 3 : 2));
+    if (mode == 2) continue;
+    // the neighbour's flux trace sum_s (sum_r alpha_{fn,r} K_n[r][s]) (grad u_n)_s, constant on
+    // a P1 cell; alpha_{fn,r} is selected per lane from the compile-time face table
+    T tn0 = 0;
+    if (mode == 0) {
+      const int fn = c[f] / L.nperm;
+      T gn[DIM];
 #pragma unroll
       for (int s = 0; s < DIM; ++s) {
-        T b = 0;
+        T a = 0;
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
-        beta[s] = b;
-        tm0 = fma(b, g[s], tm0);
+        for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], un[f][m], a);
+        gn[s] = a;
       }
-      int nb, c = 0;
-      if constexpr (PACK) {
-        const int w = nbr[f * stride + e];
-        nb = w >= 0 ? w >> 5 : w;
-        c = w & 31;
-      } else {
-        nb = nbr[f * stride + e];
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        T al = (T)face_alpha(DIM, 0, r);
+#pragma unroll
+        for (int k = 1; k < Nf; ++k) al = fn == k ? (T)face_alpha(DIM, k, r) : al;
+        T t = 0;
+#pragma unroll
+        for (int s = 0; s < DIM; ++s) t = fma(kn[f][kidx(DIM, r, s)], gn[s], t);
+        tn0 = fma(al, t, tn0);
       }
-      int mode = 0;
-      int tbase = 0;
-      T tn0 = 0;
-      if (nb >= 0) {
-        if constexpr (!PACK) c = code[f * stride + e];
-        const int fn = c / L.nperm;
-        tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
-        T gn[DIM];
+    }
+    // slip: n_c (n.U) at the face nodes and n_c sum_d n_d (beta.grad u_d), the gradients of
+    // the components recomputed from their nodal values (constant on a P1 cell)
+    T nc = 0, sm0 = 0;
+    const double* bnv = nullptr;
+    if constexpr (Slip::enabled) {
+      if (mode == 3) {
+        bnv = slip.bn + (int64_t)(-nb[f] - 1) * DIM;
+        nc = (T)bnv[slip.comp];
 #pragma unroll
-        for (int s = 0; s < DIM; ++s) gn[s] = 0;
+        for (int d = 0; d < DIM; ++d) {
+          T t = 0;
 #pragma unroll
-        for (int m = 0; m < Np; ++m) {
-          const T um = u[m * stride + nb];
+          for (int s2 = 0; s2 < DIM; ++s2) {
+            T a = 0;
 #pragma unroll
-          for (int s = 0; s < DIM; ++s) gn[s] = fma(C[L.off_D + (s * Np) * Np + m], um, gn[s]);
-        }
-        // the neighbour's sum_r alpha_{fn,r} K[r][s]: the sum of the rows of K on its slanted
-        // face (fn = DIM-1), minus one row of K on the faces through its vertex 0, so only the
-        // entries that face needs are gathered
-        if (fn == DIM - 1) {
-#pragma unroll
-          for (int s = 0; s < DIM; ++s) {
-            T b = 0;
-#pragma unroll
-            for (int r = 0; r < DIM; ++r) b += (T)Kt[kidx(DIM, r, s) * stride + nb];
-            tn0 = fma(b, gn[s], tn0);
+            for (int m = 0; m < Np; ++m)
+              a = fma(C[L.off_D + (s2 * Np) * Np + m], slip.u0[d * slip.cstride + m * stride + e], a);
+            t = fma(beta[s2], a, t);
           }
-        } else {
-          const int rr = fn < DIM - 1 ? DIM - 1 - fn : DIM - fn;
-#pragma unroll
-          for (int s = 0; s < DIM; ++s)
-            tn0 = fma(-(T)Kt[(int64_t)kidx(DIM, rr, s) * stride + nb], gn[s], tn0);
+          sm0 = fma((T)bnv[d], t, sm0);
         }
-      } else {
-        const int bt = bctype[-nb - 1];
-        mode = bt == 0 ? 1 : (Slip::enabled && bt == 2 ? 3 : 2);
       }
-      if (mode == 2) continue;
-      // slip: n_c (n.U) at the face nodes and n_c sum_d n_d (beta.grad u_d), the gradients of
-      // the components recomputed from their nodal values (constant on a P1 cell)
-      T nc = 0, sm0 = 0;
-      const double* bnv = nullptr;
+    }
+#pragma unroll
+    for (int j = 0; j < Nfp; ++j) {
+      const int p = FM.v[f * Nfp + j];
+      T um = uu[p];
+      T Tm = kk[p] * tm0;
       if constexpr (Slip::enabled) {
         if (mode == 3) {
-          bnv = slip.bn + (int64_t)(-nb - 1) * DIM;
-          nc = (T)bnv[slip.comp];
+          T su = 0;
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) {
-            T t = 0;
-#pragma unroll
-            for (int s2 = 0; s2 < DIM; ++s2) {
-              T a = 0;
-#pragma unroll
-              for (int m = 0; m < Np; ++m)
-                a = fma(C[L.off_D + (s2 * Np) * Np + m], slip.u0[d * slip.cstride + m * stride + e], a);
-              t = fma(beta[s2], a, t);
-            }
-            sm0 = fma((T)bnv[d], t, sm0);
-          }
+          for (int d = 0; d < DIM; ++d) su = fma((T)bnv[d], slip.u0[d * slip.cstride + p * stride + e], su);
+          um = nc * su;
+          Tm = kk[p] * nc * sm0;
         }
       }
-      const T ts = (T)tauS[f * stride + e];
+      T up, Tp;
+      if (mode == 0) {
+        up = pick(un[f], nd[f][j]);
+        Tp = (VAR == 0 ? (T)1 : pick(fn_k[f], nd[f][j])) * tn0;
+      } else {
+        up = -um;
+        Tp = -Tm;
+      }
+      const T jump = um - up;
+      const T F = (T)0.5 * (ts[f] * jump - (Tm - Tp));
+      const T sig = (T)-0.5 * jump * kk[p];
 #pragma unroll
-      for (int j = 0; j < Nfp; ++j) {
-        const int p = FM.v[f * Nfp + j];
-        T um = uu[p];
-        T Tm = kk[p] * tm0;
-        if constexpr (Slip::enabled) {
-          if (mode == 3) {
-            T su = 0;
+      for (int i = 0; i < Nfp; ++i)
+        acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) su = fma((T)bnv[d], slip.u0[d * slip.cstride + p * stride + e], su);
-            um = nc * su;
-            Tm = kk[p] * nc * sm0;
-          }
-        }
-        T up, Tp;
-        if (mode == 0) {
-          const int node = tab[tbase + j] >> 6;
-          up = u[(int64_t)node * stride + nb];
-          Tp = kval<VAR>(kfac, (int64_t)node * stride + nb) * tn0;
-        } else {
-          up = -um;
-          Tp = -Tm;
-        }
-        const T jump = um - up;
-        const T F = (T)0.5 * (ts * jump - (Tm - Tp));
-        const T sig = (T)-0.5 * jump * kk[p];
+      for (int r = 0; r < DIM; ++r) {
+        const T bs = beta[r] * sig;
 #pragma unroll
-        for (int i = 0; i < Nfp; ++i)
-          acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-          const T bs = beta[r] * sig;
-#pragma unroll
-          for (int i = 0; i < Np; ++i)
-            acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
-        }
+        for (int i = 0; i < Np; ++i)
+          acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
       }
     }
+  }
 #pragma unroll
-    for (int i = 0; i < Np; ++i) {
-      store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
-      dotv += (double)uu[i] * (double)acc[i];
-    }
+  for (int i = 0; i < Np; ++i) {
+    store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
+    dotv += (double)uu[i] * (double)acc[i];
   }
   return dotv;
 }

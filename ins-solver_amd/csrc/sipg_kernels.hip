@@ -23,7 +23,6 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
-#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -473,14 +472,7 @@ template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
                        const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
                        hipStream_t s) {
-  if (l.frag && mfma_apply_supported(m.dim, l.N)) {
-    if constexpr (std::is_same<T, float>::value) {
-      launch_apply_mfma_cheb(m, l, u, tr, kfac, f, inv, d, x_next, c1, c2, s);
-      return true;
-    } else {
-      return false;
-    }
-  }
+  if (l.frag && mfma_apply_supported(m.dim, l.N)) return false;
   if (x_next == u) throw std::invalid_argument("fused Chebyshev step: x_next must not alias x");
   if (m.E == 0) return true;
   const T* C = static_cast<const T*>(l.C);

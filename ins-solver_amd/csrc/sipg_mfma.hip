@@ -83,35 +83,13 @@ struct Shape {
   static constexpr int WAVES = N == 4 ? 16 : 8;
 };
 
-// Where the element pass leaves (A u) at flat index k: stored, or consumed by one Chebyshev step
-// of the smoother (res = D^-1 (f - A x), d = c1 d + c2 res, x_next = x + d; d not read when
-// c1 = 0), so that A x never goes to HBM.
-struct StoreAx {
-  float* o;
-  __device__ __forceinline__ void operator()(uint32_t k, float acc) const { o[k] = acc; }
-};
-struct ChebStep {
-  const float* f;
-  const float* inv;
-  const float* x;
-  float* d;
-  float* x_next;
-  float c1, c2;
-  __device__ __forceinline__ void operator()(uint32_t k, float acc) const {
-    const float res = inv[k] * (f[k] - acc);
-    const float di = c1 == 0.f ? c2 * res : fmaf(c1, d[k], c2 * res);
-    d[k] = di;
-    x_next[k] = x[k] + di;
-  }
-};
-
-template <int N, bool VAR, class Out>
+template <int N, bool VAR>
 __global__ void __launch_bounds__(Shape<N>::WAVES * 64)
 k_apply_mfma(int64_t E, int64_t stride, const float* __restrict__ u, const float* __restrict__ tr,
              const double* __restrict__ Kt, const double* __restrict__ tauS,
              const int* __restrict__ nbr, const int* __restrict__ code,
              const int* __restrict__ bctype, const float* __restrict__ kfac,
-             const float* __restrict__ frag, const int* __restrict__ tab, Out out) {
+             const float* __restrict__ frag, const int* __restrict__ tab, float* __restrict__ out) {
   using S = Shape<N>;
   using T = float;
   constexpr OpLayout L = S::L;
@@ -242,7 +220,7 @@ k_apply_mfma(int64_t E, int64_t stride, const float* __restrict__ u, const float
     for (int mo = 0; mo < MTF; ++mo)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (live) out((16 * mo + 4 * g + i) * st + ec, acc[mo][i]);
+        if (live) out[(16 * mo + 4 * g + i) * st + ec] = acc[mo][i];
     // 4-row groups: add the four k-slices held by lanes c, c+16, c+32, c+48
 #pragma unroll
     for (int rg = 0; rg < R4; ++rg)
@@ -252,7 +230,7 @@ k_apply_mfma(int64_t E, int64_t stride, const float* __restrict__ u, const float
         v += __shfl_xor(v, 16, 64);
         v += __shfl_xor(v, 32, 64);
         const int node = 16 * MTF + 4 * rg + i;
-        if (live && g == 0 && node < Np) out(node * st + ec, v);
+        if (live && g == 0 && node < Np) out[node * st + ec] = v;
       }
   }
 }
@@ -285,35 +263,19 @@ int fragment_count() {
   return Shape<N>::NFRAG * 64;
 }
 
-template <int N, bool VAR, class Out>
+template <int N, bool VAR>
 void launch_order(const MeshView& m, const LevelView& l, const float* u, const float* tr,
-                  const float* kfac, const Out& out, hipStream_t s) {
+                  const float* kfac, float* out, hipStream_t s) {
   using SH = Shape<N>;
   // 32-bit element offsets: the largest array indexed is the trace, Nf*Nfp rows
   if ((int64_t)SH::Nf * SH::Nfp * m.stride * (int64_t)sizeof(double) >= (int64_t(1) << 32))
     throw std::runtime_error("matrix-core element pass: mesh too large for 32-bit offsets");
   const int64_t groups = (((m.E + 15) / 16) + SH::WAVES - 1) / SH::WAVES;
-  // the store and the fused Chebyshev variants take the same registers and LDS
-  const int res = resident_blocks((const void*)k_apply_mfma<N, VAR, StoreAx>, SH::WAVES * 64, N, VAR ? 1 : 0);
+  const int res = resident_blocks((const void*)k_apply_mfma<N, VAR>, SH::WAVES * 64, N, VAR ? 1 : 0);
   const dim3 grid((unsigned)std::min<int64_t>(groups, res));
-  hipLaunchKernelGGL((k_apply_mfma<N, VAR, Out>), grid, dim3(SH::WAVES * 64), 0, s, m.E, m.stride, u, tr,
+  hipLaunchKernelGGL((k_apply_mfma<N, VAR>), grid, dim3(SH::WAVES * 64), 0, s, m.E, m.stride, u, tr,
                      m.Kt, l.tauS, m.nbr, m.code, m.bctype, kfac,
                      static_cast<const float*>(l.frag), l.tab, out);
-}
-
-template <class Out>
-void launch_any_order(const MeshView& m, const LevelView& l, const float* u, const float* tr,
-                      const float* kfac, const Out& out, hipStream_t s) {
-  switch (l.N * 2 + (kfac ? 1 : 0)) {
-    case 4: launch_order<2, false>(m, l, u, tr, kfac, out, s); break;
-    case 5: launch_order<2, true>(m, l, u, tr, kfac, out, s); break;
-    case 6: launch_order<3, false>(m, l, u, tr, kfac, out, s); break;
-    case 7: launch_order<3, true>(m, l, u, tr, kfac, out, s); break;
-    case 8: launch_order<4, false>(m, l, u, tr, kfac, out, s); break;
-    case 9: launch_order<4, true>(m, l, u, tr, kfac, out, s); break;
-    default: throw std::runtime_error("unsupported order");
-  }
-  check("k_apply_mfma");
 }
 
 }  // namespace
@@ -377,17 +339,17 @@ void launch_apply_mfma(const MeshView& m, const LevelView& l, const T* u, const 
     if (!mfma_apply_supported(m.dim, l.N) || !l.frag)
       throw std::runtime_error("matrix-core element pass: unsupported order or no fragments");
     if (partial) throw std::runtime_error("matrix-core element pass: no fused dot");
-    launch_any_order(m, l, u, tr, kfac, StoreAx{out}, s);
+    switch (l.N * 2 + (kfac ? 1 : 0)) {
+      case 4: launch_order<2, false>(m, l, u, tr, kfac, out, s); break;
+      case 5: launch_order<2, true>(m, l, u, tr, kfac, out, s); break;
+      case 6: launch_order<3, false>(m, l, u, tr, kfac, out, s); break;
+      case 7: launch_order<3, true>(m, l, u, tr, kfac, out, s); break;
+      case 8: launch_order<4, false>(m, l, u, tr, kfac, out, s); break;
+      case 9: launch_order<4, true>(m, l, u, tr, kfac, out, s); break;
+      default: throw std::runtime_error("unsupported order");
+    }
+    check("k_apply_mfma");
   }
-}
-
-void launch_apply_mfma_cheb(const MeshView& m, const LevelView& l, const float* u, const float* tr,
-                            const float* kfac, const float* f, const float* inv, float* d, float* x_next,
-                            float c1, float c2, hipStream_t s) {
-  if (!mfma_apply_supported(m.dim, l.N) || !l.frag)
-    throw std::runtime_error("matrix-core element pass: unsupported order or no fragments");
-  if (x_next == u) throw std::invalid_argument("fused Chebyshev step: x_next must not alias x");
-  launch_any_order(m, l, u, tr, kfac, ChebStep{f, inv, u, d, x_next, c1, c2}, s);
 }
 
 template void launch_apply_mfma<float>(const MeshView&, const LevelView&, const float*, const float*,

@@ -742,9 +742,8 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   // no mass term (the next iterate goes to x2, then x and x2 trade places), else apply + stream.
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
-  // the matrix-core element pass (FP32) fuses the step as well
   const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && !(oi_ && l == 0) &&
-                    !(v.frag && mfma_apply_supported(mv.dim, v.N) && !std::is_same<T, float>::value);
+                    !(v.frag && mfma_apply_supported(mv.dim, v.N));
   auto step = [&](T c1, T c2) {
     ++smoother_applies_;
     if (fuse) {
