@@ -66,6 +66,12 @@ __host__ __device__ constexpr int kidx(int dim, int r, int s) {
   return r == 0 ? s : (r == 1 ? 2 + s : 5);
 }
 
+// Node map of the order-1 faces as 2-bit fields: the neighbour's node at face node j of face f,
+// for the neighbour code c = fn * nperm + permutation, is (w[f][j] >> 2c) & 3 (c < 24 in 3D).
+struct P1NodeBits {
+  unsigned long long w[4][3];
+};
+
 // Compile-time face masks (combinatorial: which warp & blend nodes lie on
 // which face, in volume node order). Must agree with the numerical masks of
 // the reference element; the host checks this at start-up.

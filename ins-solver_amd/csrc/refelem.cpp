@@ -6,6 +6,7 @@
 // DGConstants (toolkit, not vendored); these are regenerated from the
 // published construction.
 #include "refelem.h"
+#include "dg_layout.h"
 
 #include <algorithm>
 #include <cmath>
@@ -535,6 +536,26 @@ std::vector<double> interp_matrix(const RefElement& to, const RefElement& from) 
   std::vector<double> Vto;
   vandermonde(to.dim, from.N, to.r, to.s, tt, Vto, nullptr);  // to.Np x from.Np modal
   return mat_mul(Vto, from.invV, to.Np, from.Np, from.Np);
+}
+
+static P1NodeBits build_p1_node_bits(int dim) {
+  const RefElement re = make_ref_element(dim, 1);
+  const int Nf = dim + 1, Nfp = dim;
+  P1NodeBits b{};
+  for (int f = 0; f < Nf; ++f)
+    for (int fn = 0; fn < Nf; ++fn)
+      for (int p = 0; p < re.nperm; ++p)
+        for (int j = 0; j < Nfp; ++j) {
+          const int q = re.nbr_face_node[(((size_t)f * Nf + fn) * re.nperm + p) * Nfp + j];
+          const unsigned long long node = (unsigned long long)re.fmask[fn * Nfp + q];
+          b.w[f][j] |= node << (2 * (fn * re.nperm + p));
+        }
+  return b;
+}
+
+const P1NodeBits& p1_node_bits(int dim) {
+  static const P1NodeBits b2 = build_p1_node_bits(2), b3 = build_p1_node_bits(3);
+  return dim == 2 ? b2 : b3;
 }
 
 }  // namespace insp

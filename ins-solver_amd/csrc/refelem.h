@@ -46,6 +46,9 @@ struct RefElement {
 const int* face_vertices(int dim);
 
 RefElement make_ref_element(int dim, int N);
+// The order-1 neighbour node map of the operator table in bit fields (dg_layout.h P1NodeBits),
+// built once per dimension.
+const struct P1NodeBits& p1_node_bits(int dim);
 
 // Nodal interpolation from order `from` to order `to` on the same reference
 // simplex: returns to.Np x from.Np.

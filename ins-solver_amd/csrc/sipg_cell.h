@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dg_layout.h"
 #include "kernels.h"
 #include "reduce.h"
@@ -116,6 +118,17 @@ struct ChebOut {
   }
 };
 
+// Face nodes per load round of the element pass (a divisor of Nfp): whole faces where the
+// registers allow it (2D, and 3D up to order 3 in FP32 and order 2 in FP64). 3D faces of order 4,
+// and of order 3 in FP64, go one node at a time: larger rounds cost a wave per SIMD and measured
+// slower (64^3x6 tets, FP64 order 3: 731 us per element pass one node at a time, 774 us two at a
+// time, 1021 us whole faces; 48^3x6, FP64 order 4: 1116 us one at a time, 1322 us five).
+constexpr int face_chunk(int dim, int N, int bytes) {
+  const int nfp = dim == 2 ? N + 1 : (N + 1) * (N + 2) / 2;
+  if (dim == 3 && (N == 4 || (N == 3 && bytes == 8))) return 1;
+  return nfp;
+}
+
 // One cell of the element pass; returns u.(A u) over the cell. PACK: nbr holds the packed
 // words (neighbour << 5) | code, -(bface+1) on the boundary, and code is not read.
 template <int DIM, int N, typename T, int VAR, bool PACK = false, class Out = T*,
@@ -138,6 +151,20 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
   T K[L.NK];
 #pragma unroll
   for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
+  int nbf[Nf], cf[Nf];
+  T tsf[Nf];
+#pragma unroll
+  for (int f = 0; f < Nf; ++f) {
+    if constexpr (PACK) {
+      const int w = nbr[f * stride + e];
+      nbf[f] = w >= 0 ? w >> 5 : w;
+      cf[f] = w & 31;
+    } else {
+      nbf[f] = nbr[f * stride + e];
+      cf[f] = nbf[f] >= 0 ? code[f * stride + e] : 0;
+    }
+    tsf[f] = (T)tauS[f * stride + e];
+  }
 
   // Volume term, one node column at a time.
   for (int j = 0; j < Np; ++j) {
@@ -162,7 +189,14 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
     }
   }
 
-  // Face terms.
+  // Face terms. The face nodes are taken in chunks of CH; the loads of a chunk are issued in
+  // two rounds before its arithmetic: the cell's own traces with the node map of the shared
+  // face, then the neighbour's values and traces at the mapped nodes. A boundary face reads
+  // the cell's own entries in the second round and does not use them. With one chunk per face
+  // the loops unroll completely and the cell's own face-node values come from registers.
+  constexpr int CH = face_chunk(DIM, N, (int)sizeof(T));
+  static_assert(Nfp % CH == 0, "face chunks");
+  constexpr bool whole = CH == Nfp;
 #pragma unroll
   for (int f = 0; f < Nf; ++f) {
     T beta[DIM];
@@ -173,25 +207,16 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
       for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
       beta[s] = b;
     }
-    int nb, c = 0;
-    if constexpr (PACK) {
-      const int w = nbr[f * stride + e];
-      nb = w >= 0 ? w >> 5 : w;
-      c = w & 31;
-    } else {
-      nb = nbr[f * stride + e];
-    }
+    const int nb = nbf[f];
     int mode = 0;  // 0 interior, 1 Dirichlet, 2 Neumann (or other: no terms), 3 slip
-    int tbase = 0;
-    if (nb >= 0) {
-      if constexpr (!PACK) c = code[f * stride + e];
-      tbase = L.tab_nbr + (f * Nf * L.nperm + c) * Nfp;
-    } else {
+    if (nb < 0) {
       const int bt = bctype[-nb - 1];
       mode = bt == 0 ? 1 : (Slip::enabled && bt == 2 ? 3 : 2);
     }
     if (mode == 2) continue;
-    const T ts = (T)tauS[f * stride + e];
+    const int64_t q = nb >= 0 ? (int64_t)nb : e;
+    const int tbase = L.tab_nbr + (f * Nf * L.nperm + (nb >= 0 ? cf[f] : 0)) * Nfp;
+    const T ts = tsf[f];
     // FP32 order 3 in 3D lifts the symmetry term through the face mass and the rows of D_r
     // (fewer coefficient loads, 0.47 vs 0.51 ms on 262k cells); elsewhere the extra live
     // registers cost more than the loads saved (FP64 order 4 spills), so the D_r^T EMAT_f
@@ -200,51 +225,69 @@ apply_cell(int64_t e, int64_t stride, const T* __restrict__ u, const T* __restri
     T sv[Nfp];
 #pragma unroll
     for (int i = 0; i < Nfp; ++i) sv[i] = 0;
-    for (int j = 0; j < Nfp; ++j) {
-      const int p = tab[L.tab_fmask + f * Nfp + j];
-      T um = u[p * stride + e];
-      T Tm = tr[(int64_t)(f * Nfp + j) * stride + e];
-      if constexpr (Slip::enabled) {
-        if (mode == 3) {
-          const double* n = slip.bn + (int64_t)(-nb - 1) * DIM;
-          T su = 0, st = 0;
+    auto chunk = [&](int j0) {
+      T Tm[CH], up[CH], Tp[CH], um[CH];
+      int pk[CH];
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) {
-            su = fma((T)n[d], slip.u0[d * slip.cstride + p * stride + e], su);
-            st = fma((T)n[d], slip.tr0[d * slip.tstride + (int64_t)(f * Nfp + j) * stride + e], st);
+      for (int jj = 0; jj < CH; ++jj) {
+        const int j = j0 + jj;
+        Tm[jj] = tr[(int64_t)(f * Nfp + j) * stride + e];
+        pk[jj] = tab[tbase + j];
+        if constexpr (whole) um[jj] = uu[FM.v[f * Nfp + j]];
+        else um[jj] = u[tab[L.tab_fmask + f * Nfp + j] * stride + e];
+      }
+#pragma unroll
+      for (int jj = 0; jj < CH; ++jj) {
+        up[jj] = u[(int64_t)(pk[jj] >> 6) * stride + q];
+        Tp[jj] = tr[(int64_t)(pk[jj] & 63) * stride + q];
+      }
+#pragma unroll
+      for (int jj = 0; jj < CH; ++jj) {
+        const int j = j0 + jj;
+        const int p = whole ? FM.v[f * Nfp + j] : tab[L.tab_fmask + f * Nfp + j];
+        T umj = um[jj];
+        T tm = Tm[jj];
+        if constexpr (Slip::enabled) {
+          if (mode == 3) {
+            const double* n = slip.bn + (int64_t)(-nb - 1) * DIM;
+            T su = 0, st = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              su = fma((T)n[d], slip.u0[d * slip.cstride + p * stride + e], su);
+              st = fma((T)n[d], slip.tr0[d * slip.tstride + (int64_t)(f * Nfp + j) * stride + e], st);
+            }
+            umj = (T)n[slip.comp] * su;
+            tm = (T)n[slip.comp] * st;
           }
-          um = (T)n[slip.comp] * su;
-          Tm = (T)n[slip.comp] * st;
+        }
+        const T upj = mode == 0 ? up[jj] : -umj;
+        const T tpj = mode == 0 ? Tp[jj] : -tm;
+        const T jump = umj - upj;
+        const T F = (T)0.5 * (ts * jump - (tm - tpj));
+        T sig = (T)-0.5 * jump;
+        sig *= kval<VAR>(kfac, p * stride + e);
+#pragma unroll
+        for (int i = 0; i < Nfp; ++i) {
+          const T mf = C[L.off_MfT + (f * Nfp + j) * Nfp + i];
+          acc[FM.v[f * Nfp + i]] = fma(mf, F, acc[FM.v[f * Nfp + i]]);
+          if constexpr (fold) sv[i] = fma(mf, sig, sv[i]);
+        }
+        if constexpr (!fold) {
+#pragma unroll
+          for (int r = 0; r < DIM; ++r) {
+            const T bs = beta[r] * sig;
+#pragma unroll
+            for (int i = 0; i < Np; ++i)
+              acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
+          }
         }
       }
-      T up, Tp;
-      if (mode == 0) {
-        const int pk = tab[tbase + j];
-        up = u[(int64_t)(pk >> 6) * stride + nb];
-        Tp = tr[(int64_t)(pk & 63) * stride + nb];
-      } else {
-        up = -um;
-        Tp = -Tm;
-      }
-      const T jump = um - up;
-      const T F = (T)0.5 * (ts * jump - (Tm - Tp));
-      T sig = (T)-0.5 * jump;
-      sig *= kval<VAR>(kfac, p * stride + e);
-#pragma unroll
-      for (int i = 0; i < Nfp; ++i) {
-        const T mf = C[L.off_MfT + (f * Nfp + j) * Nfp + i];
-        acc[FM.v[f * Nfp + i]] = fma(mf, F, acc[FM.v[f * Nfp + i]]);
-        if constexpr (fold) sv[i] = fma(mf, sig, sv[i]);
-      }
-      if constexpr (!fold) {
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-          const T bs = beta[r] * sig;
-#pragma unroll
-          for (int i = 0; i < Np; ++i)
-            acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
-        }
-      }
+    };
+    if constexpr (whole) {
+      chunk(0);
+    } else {
+#pragma unroll 1
+      for (int j0 = 0; j0 < Nfp; j0 += CH) chunk(j0);
     }
     // symmetry term sum_r beta_r D_r^T (EMAT_f sig): the face mass first, then the rows
     // of D_r at the face nodes
@@ -306,13 +349,22 @@ __device__ __forceinline__ T pick(const T (&v)[n], int i) {
 // nodal values, metric, penalties), then everything addressed by a neighbour index (the
 // neighbour's nodal values and whole metric, and the node map of the shared face). A
 // boundary face reads the cell's own entries in the second round and does not use them.
-template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip>
+// Where the order-1 element pass finds the neighbour's node at a face node: in the operator's
+// integer table (TabNodes: one load per face node), or in bit fields passed with the launch
+// (BitNodes: no load, and no registers held while the other loads are in flight).
+struct TabNodes {};
+struct BitNodes {
+  P1NodeBits b;
+};
+
+template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip,
+          class Nodes = TabNodes>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
               const int* __restrict__ code, const int* __restrict__ bctype,
               const T* __restrict__ kfac, cptr<T> __restrict__ C, cptr<int> __restrict__ tab,
-              Out out, const Slip& slip = Slip{}) {
+              Out out, const Slip& slip = Slip{}, const Nodes& nodes = Nodes{}) {
   constexpr OpLayout L = op_layout(DIM, 1);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf, NK = L.NK;
   constexpr FmaskTable FM = make_fmask(DIM, 1);
@@ -358,7 +410,12 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
 #pragma unroll
     for (int k = 0; k < NK; ++k) kn[f][k] = (T)Kt[k * stride + q];
 #pragma unroll
-    for (int j = 0; j < Nfp; ++j) nd[f][j] = tab[L.tab_nbr + (f * Nf * L.nperm + cf) * Nfp + j] >> 6;
+    for (int j = 0; j < Nfp; ++j) {
+      if constexpr (std::is_same<Nodes, BitNodes>::value)
+        nd[f][j] = (int)((nodes.b.w[f][j] >> (2 * cf)) & 3ull);
+      else
+        nd[f][j] = tab[L.tab_nbr + (f * Nf * L.nperm + cf) * Nfp + j] >> 6;
+    }
     bt[f] = in ? 0 : bctype[-nb[f] - 1];
   }
 

@@ -29,6 +29,7 @@
 #include "dg_layout.h"
 #include "kernels.h"
 #include "reduce.h"
+#include "refelem.h"
 #include "sipg_cell.h"
 
 namespace insp {
@@ -77,12 +78,13 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
            const double* __restrict__ tauS, const int* __restrict__ nbr,
            const int* __restrict__ code, const int* __restrict__ bctype,
            const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
-           T* __restrict__ out, double* __restrict__ partial) {
+           T* __restrict__ out, double* __restrict__ partial, P1NodeBits bits) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_p1_cell<DIM, T, VAR, PACK>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C),
-                                      as_const(tab), out);
+    dotv = apply_p1_cell<DIM, T, VAR, PACK, T*, NoSlip, BitNodes>(
+        e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab), out, NoSlip{},
+        BitNodes{bits});
   if (partial) {
     const double t = block_sum1<256>(dotv);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -113,12 +115,12 @@ k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double
                 const int* __restrict__ code, const int* __restrict__ bctype,
                 const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
                 const T* __restrict__ f, const T* __restrict__ inv, T* __restrict__ d,
-                T* __restrict__ x_next, T c1, T c2) {
+                T* __restrict__ x_next, T c1, T c2, P1NodeBits bits) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   if (e < E)
-    (void)apply_p1_cell<DIM, T, VAR, PACK>(e, stride, u, Kt, tauS, nbr, code, bctype, kfac,
-                                           as_const(C), as_const(tab),
-                                           ChebOut<T>{f, inv, d, x_next, c1, c2});
+    (void)apply_p1_cell<DIM, T, VAR, PACK, ChebOut<T>, NoSlip, BitNodes>(
+        e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab),
+        ChebOut<T>{f, inv, d, x_next, c1, c2}, NoSlip{}, BitNodes{bits});
 }
 
 // Element pass of one component of the vector (velocity) operator with slip faces (FP64;
@@ -429,9 +431,10 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
     // packed neighbour words when the mesh has them (one 4-byte load per face instead of two)
     const bool pack = m.nbc != nullptr;
     const int* nw = pack ? m.nbc : m.nbr;
+    const P1NodeBits& bits = p1_node_bits(m.dim);
 #define INSP_P1(D, V, P)                                                                          \
   hipLaunchKernelGGL((k_apply_p1<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, u,  \
-                     m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial)
+                     m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial, bits)
     if (m.dim == 2) {
       if (kfac) { if (pack) INSP_P1(2, true, true); else INSP_P1(2, true, false); }
       else { if (pack) INSP_P1(2, false, true); else INSP_P1(2, false, false); }
@@ -479,9 +482,11 @@ bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const 
   const bool pack = m.nbc != nullptr;
   const int* nw = pack ? m.nbc : m.nbr;
   if (l.N == 1) {
+    const P1NodeBits& bits = p1_node_bits(m.dim);
 #define INSP_P1C(D, V, P)                                                                         \
   hipLaunchKernelGGL((k_apply_p1_cheb<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, \
-                     u, m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, inv, d, x_next, c1, c2)
+                     u, m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, inv, d, x_next, c1, c2, \
+                     bits)
     if (m.dim == 2) {
       if (kfac) { if (pack) INSP_P1C(2, true, true); else INSP_P1C(2, true, false); }
       else { if (pack) INSP_P1C(2, false, true); else INSP_P1C(2, false, false); }
