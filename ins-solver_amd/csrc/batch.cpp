@@ -49,32 +49,24 @@ struct BatchEngine::Prob {
 };
 
 struct BatchEngine::View {
-  DBuf<int2> tiles;
-  DBuf<int> probs;
-  DBuf<int2> ranges;
-  DBuf<int> counts;
+  // the view's index tables in one device buffer, filled from one pinned host image per refresh:
+  // tiles (int2, tile_cap), probs (int, prob_cap), ranges (int2, max slots), counts (int, 12),
+  // AMG row tiles (int2, levels x ltile_cap), AMG tile counts (int, levels)
+  DBuf<char> meta;
+  char* h_meta = nullptr;
+  size_t meta_bytes = 0;
+  size_t o_tiles = 0, o_probs = 0, o_ranges = 0, o_counts = 0, o_ltiles = 0, o_lcount = 0;
   DBuf<double> partial;
-  int tile_cap = 0, prob_cap = 0;
-  DBuf<int2> ltiles;
-  DBuf<int> lcount;
-  int ltile_cap = 0;
-  int2* h_ltiles = nullptr;
-  int* h_lcount = nullptr;
+  int tile_cap = 0, prob_cap = 0, ltile_cap = 0;
   int levels = 0;                // deepest AMG hierarchy in the view
   int ocap[5] = {0, 0, 0, 0, 0};  // grid per order
-  int2* h_tiles = nullptr;
-  int* h_probs = nullptr;
-  int2* h_ranges = nullptr;
-  int* h_counts = nullptr;
-  int max_slots = 0;
   BatchView bv{};
+  template <typename X>
+  X* host(size_t off) { return reinterpret_cast<X*>(h_meta + off); }
+  template <typename X>
+  X* dev(size_t off) { return reinterpret_cast<X*>(meta.p + off); }
   ~View() {
-    if (h_tiles) (void)hipHostFree(h_tiles);
-    if (h_probs) (void)hipHostFree(h_probs);
-    if (h_ranges) (void)hipHostFree(h_ranges);
-    if (h_counts) (void)hipHostFree(h_counts);
-    if (h_ltiles) (void)hipHostFree(h_ltiles);
-    if (h_lcount) (void)hipHostFree(h_lcount);
+    if (h_meta) (void)hipHostFree(h_meta);
   }
 };
 
@@ -196,65 +188,6 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
     while (c < need) c *= 2;
     return c;
   };
-  const int tcap = grow(ntiles, v.tile_cap), pcap = grow(nprobs, v.prob_cap);
-  if (tcap != v.tile_cap || !v.tiles.p) {
-    v.tiles.alloc(tcap);
-    v.partial.alloc((size_t)4 * tcap);
-    if (v.h_tiles) (void)hipHostFree(v.h_tiles);
-    INSP_HIP(hipHostMalloc((void**)&v.h_tiles, sizeof(int2) * tcap, hipHostMallocDefault));
-    v.tile_cap = tcap;
-  }
-  if (pcap != v.prob_cap || !v.probs.p) {
-    v.probs.alloc(pcap);
-    if (v.h_probs) (void)hipHostFree(v.h_probs);
-    INSP_HIP(hipHostMalloc((void**)&v.h_probs, sizeof(int) * pcap, hipHostMallocDefault));
-    v.prob_cap = pcap;
-  }
-  if (!v.ranges.p) {
-    v.ranges.alloc(max_slots_);
-    v.counts.alloc(12);
-    INSP_HIP(hipHostMalloc((void**)&v.h_ranges, sizeof(int2) * max_slots_, hipHostMallocDefault));
-    INSP_HIP(hipHostMalloc((void**)&v.h_counts, sizeof(int) * 12, hipHostMallocDefault));
-    std::memset(v.h_ranges, 0, sizeof(int2) * max_slots_);
-  }
-  // tiles grouped by order, so each order's kernels run over one contiguous range
-  int t = 0;
-  for (int j = 0; j < nprobs; ++j) v.h_probs[j] = slots[j];
-  for (int o = 1; o <= kMaxOrder; ++o) {
-    v.h_counts[7 + o] = t;  // offset of order o
-    for (int j = 0; j < nprobs; ++j) {
-      const Prob& P = *probs_[slots[j]];
-      if (P.N != o) continue;
-      v.h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / tile_));
-      for (int64_t c = 0; c < P.S; c += tile_) v.h_tiles[t++] = make_int2(slots[j], (int)c);
-    }
-    v.h_counts[2 + o] = t - v.h_counts[7 + o];
-    v.ocap[o] = grow(v.h_counts[2 + o], v.ocap[o]);
-  }
-  v.h_counts[0] = ntiles;
-  v.h_counts[1] = nprobs;
-  INSP_HIP(hipMemcpyAsync(v.tiles.p, v.h_tiles, sizeof(int2) * std::max(ntiles, 1),
-                          hipMemcpyHostToDevice, stream_));
-  INSP_HIP(hipMemcpyAsync(v.probs.p, v.h_probs, sizeof(int) * std::max(nprobs, 1),
-                          hipMemcpyHostToDevice, stream_));
-  INSP_HIP(hipMemcpyAsync(v.ranges.p, v.h_ranges, sizeof(int2) * max_slots_, hipMemcpyHostToDevice,
-                          stream_));
-  INSP_HIP(hipMemcpyAsync(v.counts.p, v.h_counts, sizeof(int) * 12, hipMemcpyHostToDevice, stream_));
-  INSP_HIP(hipStreamSynchronize(stream_));
-  v.bv.P = d_desc_;
-  v.bv.tiles = v.tiles.p;
-  v.bv.ntiles = v.counts.p;
-  v.bv.probs = v.probs.p;
-  v.bv.ranges = v.ranges.p;
-  v.bv.nprobs = v.counts.p + 1;
-  v.bv.tile_cap = v.tile_cap;
-  v.bv.prob_cap = v.prob_cap;
-  v.bv.partial = v.partial.p;
-  v.bv.dim = cfg_.dim;
-  v.bv.tile = tile_;
-  v.bv.ocount = v.counts.p + 2;
-  v.bv.ooff = v.counts.p + 7;
-  for (int o = 0; o <= kMaxOrder; ++o) v.bv.ocap[o] = v.ocap[o];
   // AMG row tiles per level (256 rows each, independent of the cell tile)
   std::vector<int> per_level(kAmgLevels, 0);
   int levels = 0;
@@ -264,33 +197,81 @@ void BatchEngine::set_view(View& v, const std::vector<int>& slots) {
     for (size_t k = 0; k < P.amg_rows.size(); ++k) per_level[k] += (P.amg_rows[k] + kTile - 1) / kTile;
   }
   const int lneed = std::max(1, *std::max_element(per_level.begin(), per_level.end()));
-  const int lcap = grow(lneed, v.ltile_cap);
-  if (lcap != v.ltile_cap || !v.ltiles.p) {
-    v.ltiles.alloc((size_t)kAmgLevels * lcap);
-    if (v.h_ltiles) (void)hipHostFree(v.h_ltiles);
-    INSP_HIP(hipHostMalloc((void**)&v.h_ltiles, sizeof(int2) * kAmgLevels * lcap, hipHostMallocDefault));
+  const int tcap = grow(ntiles, v.tile_cap), pcap = grow(nprobs, v.prob_cap), lcap = grow(lneed, v.ltile_cap);
+  if (tcap != v.tile_cap || !v.partial.p) v.partial.alloc((size_t)4 * tcap);
+  if (tcap != v.tile_cap || pcap != v.prob_cap || lcap != v.ltile_cap || !v.meta.p) {
+    size_t off = 0;
+    auto seg = [&](size_t bytes) {
+      const size_t at = off;
+      off += align256(bytes);
+      return at;
+    };
+    v.o_tiles = seg(sizeof(int2) * tcap);
+    v.o_probs = seg(sizeof(int) * pcap);
+    v.o_ranges = seg(sizeof(int2) * max_slots_);
+    v.o_counts = seg(sizeof(int) * 12);
+    v.o_ltiles = seg(sizeof(int2) * kAmgLevels * lcap);
+    v.o_lcount = seg(sizeof(int) * kAmgLevels);
+    v.meta_bytes = off;
+    v.meta.alloc(off);
+    if (v.h_meta) (void)hipHostFree(v.h_meta);
+    v.h_meta = nullptr;
+    INSP_HIP(hipHostMalloc((void**)&v.h_meta, off, hipHostMallocDefault));
+    std::memset(v.h_meta, 0, off);
+    v.tile_cap = tcap;
+    v.prob_cap = pcap;
     v.ltile_cap = lcap;
   }
-  if (!v.lcount.p) {
-    v.lcount.alloc(kAmgLevels);
-    INSP_HIP(hipHostMalloc((void**)&v.h_lcount, sizeof(int) * kAmgLevels, hipHostMallocDefault));
+  int2* h_tiles = v.host<int2>(v.o_tiles);
+  int* h_probs = v.host<int>(v.o_probs);
+  int2* h_ranges = v.host<int2>(v.o_ranges);
+  int* h_counts = v.host<int>(v.o_counts);
+  int2* h_ltiles = v.host<int2>(v.o_ltiles);
+  int* h_lcount = v.host<int>(v.o_lcount);
+  // tiles grouped by order, so each order's kernels run over one contiguous range
+  int t = 0;
+  for (int j = 0; j < nprobs; ++j) h_probs[j] = slots[j];
+  for (int o = 1; o <= kMaxOrder; ++o) {
+    h_counts[7 + o] = t;  // offset of order o
+    for (int j = 0; j < nprobs; ++j) {
+      const Prob& P = *probs_[slots[j]];
+      if (P.N != o) continue;
+      h_ranges[slots[j]] = make_int2(t, t + (int)(P.S / tile_));
+      for (int64_t c = 0; c < P.S; c += tile_) h_tiles[t++] = make_int2(slots[j], (int)c);
+    }
+    h_counts[2 + o] = t - h_counts[7 + o];
+    v.ocap[o] = grow(h_counts[2 + o], v.ocap[o]);
   }
+  h_counts[0] = ntiles;
+  h_counts[1] = nprobs;
   for (int k = 0; k < kAmgLevels; ++k) {
-    int t = 0;
+    int lt = 0;
     for (int sl : slots) {
       const Prob& P = *probs_[sl];
       if (k >= (int)P.amg_rows.size()) continue;
-      for (int r0 = 0; r0 < P.amg_rows[k]; r0 += kTile) v.h_ltiles[(size_t)k * lcap + t++] = make_int2(sl, r0);
+      for (int r0 = 0; r0 < P.amg_rows[k]; r0 += kTile) h_ltiles[(size_t)k * lcap + lt++] = make_int2(sl, r0);
     }
-    v.h_lcount[k] = t;
+    h_lcount[k] = lt;
   }
   v.levels = levels;
-  INSP_HIP(hipMemcpyAsync(v.ltiles.p, v.h_ltiles, sizeof(int2) * kAmgLevels * lcap, hipMemcpyHostToDevice,
-                          stream_));
-  INSP_HIP(hipMemcpyAsync(v.lcount.p, v.h_lcount, sizeof(int) * kAmgLevels, hipMemcpyHostToDevice, stream_));
+  INSP_HIP(hipMemcpyAsync(v.meta.p, v.h_meta, v.meta_bytes, hipMemcpyHostToDevice, stream_));
   INSP_HIP(hipStreamSynchronize(stream_));
-  v.bv.ltiles = v.ltiles.p;
-  v.bv.lcount = v.lcount.p;
+  v.bv.P = d_desc_;
+  v.bv.tiles = v.dev<int2>(v.o_tiles);
+  v.bv.ntiles = v.dev<int>(v.o_counts);
+  v.bv.probs = v.dev<int>(v.o_probs);
+  v.bv.ranges = v.dev<int2>(v.o_ranges);
+  v.bv.nprobs = v.dev<int>(v.o_counts) + 1;
+  v.bv.tile_cap = v.tile_cap;
+  v.bv.prob_cap = v.prob_cap;
+  v.bv.partial = v.partial.p;
+  v.bv.dim = cfg_.dim;
+  v.bv.tile = tile_;
+  v.bv.ocount = v.dev<int>(v.o_counts) + 2;
+  v.bv.ooff = v.dev<int>(v.o_counts) + 7;
+  for (int o = 0; o <= kMaxOrder; ++o) v.bv.ocap[o] = v.ocap[o];
+  v.bv.ltiles = v.dev<int2>(v.o_ltiles);
+  v.bv.lcount = v.dev<int>(v.o_lcount);
   v.bv.ltile_cap = v.ltile_cap;
 }
 
@@ -482,9 +463,6 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
     int* nbr = (int*)take(Nf * S * 4);
     int* code = (int*)take(Nf * S * 4);
     int* bct = (int*)take((size_t)std::max<int64_t>(E, 1) * Nf * 4);
-    double* tauF = (double*)take(Nf * S * 8);
-    double* tauC = (double*)take(Nf * S * 8);
-    D.Kt = Kt; D.J = J; D.nbr = nbr; D.code = code; D.bctype = bct; D.tauF = tauF; D.tauC = tauC;
     double *kF64 = nullptr, *kC64 = nullptr;
     float *kF32 = nullptr, *kC32 = nullptr;
     if (P->var) {
@@ -492,12 +470,18 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
       kC64 = (double*)take(Np1 * S * 8); kC32 = (float*)take(Np1 * S * 4);
     }
     D.kF[0] = kF64; D.kF[1] = kF32; D.kC[0] = kC64; D.kC[1] = kC32;
+    D.v[BV_B] = take(Np * S * 8);
+    // everything up to here comes from the host: one image, one copy
+    const size_t upload_bytes = off;
+    double* tauF = (double*)take(Nf * S * 8);
+    double* tauC = (double*)take(Nf * S * 8);
+    D.Kt = Kt; D.J = J; D.nbr = nbr; D.code = code; D.bctype = bct; D.tauF = tauF; D.tauC = tauC;
     double* iF64 = (double*)take(Np * S * 8);
     float* iF32 = (float*)take(Np * S * 4);
     double* iC64 = (double*)take(Np1 * S * 8);
     float* iC32 = (float*)take(Np1 * S * 4);
     D.invF[0] = iF64; D.invF[1] = iF32; D.invC[0] = iC64; D.invC[1] = iC32;
-    for (int f = BV_B; f <= BV_AP; ++f) D.v[f] = take(Np * S * 8);
+    for (int f = BV_B + 1; f <= BV_AP; ++f) D.v[f] = take(Np * S * 8);
     D.v[BV_TRD] = take((size_t)Nf * Nfp * S * 8);
     for (int f = BV_VX; f <= BV_VD; ++f) D.v[f] = take(Np * S * t);
     D.v[BV_TRT] = cfg_.fp32 ? take((size_t)Nf * Nfp * S * t) : D.v[BV_TRD];
@@ -517,13 +501,19 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
         cd[(size_t)f * S + e] = n >= 0 ? m.nbr_face[(size_t)e * Nf + f] * nperm + m.nbr_perm[(size_t)e * Nf + f] : 0;
       }
     }
-    INSP_HIP(hipMemcpy(Kt, kt.data(), kt.size() * 8, hipMemcpyHostToDevice));
-    INSP_HIP(hipMemcpy(J, jj.data(), jj.size() * 8, hipMemcpyHostToDevice));
-    INSP_HIP(hipMemcpy(nbr, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
-    INSP_HIP(hipMemcpy(code, cd.data(), cd.size() * 4, hipMemcpyHostToDevice));
+    std::vector<char> img(upload_bytes, 0);
+    auto put = [&](void* dst, const void* src, size_t nbytes) {
+      const size_t at = (size_t)((char*)dst - base);
+      if (at + nbytes > upload_bytes) throw std::runtime_error("arena upload layout overflow");
+      std::memcpy(img.data() + at, src, nbytes);
+    };
+    put(Kt, kt.data(), kt.size() * 8);
+    put(J, jj.data(), jj.size() * 8);
+    put(nbr, nb.data(), nb.size() * 4);
+    put(code, cd.data(), cd.size() * 4);
     std::vector<int> bt = sp.bc_types;
     if (bt.empty()) bt.assign((size_t)std::max<int64_t>(P->nbf, 1), 0);
-    INSP_HIP(hipMemcpy(bct, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+    put(bct, bt.data(), bt.size() * 4);
     const RefElement reN = make_ref_element(d, P->N), re1 = make_ref_element(d, 1);
     if (P->var) {
       std::vector<double> kf((size_t)Np * S, 0.0), kc((size_t)Np1 * S, 0.0);
@@ -537,11 +527,18 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
         }
       }
       std::vector<float> kff(kf.begin(), kf.end()), kcf(kc.begin(), kc.end());
-      INSP_HIP(hipMemcpy(kF64, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
-      INSP_HIP(hipMemcpy(kF32, kff.data(), kff.size() * 4, hipMemcpyHostToDevice));
-      INSP_HIP(hipMemcpy(kC64, kc.data(), kc.size() * 8, hipMemcpyHostToDevice));
-      INSP_HIP(hipMemcpy(kC32, kcf.data(), kcf.size() * 4, hipMemcpyHostToDevice));
+      put(kF64, kf.data(), kf.size() * 8);
+      put(kF32, kff.data(), kff.size() * 4);
+      put(kC64, kc.data(), kc.size() * 8);
+      put(kC32, kcf.data(), kcf.size() * 4);
     }
+    {  // rhs in SoA
+      std::vector<double> rb((size_t)Np * S, 0.0);
+      for (int64_t e = 0; e < E; ++e)
+        for (int i = 0; i < Np; ++i) rb[(size_t)i * S + e] = sp.rhs[(size_t)e * Np + i];
+      put(D.v[BV_B], rb.data(), rb.size() * 8);
+    }
+    INSP_HIP(hipMemcpy(base, img.data(), upload_bytes, hipMemcpyHostToDevice));
     // penalties (finest order in the penalty on both levels) and inverse diagonals
     MeshView mv{d, E, S, Kt, J, nbr, code, bct};
     LevelView lvN{P->N, nullptr, tables_.C64[P->N], tables_.tab[P->N], tauF};
@@ -556,12 +553,7 @@ std::vector<int> BatchEngine::add_many(std::vector<ProblemSpec>&& specs) {
     reciprocal<double>(Np1, E, S, tmp, iC64, stream_);
     reciprocal<float>(Np1, E, S, tmp, iC32, stream_);
     fill<double>(Np * S, tmp, 0.0, stream_);
-    // rhs in SoA
-    std::vector<double> rb((size_t)Np * S, 0.0);
-    for (int64_t e = 0; e < E; ++e)
-      for (int i = 0; i < Np; ++i) rb[(size_t)i * S + e] = sp.rhs[(size_t)e * Np + i];
     INSP_HIP(hipStreamSynchronize(stream_));
-    INSP_HIP(hipMemcpy(D.v[BV_B], rb.data(), rb.size() * 8, hipMemcpyHostToDevice));
     if (sp.rhs_nodal) {  // b = J M f
       launch_mass(mv, lvN, (const double*)D.v[BV_B], (double*)D.v[BV_AP], stream_);
       copy<double>((int64_t)Np * S, (const double*)D.v[BV_AP], (double*)D.v[BV_B], stream_);
@@ -746,7 +738,10 @@ void parallel_for(size_t n, F&& fn) {
 }  // namespace
 
 struct BatchEngine::ProbeScratch {
-  DBuf<int> cv, vptr, vlist, gptr, gcol, colour;
+  // the integer maps of one problem in one buffer (one copy per problem): cv, vptr, vlist, gptr,
+  // gcol, colour at off[0..5], each segment sized for the largest problem
+  DBuf<int> ints;
+  size_t off[7] = {0, 0, 0, 0, 0, 0, 0};
   DBuf<double> val, xv, yv;
 };
 
@@ -772,12 +767,9 @@ void BatchEngine::setup_amg_all(const std::vector<int>& slots, const std::vector
     ngp = std::max(ngp, A.g.ptr.size());
     ngc = std::max(ngc, A.g.col.size());
   }
-  ps.cv.alloc(ncv, false);
-  ps.vptr.alloc(nvp, false);
-  ps.vlist.alloc(nvl, false);
-  ps.gptr.alloc(ngp, false);
-  ps.gcol.alloc(ngc, false);
-  ps.colour.alloc(nvp, false);
+  const size_t seg[6] = {ncv, nvp, nvl, ngp, ngc, nvp};
+  for (int k = 0; k < 6; ++k) ps.off[k + 1] = ps.off[k] + seg[k];
+  ps.ints.alloc(ps.off[6], false);
   ps.val.alloc(ngc, false);
   ps.xv.alloc(nvp, false);
   ps.yv.alloc(nvp, false);
@@ -817,19 +809,17 @@ void BatchEngine::amg_probe(int slot, AmgPrep& A, ProbeScratch& ps) {
   Prob& P = *probs_[slot];
   const int d1 = cfg_.dim + 1, nv = (int)A.vptr.size() - 1;
   const int64_t E = P.E, S = P.S;
-  auto h2d = [&](void* d, const void* h, size_t bytes) {
-    INSP_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
-  };
-  h2d(ps.cv.p, A.cv.data(), A.cv.size() * 4);
-  h2d(ps.vptr.p, A.vptr.data(), A.vptr.size() * 4);
-  h2d(ps.vlist.p, A.vlist.data(), A.vlist.size() * 4);
-  h2d(ps.gptr.p, A.g.ptr.data(), A.g.ptr.size() * 4);
-  h2d(ps.gcol.p, A.g.col.data(), A.g.col.size() * 4);
-  h2d(ps.colour.p, A.colour.data(), A.colour.size() * 4);
+  const std::vector<int>* src[6] = {&A.cv, &A.vptr, &A.vlist, &A.g.ptr, &A.g.col, &A.colour};
+  std::vector<int> h(ps.off[6], 0);
+  for (int k = 0; k < 6; ++k) {
+    if (src[k]->size() > ps.off[k + 1] - ps.off[k]) throw std::runtime_error("probe scratch overflow");
+    std::copy(src[k]->begin(), src[k]->end(), h.begin() + (std::ptrdiff_t)ps.off[k]);
+  }
+  INSP_HIP(hipMemcpyAsync(ps.ints.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
   struct {
     int* p;
-  } d_cv{ps.cv.p}, d_vptr{ps.vptr.p}, d_vlist{ps.vlist.p}, gptr{ps.gptr.p}, gcol{ps.gcol.p},
-      dcol{ps.colour.p};
+  } d_cv{ps.ints.p + ps.off[0]}, d_vptr{ps.ints.p + ps.off[1]}, d_vlist{ps.ints.p + ps.off[2]},
+      gptr{ps.ints.p + ps.off[3]}, gcol{ps.ints.p + ps.off[4]}, dcol{ps.ints.p + ps.off[5]};
   DBuf<double>& val = ps.val;
   DBuf<double>& xv = ps.xv;
   DBuf<double>& yv = ps.yv;
@@ -882,17 +872,17 @@ void BatchEngine::amg_upload(int slot, AmgPrep& A) {
   }
   bytes += a((size_t)H.n_coarse * H.n_coarse * t);
   INSP_HIP(hipMalloc(&P.amg_arena, bytes));
-  INSP_HIP(hipMemset(P.amg_arena, 0, bytes));
-  INSP_HIP(hipDeviceSynchronize());
   P.amg_bytes = bytes;
   bytes_in_use_ += bytes;
+  // the arena is laid out in a host image (zero where the device fills it) and sent in one copy
+  std::vector<char> img(bytes, 0);
   char* base = (char*)P.amg_arena;
   size_t off = 0;
   auto put = [&](const void* h, size_t nbytes) -> void* {
     if (off + a(nbytes) > bytes) throw std::runtime_error("AMG arena layout overflow");
     void* d = base + off;
+    if (h && nbytes) std::memcpy(img.data() + off, h, nbytes);
     off += a(nbytes);
-    if (h && nbytes) INSP_HIP(hipMemcpy(d, h, nbytes, hipMemcpyHostToDevice));
     return d;
   };
   auto put_t = [&](const std::vector<double>& v) -> void* {
@@ -937,6 +927,7 @@ void BatchEngine::amg_upload(int slot, AmgPrep& A) {
   }
   D.nc = H.n_coarse;
   D.cinv = put_t(H.coarse_inv);
+  INSP_HIP(hipMemcpy(base, img.data(), bytes, hipMemcpyHostToDevice));
 }
 
 // r = b - A x, convergence test, then z = M r, p = z for problems not yet converged.
@@ -1022,7 +1013,7 @@ int BatchEngine::iterate(int k) {
 int BatchEngine::iterate_launch(int k) {
   launched_ = 0;
   if (running_slots_.empty() || k < 1) return 0;
-  const bool refresh = graph_tile_cap_ == -2 || !run_->tiles.p;
+  const bool refresh = graph_tile_cap_ == -2 || !run_->meta.p;
   if (refresh) {
     const int old_t = run_->tile_cap, old_p = run_->prob_cap, old_l = run_->ltile_cap,
               old_lev = run_->levels;
