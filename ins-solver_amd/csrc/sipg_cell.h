@@ -41,7 +41,10 @@ __device__ __forceinline__ T kval(const T* __restrict__ kfac, int64_t idx) {
   else return kfac ? kfac[idx] : (T)1;
 }
 
-// Trace pass for one cell: T = sJ n.(k grad u) at every face node.
+// Trace pass for one cell: T = sJ n.(k grad u) at every face node. Per node h = k K grad u is
+// formed once; a face's trace is sum_r alpha_{f,r} h_r with the compile-time face coefficients
+// (one row of h, or the sum of its rows on the slanted face), so no per-face coefficient table
+// stays live across the node loop.
 template <int DIM, int N, typename T, int VAR>
 __device__ __forceinline__ void
 trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
@@ -55,16 +58,6 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
   T K[L.NK];
 #pragma unroll
   for (int k = 0; k < L.NK; ++k) K[k] = (T)Kt[k * stride + e];
-  T beta[Nf][DIM];
-#pragma unroll
-  for (int f = 0; f < Nf; ++f)
-#pragma unroll
-    for (int s = 0; s < DIM; ++s) {
-      T b = 0;
-#pragma unroll
-      for (int r = 0; r < DIM; ++r) b += (T)face_alpha(DIM, f, r) * K[kidx(DIM, r, s)];
-      beta[f][s] = b;
-    }
 #pragma unroll
   for (int p = 0; p < Np; ++p) {
     T g[DIM];
@@ -75,8 +68,15 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
       for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np + p) * Np + m], uu[m], a);
       g[s] = a;
     }
-    T kp = 1;
-    kp = kval<VAR>(kfac, p * stride + e);
+    const T kp = kval<VAR>(kfac, p * stride + e);
+    T h[DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      T a = 0;
+#pragma unroll
+      for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
+      h[r] = kp * a;
+    }
 #pragma unroll
     for (int f = 0; f < Nf; ++f)
 #pragma unroll
@@ -84,8 +84,9 @@ trace_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __r
         if (FM.v[f * Nfp + j] == p) {
           T v = 0;
 #pragma unroll
-          for (int s = 0; s < DIM; ++s) v = fma(beta[f][s], g[s], v);
-          tr[(int64_t)(f * Nfp + j) * stride + e] = kp * v;
+          for (int r = 0; r < DIM; ++r)
+            if (face_alpha(DIM, f, r) != 0.0) v += (T)face_alpha(DIM, f, r) * h[r];
+          tr[(int64_t)(f * Nfp + j) * stride + e] = v;
         }
   }
 }

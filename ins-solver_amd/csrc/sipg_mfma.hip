@@ -170,44 +170,58 @@ k_apply_mfma(int64_t E, int64_t stride, const float* __restrict__ u, const float
     }
 
     // faces: k-step f*J4+q covers face f, lane (g, c) forms the flux of its face node
-    // 4q+g (slots past Nfp are zero) and the lift by EMAT_f and W_fr follows
+    // 4q+g (slots past Nfp are zero) and the lift by EMAT_f and W_fr follows. The loads go in
+    // rounds ahead of the arithmetic: the words, codes and penalties of all faces first, then
+    // per face the cell's own and the neighbour's values and traces at every face node (a lane
+    // with no term reads valid entries of its own cell and discards them).
+    const uint32_t ea = live ? ec : 0u;
+    int nbf[Nf], cf[Nf], modef[Nf];
+    T tsf[Nf];
+#pragma unroll
+    for (int f = 0; f < Nf; ++f) {
+      nbf[f] = at(nbr, f * st + ea);
+      cf[f] = at(code, f * st + ea);
+      tsf[f] = (T)at(tauS, f * st + ea);
+    }
+#pragma unroll
+    for (int f = 0; f < Nf; ++f) {
+      // 0 interior, 1 Dirichlet, 2 Neumann or no cell: no terms
+      int mode = 2;
+      if (live) mode = nbf[f] >= 0 ? 0 : (bctype[-nbf[f] - 1] == 0 ? 1 : 2);
+      modef[f] = mode;
+    }
 #pragma unroll 1
     for (int f = 0; f < Nf; ++f) {
       T beta[3];
 #pragma unroll
       for (int s = 0; s < 3; ++s) beta[s] = face_beta(f, K[kidx(3, 0, s)], K[kidx(3, 1, s)], K[kidx(3, 2, s)]);
-      int nb = 0, tb = 0, mode = 2;  // 0 interior, 1 Dirichlet, 2 Neumann or no cell: no terms
-      T ts = 0;
-      if (live) {
-        nb = at(nbr, f * st + ec);
-        if (nb >= 0) {
-          mode = 0;
-          tb = L.tab_nbr + (f * Nf * L.nperm + at(code, f * st + ec)) * Nfp;
-        } else {
-          mode = bctype[-nb - 1] == 0 ? 1 : 2;
-        }
-        ts = (T)at(tauS, f * st + ec);
+      const int mode = pick(modef, f), nbw = pick(nbf, f);
+      const T ts = pick(tsf, f);
+      const uint32_t qn = mode == 0 ? (uint32_t)nbw : ea;
+      const int tb = L.tab_nbr + (f * Nf * L.nperm + (mode == 0 ? pick(cf, f) : 0)) * Nfp;
+      T umq[J4], tmq[J4], upq[J4], tpq[J4], kq[J4];
+#pragma unroll
+      for (int q = 0; q < J4; ++q) {
+        const int j = 4 * q + g, jj = j < Nfp ? j : 0;
+        const int p = tl[L.tab_fmask + f * Nfp + jj];
+        const int pk = tl[tb + jj];
+        umq[q] = at(u, p * st + ea);
+        tmq[q] = at(tr, (f * Nfp + jj) * st + ea);
+        upq[q] = at(u, (pk >> 6) * st + qn);
+        tpq[q] = at(tr, (pk & 63) * st + qn);
+        kq[q] = VAR ? at(kfac, p * st + ea) : (T)1;
       }
 #pragma unroll
       for (int q = 0; q < J4; ++q) {
         const int j = 4 * q + g, kk = f * J4 + q;
         T F = 0, sg = 0;
         if (j < Nfp && mode != 2) {
-          const int p = tl[L.tab_fmask + f * Nfp + j];
-          const T um = at(u, p * st + ec);
-          const T Tm = at(tr, (f * Nfp + j) * st + ec);
-          T up, Tp;
-          if (mode == 0) {
-            const int pk = tl[tb + j];
-            up = at(u, (pk >> 6) * st + nb);
-            Tp = at(tr, (pk & 63) * st + nb);
-          } else {
-            up = -um;
-            Tp = -Tm;
-          }
+          const T um = umq[q], Tm = tmq[q];
+          const T up = mode == 0 ? upq[q] : -um;
+          const T Tp = mode == 0 ? tpq[q] : -Tm;
           const T jump = um - up;
           F = (T)0.5 * (ts * jump - (Tm - Tp));
-          sg = (T)-0.5 * jump * (VAR ? at(kfac, p * st + ec) : (T)1);
+          sg = (T)-0.5 * jump * kq[q];
         }
         step2(3 * Q + kk, F);
 #pragma unroll
