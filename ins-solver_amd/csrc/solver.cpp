@@ -102,7 +102,11 @@ PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
     own_stream_ = true;
   }
   const int64_t E = mesh_.n_cells;
+  // SoA row pitch: a multiple of 64 cells, and an odd one, so that the rows of one cell (its
+  // values, metric, traces) do not all fall on the same cache sets as they would with a pitch of
+  // 2^k times a small factor (64^3 x 6 cells = 3 x 2^19)
   stride_ = round_up(E, 64);
+  if (((stride_ / 64) & 1) == 0) stride_ += 64;
   const int d = mesh_.dim, Nf = d + 1, NK = d == 2 ? 3 : 6;
   const int nperm = d == 2 ? 2 : 6;
   std::vector<double> kt((size_t)NK * stride_, 0.0), jj((size_t)stride_, 0.0);

@@ -256,13 +256,14 @@ __global__ void __launch_bounds__(kBlock)
 k_rand_fill(int rows, int64_t E, int64_t stride, uint64_t seed, double* __restrict__ x) {
   const int64_t n = (int64_t)rows * stride;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    // splitmix64 of (seed, index)
-    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    // splitmix64 of (seed, row * E + cell): the values do not depend on the row padding
+    const int64_t row = i / stride, cell = i - row * stride;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)row * (uint64_t)E + (uint64_t)cell + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
     const double u = (double)(z >> 11) * 0x1.0p-53;
-    x[i] = (i % stride < E) ? 2.0 * u - 1.0 : 0.0;
+    x[i] = cell < E ? 2.0 * u - 1.0 : 0.0;
   }
 }
 

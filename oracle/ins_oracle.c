@@ -1793,7 +1793,6 @@ int oracle_solve_pmg(oracle_t* o, const int* orders, int nl, const int* pre_it, 
                      double atol, int max_iter, int nullspace, int* iters, double* relres,
                      double* lmax_out) {
   const int64_t E = o->E;
-  const int64_t stride = ((E + 63) / 64) * 64;
   pmg_ctx c;
   memset(&c, 0, sizeof(c));
   c.o = o; c.nl = nl; c.pre = pre_it; c.post = post_it; c.cheb = cheb_orders;
@@ -1830,7 +1829,7 @@ int oracle_solve_pmg(oracle_t* o, const int* orders, int nl, const int* pre_it, 
     interp_mono(o->dim, &c.L[l].R, &c.L[l + 1].R, c.L[l].P);
   }
   /* power iteration on D^-1 A (set_rand_vec / rayleigh_quotient / vec_normalise); the
-   * start vector uses the same counter hash as the device (index row*stride + cell) */
+   * start vector uses the same counter hash as the device (index row*E + cell) */
   for (int l = 0; l + 1 < nl; ++l) {
     level_t* L = &c.L[l];
     const int np = L->R.Np;
@@ -1839,7 +1838,7 @@ int oracle_solve_pmg(oracle_t* o, const int* orders, int nl, const int* pre_it, 
     const uint64_t seed = 0x5eedULL + 977ULL * L->N;
     for (int64_t e = 0; e < E; ++e)
       for (int i = 0; i < np; ++i) {
-        const uint64_t idx = (uint64_t)i * stride + e;
+        const uint64_t idx = (uint64_t)i * (uint64_t)E + (uint64_t)e;
         const uint64_t z = mix64(seed + 0x9E3779B97F4A7C15ull * (idx + 1));
         v[e * np + i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
       }
