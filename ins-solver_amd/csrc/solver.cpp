@@ -12,7 +12,7 @@ namespace {
 // Device scalar slots.
 enum : int {
   S_RZ = 0, S_PAP = 1, S_RR = 2, S_RZN = 3, S_BNUM = 4,
-  S_CG_A = 8, S_CG_PAP = 9, S_CG_B = 10,
+  S_CG_A = 8, S_CG_PAP = 9, S_CG_B = 10, S_CG_ASAVE = 11,
   S_EIG_TOP = 12, S_EIG_BOT = 13, S_EIG_NRM = 14,
   S_MEAN = 16, S_BB = 17,
   S_COUNT = 32
@@ -25,7 +25,7 @@ template <typename T>
 struct PoissonSolver::Work {
   DBuf<T> x, b, ax, d;     // V-cycle vectors
   DBuf<T> x2;              // smoothing levels: the next iterate of a fused Chebyshev step
-  DBuf<T> r, z, p, Ap;     // coarse CG vectors (coarsest level only)
+  DBuf<T> z, p, p2, Ap;    // coarse CG vectors (coarsest level only; p, p2: two directions)
   DBuf<T> tr;              // face traces
   DBuf<T> inv;             // inverse diagonal
   DBuf<T> k;               // factor (empty = constant)
@@ -605,7 +605,7 @@ void PoissonSolver::init() {
         w.x.alloc(n); w.b.alloc(n); w.ax.alloc(n); w.d.alloc(n);
         if (l + 1 < levels_.size() || pmg_.coarse == CoarseType::AMG) w.x2.alloc(n);
         if (l + 1 == levels_.size() && pmg_.coarse == CoarseType::JACOBI_CG) {
-          w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.Ap.alloc(n);
+          w.z.alloc(n); w.p.alloc(n); w.p2.alloc(n); w.Ap.alloc(n);
         }
       }
     };
@@ -781,25 +781,40 @@ void PoissonSolver::coarse_cg(int l) {
   const int64_t n = (int64_t)lv.L.Np * stride_;
   RedScratch rs{partial_.p};
   // Jacobi-CG carried in the preconditioned residual z = D^-1 r alone: per iteration the
-  // apply (p.Ap fused), z -= alpha D^-1 Ap with r.z = sum z^2 D, then x and p in one pass.
+  // apply (p.Ap fused), z -= alpha D^-1 Ap with r.z = sum z^2 D, then the direction update. x is
+  // brought up to date every second iteration, from the two last directions (two buffers): the
+  // same operations in the same order as one x update per iteration, with 4.5 instead of 5
+  // vector passes per iteration.
   fill<T>(n, w.x.p, (T)0, stream_);
   jacobi<T>(n, w.b.p, w.inv.p, w.z.p, stream_);
   copy<T>(n, w.z.p, w.p.p, stream_);
   dot<T>(n, w.b.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
   int a = S_CG_A, b = S_CG_B;
+  T* pc = w.p.p;   // the current direction
+  T* pq = w.p2.p;  // the previous one (odd iterations) or the buffer for the next one
   for (int it = 0; it < pmg_.coarse_iters; ++it) {
     if (std::is_same<T, float>::value && matrix_pass(l) && !(oi_ && l == 0)) {
       // the matrix-core element pass has no fused dot
-      apply_level<T>(l, w.p.p, w.Ap.p);
-      dot<T>(n, w.p.p, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
+      apply_level<T>(l, pc, w.Ap.p);
+      dot<T>(n, pc, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
     } else {
-      apply_level<T>(l, w.p.p, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
+      apply_level<T>(l, pc, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
       finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
     }
     ++smoother_applies_;
     cg_update_z<T>(n, w.z.p, w.Ap.p, w.inv.p, scal_.p, a, S_CG_PAP, b, rs, stream_);
-    cg_update_xp<T>(n, w.x.p, w.p.p, w.z.p, scal_.p, a, S_CG_PAP, b, it + 1 == pmg_.coarse_iters,
-                    stream_);
+    const bool last = it + 1 == pmg_.coarse_iters;
+    if (it % 2 == 1) {
+      // x += alpha_{k-1} p_{k-1} + alpha_k p_k; the next direction overwrites p_{k-1}
+      cg_update_x2p<T>(n, w.x.p, pq, pc, w.z.p, scal_.p, S_CG_ASAVE, a, S_CG_PAP, b, last, stream_);
+      std::swap(pc, pq);
+    } else if (last) {
+      cg_update_xp<T>(n, w.x.p, pc, w.z.p, scal_.p, a, S_CG_PAP, b, true, stream_);
+    } else {
+      // the next direction into the other buffer; alpha_k kept for the x update
+      cg_update_p<T>(n, pq, pc, w.z.p, scal_.p, a, S_CG_PAP, b, S_CG_ASAVE, stream_);
+      std::swap(pc, pq);
+    }
     std::swap(a, b);
   }
 }

@@ -58,6 +58,16 @@ void cg_update_z(int64_t n, T* z, const T* Ap, const T* invdiag, double* scal, i
 template <typename T>
 void cg_update_xp(int64_t n, T* x, T* p, const T* z, const double* scal, int anum, int aden,
                   int bnum, bool last, hipStream_t s);
+// p_next = z + beta p (beta = scal[bnum]/scal[anum]); alpha = scal[anum]/scal[aden] is stored in
+// scal[asave] for the x update of the next iteration. p_next must differ from p.
+template <typename T>
+void cg_update_p(int64_t n, T* p_next, const T* p, const T* z, double* scal, int anum, int aden,
+                 int bnum, int asave, hipStream_t s);
+// x += scal[asave] p0 + alpha p1 (alpha = scal[anum]/scal[aden]), then p0 = z + beta p1 unless last
+// (beta = scal[bnum]/scal[anum]).
+template <typename T>
+void cg_update_x2p(int64_t n, T* x, T* p0, const T* p1, const T* z, const double* scal, int asave,
+                   int anum, int aden, int bnum, bool last, hipStream_t s);
 // z = invdiag * r
 template <typename T>
 void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s);

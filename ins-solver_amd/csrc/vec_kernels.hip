@@ -204,6 +204,32 @@ k_cg_update_xp(int64_t n, T* __restrict__ x, T* __restrict__ p, const T* __restr
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
+k_cg_update_p(int64_t n, T* __restrict__ p_next, const T* __restrict__ p, const T* __restrict__ z,
+              double* __restrict__ scal, int anum, int aden, int bnum, int asave) {
+  const T beta = (T)safe_div(scal[bnum], scal[anum]);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p_next[i] = fma(beta, p[i], z[i]);
+  // no block reads scal[asave] in this launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[asave] = safe_div(scal[anum], scal[aden]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_x2p(int64_t n, T* __restrict__ x, T* __restrict__ p0, const T* __restrict__ p1,
+                const T* __restrict__ z, const double* __restrict__ scal, int asave, int anum,
+                int aden, int bnum, bool last) {
+  const T a0 = (T)scal[asave];
+  const T a1 = (T)safe_div(scal[anum], scal[aden]);
+  const T beta = (T)safe_div(scal[bnum], scal[anum]);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T q0 = p0[i], q1 = p1[i];
+    x[i] = fma(a1, q1, fma(a0, q0, x[i]));
+    if (!last) p0[i] = fma(beta, q1, z[i]);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
 k_jacobi(int64_t n, const T* __restrict__ r, const T* __restrict__ invdiag, T* __restrict__ z) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     z[i] = invdiag[i] * r[i];
@@ -407,6 +433,20 @@ void cg_update_xp(int64_t n, T* x, T* p, const T* z, const double* scal, int anu
 }
 
 template <typename T>
+void cg_update_p(int64_t n, T* p_next, const T* p, const T* z, double* scal, int anum, int aden,
+                 int bnum, int asave, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update_p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p_next, p, z, scal,
+                     anum, aden, bnum, asave); launch_check("k_cg_update_p");
+}
+
+template <typename T>
+void cg_update_x2p(int64_t n, T* x, T* p0, const T* p1, const T* z, const double* scal, int asave,
+                   int anum, int aden, int bnum, bool last, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_update_x2p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, p0, p1, z,
+                     scal, asave, anum, aden, bnum, last); launch_check("k_cg_update_x2p");
+}
+
+template <typename T>
 void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s) {
   hipLaunchKernelGGL((k_jacobi<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, invdiag, z); launch_check("k_jacobi");
 }
@@ -498,6 +538,10 @@ void scal_copy(double* scal, int dst, int src, hipStream_t s) {
                                RedScratch, hipStream_t);                                          \
   template void cg_update_xp<T>(int64_t, T*, T*, const T*, const double*, int, int, int, bool,    \
                                 hipStream_t);                                                     \
+  template void cg_update_p<T>(int64_t, T*, const T*, const T*, double*, int, int, int, int,      \
+                               hipStream_t);                                                      \
+  template void cg_update_x2p<T>(int64_t, T*, T*, const T*, const T*, const double*, int, int,    \
+                                 int, int, bool, hipStream_t);                                    \
   template void jacobi<T>(int64_t, const T*, const T*, T*, hipStream_t);                          \
   template void cheb_step<T>(int64_t, const T*, const T*, const T*, T*, T*, T, T, bool,           \
                              hipStream_t);                                                        \
