@@ -67,7 +67,8 @@ __host__ __device__ constexpr int kidx(int dim, int r, int s) {
 }
 
 // Node map of the order-1 faces as 2-bit fields: the neighbour's node at face node j of face f,
-// for the neighbour code c = fn * nperm + permutation, is (w[f][j] >> 2c) & 3 (c < 24 in 3D).
+// for the neighbour code c = fn * nperm + permutation, is (w[f][j] >> 2c) & 3 (c < 24 in 3D; in
+// 3D every row f is the same).
 struct P1NodeBits {
   unsigned long long w[4][3];
 };

@@ -550,6 +550,11 @@ static P1NodeBits build_p1_node_bits(int dim) {
           const unsigned long long node = (unsigned long long)re.fmask[fn * Nfp + q];
           b.w[f][j] |= node << (2 * (fn * re.nperm + p));
         }
+  // in 3D the map does not depend on the cell's own face; the kernels read row 0 only
+  if (dim == 3)
+    for (int f = 1; f < Nf; ++f)
+      for (int j = 0; j < Nfp; ++j)
+        if (b.w[f][j] != b.w[0][j]) throw std::logic_error("order-1 node map depends on the face in 3D");
   return b;
 }
 

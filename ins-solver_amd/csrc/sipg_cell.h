@@ -412,8 +412,8 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     for (int k = 0; k < NK; ++k) kn[f][k] = (T)Kt[k * stride + q];
 #pragma unroll
     for (int j = 0; j < Nfp; ++j) {
-      if constexpr (std::is_same<Nodes, BitNodes>::value)
-        nd[f][j] = (int)((nodes.b.w[f][j] >> (2 * cf)) & 3ull);
+      if constexpr (std::is_same<Nodes, BitNodes>::value)  // 3D: the same row for every face
+        nd[f][j] = (int)((nodes.b.w[DIM == 3 ? 0 : f][j] >> (2 * cf)) & 3ull);
       else
         nd[f][j] = tab[L.tab_nbr + (f * Nf * L.nperm + cf) * Nfp + j] >> 6;
     }
