@@ -70,7 +70,14 @@ struct DevCsr {
   const int* ptr = nullptr;
   const int* col = nullptr;
   const T* val = nullptr;
+  int lanes = 1;  // lanes per row of the product (lanes_per_row), 1: one row per lane
 };
+
+// Lanes that share one row of a CSR product: about two nonzeros per lane, 2..32.
+inline int lanes_per_row(int64_t nnz, int n) {
+  const double avg = n > 0 ? (double)nnz / n : 0.0;
+  return avg < 6 ? 2 : avg < 12 ? 4 : avg < 24 ? 8 : avg < 48 ? 16 : 32;
+}
 
 namespace amg {
 // y = A x (mode 0), y += A x (mode 1), y = b - A x (mode 2).

@@ -1,4 +1,4 @@
-// Device kernels of the vertex coarse space: CSR products (one row per lane),
+// Device kernels of the vertex coarse space: CSR products (a group of lanes per row),
 // the dense coarsest solve, vertex transfer to/from the order-1 DG space and
 // the probing kernels that assemble the vertex operator.
 #include <hip/hip_runtime.h>
@@ -24,14 +24,21 @@ inline void check(const char* what) {
 constexpr int kB = 256;
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
-template <typename T>
+// LANES adjacent lanes share a row: lane l takes the row's nonzeros l, l + LANES, ... (the group's
+// loads of val and col are contiguous), then the group adds its partial sums by shuffles in a
+// fixed order.
+template <typename T, int LANES>
 __global__ void __launch_bounds__(kB)
 k_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col, const T* __restrict__ val,
        const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ b, int mode) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
+  const int64_t g = blockIdx.x * (int64_t)kB + threadIdx.x;
+  const int i = (int)(g / LANES), l = (int)(g % LANES);
   T s = 0;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k) s = fma(val[k], x[col[k]], s);
+  if (i < n)
+    for (int k = ptr[i] + l; k < ptr[i + 1]; k += LANES) s = fma(val[k], x[col[k]], s);
+#pragma unroll
+  for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
+  if (i >= n || l != 0) return;
   if (mode == 0) y[i] = s;
   else if (mode == 1) y[i] += s;
   else y[i] = b[i] - s;
@@ -93,8 +100,19 @@ k_probe_extract(int nv, const int* __restrict__ ptr, const int* __restrict__ col
 template <typename T>
 void spmv(const DevCsr<T>& A, const T* x, T* y, const T* b, int mode, hipStream_t s) {
   if (A.n == 0) return;
-  hipLaunchKernelGGL((k_spmv<T>), dim3(blocks(A.n)), dim3(kB), 0, s, A.n, A.ptr, A.col, A.val, x, y, b,
-                     mode);
+  const unsigned nb = blocks((int64_t)A.n * A.lanes);
+#define INSP_SPMV(LN)                                                                             \
+  hipLaunchKernelGGL((k_spmv<T, LN>), dim3(nb), dim3(kB), 0, s, A.n, A.ptr, A.col, A.val, x, y, b, mode)
+  switch (A.lanes) {
+    case 1: INSP_SPMV(1); break;
+    case 2: INSP_SPMV(2); break;
+    case 4: INSP_SPMV(4); break;
+    case 8: INSP_SPMV(8); break;
+    case 16: INSP_SPMV(16); break;
+    case 32: INSP_SPMV(32); break;
+    default: throw std::invalid_argument("CSR product: lanes per row must be 1, 2, 4, 8, 16 or 32");
+  }
+#undef INSP_SPMV
   check("k_spmv");
 }
 

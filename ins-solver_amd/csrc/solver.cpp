@@ -70,6 +70,7 @@ struct PoissonSolver::Amg {
   // hierarchy
   struct Lev {
     int n = 0, m = 0;              // rows, rows of the next level
+    int lanesA = 1, lanesP = 1, lanesR = 1;  // lanes per row of each product
     DBuf<int> Aptr, Acol, Pptr, Pcol, Rptr, Rcol;
     AmgLevelBufs<double> f64;
     AmgLevelBufs<float> f32;
@@ -938,6 +939,7 @@ void PoissonSolver::build_coarse_space() {
     L->lmax = h.lmax;
     L->Aptr.upload(h.A.ptr.data(), h.A.ptr.size());
     L->Acol.upload(h.A.col.data(), h.A.col.size());
+    L->lanesA = lanes_per_row((int64_t)h.A.col.size(), h.A.n);
     std::vector<double> inv((size_t)h.A.n);
     for (int i = 0; i < h.A.n; ++i) inv[i] = h.diag[i] != 0 ? 1.0 / h.diag[i] : 0.0;
     const bool has_next = k + 1 < H.levels.size();
@@ -946,6 +948,8 @@ void PoissonSolver::build_coarse_space() {
       L->Pcol.upload(h.P.col.data(), h.P.col.size());
       L->Rptr.upload(h.R.ptr.data(), h.R.ptr.size());
       L->Rcol.upload(h.R.col.data(), h.R.col.size());
+      L->lanesP = lanes_per_row((int64_t)h.P.col.size(), h.P.n);
+      L->lanesR = lanes_per_row((int64_t)h.R.col.size(), h.R.n);
     }
     auto fill_bufs = [&](auto& B, auto tag) {
       using T = decltype(tag);
@@ -984,7 +988,7 @@ void PoissonSolver::amg_vcycle(int k) {
   Amg& A = *amg_;
   Amg::Lev& L = *A.lev[k];
   AmgLevelBufs<T>& B = L.bufs<T>();
-  const DevCsr<T> Am{L.n, L.n, L.Aptr.p, L.Acol.p, B.Aval.p};
+  const DevCsr<T> Am{L.n, L.n, L.Aptr.p, L.Acol.p, B.Aval.p, L.lanesA};
   if (k + 1 == (int)A.lev.size()) {
     const T* M = std::is_same<T, float>::value ? (const T*)(const void*)A.cinv32.p
                                                : (const T*)(const void*)A.cinv64.p;
@@ -1013,10 +1017,10 @@ void PoissonSolver::amg_vcycle(int k) {
   amg::spmv<T>(Am, B.x.p, B.r.p, B.b.p, 2, stream_);  // r = b - A x
   Amg::Lev& C = *A.lev[k + 1];
   AmgLevelBufs<T>& CB = C.bufs<T>();
-  const DevCsr<T> R{L.m, L.n, L.Rptr.p, L.Rcol.p, B.Rval.p};
+  const DevCsr<T> R{L.m, L.n, L.Rptr.p, L.Rcol.p, B.Rval.p, L.lanesR};
   amg::spmv<T>(R, B.r.p, CB.b.p, nullptr, 0, stream_);
   amg_vcycle<T>(k + 1);
-  const DevCsr<T> P{L.n, L.m, L.Pptr.p, L.Pcol.p, B.Pval.p};
+  const DevCsr<T> P{L.n, L.m, L.Pptr.p, L.Pcol.p, B.Pval.p, L.lanesP};
   amg::spmv<T>(P, CB.x.p, B.x.p, nullptr, 1, stream_);  // x += P x_c
   smooth(false);
 }
@@ -1041,7 +1045,7 @@ void PoissonSolver::coarse_amg(int l) {
   amg_vcycle<T>(0);
   if (pmg_.amg_cycles > 1) {
     // further cycles on the vertex residual: x_v += V(b_v - A_v x_v)
-    const DevCsr<T> Am{V.n, V.n, V.Aptr.p, V.Acol.p, VB.Aval.p};
+    const DevCsr<T> Am{V.n, V.n, V.Aptr.p, V.Acol.p, VB.Aval.p, V.lanesA};
     copy<T>(V.n, VB.b.p, VB.b0.p, stream_);
     copy<T>(V.n, VB.x.p, VB.acc.p, stream_);
     for (int c = 1; c < pmg_.amg_cycles; ++c) {
