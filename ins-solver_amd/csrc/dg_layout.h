@@ -47,6 +47,18 @@ __host__ __device__ constexpr OpLayout op_layout(int dim, int N) {
   return L;
 }
 
+// Order 1: the nodes are the reference vertices and the basis is phi_0 = -(1 + sum_k xi_k)/2,
+// phi_{k+1} = (1 + xi_k)/2, so every derivative is a constant and the order-1 kernels use these
+// values as compile-time constants (the table builder checks D, M and Mface against them).
+//   d phi_i / d xi_r
+__host__ __device__ constexpr double p1_dphi(int r, int i) {
+  return i == 0 ? -0.5 : (i == r + 1 ? 0.5 : 0.0);
+}
+//   column sum of the reference mass matrix (the integral of phi_j): area 2 / 3, volume 4/3 / 4
+__host__ __device__ constexpr double p1_mass_colsum(int dim) { return dim == 2 ? 2.0 / 3.0 : 1.0 / 3.0; }
+//   face mass in face coordinates: c (1 + delta_ij), c = length 2 / 6 or area 2 / 12
+__host__ __device__ constexpr double p1_face_mass_c(int dim) { return dim == 2 ? 1.0 / 3.0 : 1.0 / 6.0; }
+
 // Signed face-direction coefficients: the outward normal of face f is
 // n_f = grad(sum_r alpha_f[r] xi_r) / |.|, xi = (r, s[, t]).
 __host__ __device__ constexpr double face_alpha(int dim, int f, int r) {

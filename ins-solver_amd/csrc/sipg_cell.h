@@ -420,15 +420,18 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     bt[f] = in ? 0 : bctype[-nb[f] - 1];
   }
 
+  // P1: grad phi_i is constant, so Bt[r][j][i] = dphi_ri m_j and Wt[f][r][j][i] = dphi_ri w_j
+  // (m, w: column sums of the mass and face mass). The volume term and every face's symmetry
+  // term collapse into one vector V: acc_i += sum_r dphi_ri V_r at the end, with
+  // V = (sum_j m_j k_j) h + sum_f beta_f sum_j w_j sig_fj.
+  (void)C;
+  // c_f: the face mass factor; c_s = -w/2 with the face mass column sum w = c_f (Nfp + 1)
+  constexpr T c_m = (T)p1_mass_colsum(DIM), c_f = (T)p1_face_mass_c(DIM),
+              c_s = (T)(-0.5 * p1_face_mass_c(DIM) * (Nfp + 1));
   double dotv = 0.0;
   T g[DIM];
 #pragma unroll
-  for (int s = 0; s < DIM; ++s) {
-    T a = 0;
-#pragma unroll
-    for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], uu[m], a);
-    g[s] = a;
-  }
+  for (int s = 0; s < DIM; ++s) g[s] = (T)0.5 * (uu[s + 1] - uu[0]);
   T h[DIM];
 #pragma unroll
   for (int r = 0; r < DIM; ++r) {
@@ -437,13 +440,15 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     for (int s = 0; s < DIM; ++s) a = fma(K[kidx(DIM, r, s)], g[s], a);
     h[r] = a;
   }
+  T V[DIM];
+  {
+    T ks = kk[0];
 #pragma unroll
-  for (int j = 0; j < Np; ++j)
+    for (int j = 1; j < Np; ++j) ks += kk[j];
+    const T mk = c_m * ks;
 #pragma unroll
-    for (int r = 0; r < DIM; ++r)
-#pragma unroll
-      for (int i = 0; i < Np; ++i)
-        acc[i] = fma(C[L.off_Bt + (r * Np + j) * Np + i], h[r] * kk[j], acc[i]);
+    for (int r = 0; r < DIM; ++r) V[r] = mk * h[r];
+  }
 #pragma unroll
   for (int f = 0; f < Nf; ++f) {
     T beta[DIM];
@@ -466,12 +471,7 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
       const int fn = c[f] / L.nperm;
       T gn[DIM];
 #pragma unroll
-      for (int s = 0; s < DIM; ++s) {
-        T a = 0;
-#pragma unroll
-        for (int m = 0; m < Np; ++m) a = fma(C[L.off_D + (s * Np) * Np + m], un[f][m], a);
-        gn[s] = a;
-      }
+      for (int s = 0; s < DIM; ++s) gn[s] = (T)0.5 * (un[f][s + 1] - un[f][0]);
 #pragma unroll
       for (int r = 0; r < DIM; ++r) {
         T al = (T)face_alpha(DIM, 0, r);
@@ -493,19 +493,16 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
         nc = (T)bnv[slip.comp];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
+          const T* ud = slip.u0 + d * slip.cstride;
+          const T u0 = ud[e];
           T t = 0;
 #pragma unroll
-          for (int s2 = 0; s2 < DIM; ++s2) {
-            T a = 0;
-#pragma unroll
-            for (int m = 0; m < Np; ++m)
-              a = fma(C[L.off_D + (s2 * Np) * Np + m], slip.u0[d * slip.cstride + m * stride + e], a);
-            t = fma(beta[s2], a, t);
-          }
+          for (int s2 = 0; s2 < DIM; ++s2) t = fma(beta[s2], (T)0.5 * (ud[(s2 + 1) * stride + e] - u0), t);
           sm0 = fma((T)bnv[d], t, sm0);
         }
       }
     }
+    T F[Nfp], Fs = 0, S = 0;
 #pragma unroll
     for (int j = 0; j < Nfp; ++j) {
       const int p = FM.v[f * Nfp + j];
@@ -529,19 +526,24 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
         Tp = -Tm;
       }
       const T jump = um - up;
-      const T F = (T)0.5 * (ts[f] * jump - (Tm - Tp));
-      const T sig = (T)-0.5 * jump * kk[p];
-#pragma unroll
-      for (int i = 0; i < Nfp; ++i)
-        acc[FM.v[f * Nfp + i]] = fma(C[L.off_MfT + (f * Nfp + j) * Nfp + i], F, acc[FM.v[f * Nfp + i]]);
-#pragma unroll
-      for (int r = 0; r < DIM; ++r) {
-        const T bs = beta[r] * sig;
-#pragma unroll
-        for (int i = 0; i < Np; ++i)
-          acc[i] = fma(C[L.off_Wt + ((f * DIM + r) * Nfp + j) * Np + i], bs, acc[i]);
-      }
+      F[j] = (T)0.5 * (ts[f] * jump - (Tm - Tp));
+      Fs += F[j];
+      S = fma(jump, kk[p], S);
     }
+    // face mass c (1 + delta_ij): sum_j M_ij F_j = c (F_i + sum_j F_j); sig_j = -jump_j k_j / 2
+#pragma unroll
+    for (int i = 0; i < Nfp; ++i) acc[FM.v[f * Nfp + i]] = fma(c_f, F[i] + Fs, acc[FM.v[f * Nfp + i]]);
+    const T sw = c_s * S;
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) V[r] = fma(beta[r], sw, V[r]);
+  }
+  {
+    T v0 = V[0];
+#pragma unroll
+    for (int r = 1; r < DIM; ++r) v0 += V[r];
+    acc[0] = fma((T)-0.5, v0, acc[0]);
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) acc[r + 1] = fma((T)0.5, V[r], acc[r + 1]);
   }
 #pragma unroll
   for (int i = 0; i < Np; ++i) {

@@ -406,6 +406,25 @@ void make_op_tables(const RefElement& re, std::vector<double>& C, std::vector<in
           C[L.off_Wt + ((f * d + r) * Nfp + j) * Np + i] = a;
         }
   for (int i = 0; i < Np * Np; ++i) C[L.off_M + i] = re.M[i];
+  if (N == 1) {
+    // the order-1 kernels take the basis derivatives, the mass column sums and the face mass as
+    // compile-time constants: they must be the tables' values
+    auto near = [](double a, double b) { return std::fabs(a - b) <= 1e-13; };
+    bool ok = true;
+    for (int r = 0; r < d; ++r)
+      for (int k = 0; k < Np; ++k)
+        for (int i = 0; i < Np; ++i) ok = ok && near(re.D[r][(size_t)k * Np + i], p1_dphi(r, i));
+    for (int j = 0; j < Np; ++j) {
+      double a = 0;
+      for (int k = 0; k < Np; ++k) a += re.M[(size_t)k * Np + j];
+      ok = ok && near(a, p1_mass_colsum(d));
+    }
+    for (int f = 0; f < Nf; ++f)
+      for (int i = 0; i < Nfp; ++i)
+        for (int j = 0; j < Nfp; ++j)
+          ok = ok && near(re.Mface[((size_t)f * Nfp + i) * Nfp + j], p1_face_mass_c(d) * (i == j ? 2 : 1));
+    if (!ok) throw std::logic_error("order-1 reference tables differ from the P1 constants");
+  }
   tab.assign((size_t)L.tab_size, -1);
   for (int i = 0; i < Nf * Nfp; ++i) tab[L.tab_fmask + i] = re.fmask[i];
   for (int f = 0; f < Nf; ++f)
