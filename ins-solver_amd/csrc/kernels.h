@@ -21,6 +21,9 @@ struct MeshView {
   // [Nf][stride] (neighbour << 5) | code inside the mesh, -(bface+1) on the boundary; null when
   // the mesh has 2^26 cells or more (the element passes then read nbr and code)
   const int* nbc = nullptr;
+  // [E][NK] the metric cell by cell (3D; null otherwise): the order-1 pass gathers a
+  // neighbour's metric as three 16-byte loads from one or two cache lines instead of six rows
+  const double* Ka = nullptr;
 };
 
 // Boundary faces of the vector (velocity) operator with the slip condition (bc type 2 on

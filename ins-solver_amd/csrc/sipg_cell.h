@@ -359,7 +359,7 @@ struct BitNodes {
 };
 
 template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip,
-          class Nodes = TabNodes>
+          class Nodes = TabNodes, bool KAOS = false>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -389,9 +389,24 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     acc[m] = 0;
     kk[m] = kval<VAR>(kfac, m * stride + e);
   }
-  T K[NK];
+  // KAOS: Kt holds the metric cell by cell ([E][NK]) instead of in rows
+  static_assert(!KAOS || NK % 2 == 0, "the cell-by-cell metric is read in pairs");
+  auto metric = [&](int64_t q, T* dst) {
+    if constexpr (KAOS) {
+      const double2* kq = reinterpret_cast<const double2*>(Kt) + q * (NK / 2);
 #pragma unroll
-  for (int k = 0; k < NK; ++k) K[k] = (T)Kt[k * stride + e];
+      for (int k = 0; k < NK / 2; ++k) {
+        const double2 v = kq[k];
+        dst[2 * k] = (T)v.x;
+        dst[2 * k + 1] = (T)v.y;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NK; ++k) dst[k] = (T)Kt[k * stride + q];
+    }
+  };
+  T K[NK];
+  metric(e, K);
   T ts[Nf];
 #pragma unroll
   for (int f = 0; f < Nf; ++f) ts[f] = (T)tauS[f * stride + e];
@@ -408,8 +423,7 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
       un[f][m] = u[m * stride + q];
       fn_k[f][m] = kval<VAR>(kfac, m * stride + q);
     }
-#pragma unroll
-    for (int k = 0; k < NK; ++k) kn[f][k] = (T)Kt[k * stride + q];
+    metric(q, kn[f]);
 #pragma unroll
     for (int j = 0; j < Nfp; ++j) {
       if constexpr (std::is_same<Nodes, BitNodes>::value)  // 3D: the same row for every face

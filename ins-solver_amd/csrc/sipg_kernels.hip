@@ -72,7 +72,7 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
 }
 
 
-template <int DIM, typename T, bool VAR, bool PACK>
+template <int DIM, typename T, bool VAR, bool PACK, bool KAOS>
 __global__ void __launch_bounds__(256)
 k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
            const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -82,7 +82,7 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_p1_cell<DIM, T, VAR, PACK, T*, NoSlip, BitNodes>(
+    dotv = apply_p1_cell<DIM, T, VAR, PACK, T*, NoSlip, BitNodes, KAOS>(
         e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab), out, NoSlip{},
         BitNodes{bits});
   if (partial) {
@@ -108,7 +108,7 @@ k_apply_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const T* __rest
 }
 
 // Order-1 element pass with the Chebyshev step fused into its output.
-template <int DIM, typename T, bool VAR, bool PACK>
+template <int DIM, typename T, bool VAR, bool PACK, bool KAOS>
 __global__ void __launch_bounds__(256)
 k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
                 const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -118,7 +118,7 @@ k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double
                 T* __restrict__ x_next, T c1, T c2, P1NodeBits bits) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   if (e < E)
-    (void)apply_p1_cell<DIM, T, VAR, PACK, ChebOut<T>, NoSlip, BitNodes>(
+    (void)apply_p1_cell<DIM, T, VAR, PACK, ChebOut<T>, NoSlip, BitNodes, KAOS>(
         e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab),
         ChebOut<T>{f, inv, d, x_next, c1, c2}, NoSlip{}, BitNodes{bits});
 }
@@ -432,15 +432,19 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
     const bool pack = m.nbc != nullptr;
     const int* nw = pack ? m.nbc : m.nbr;
     const P1NodeBits& bits = p1_node_bits(m.dim);
-#define INSP_P1(D, V, P)                                                                          \
-  hipLaunchKernelGGL((k_apply_p1<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, u,  \
-                     m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial, bits)
+#define INSP_P1(D, V, P, A)                                                                       \
+  hipLaunchKernelGGL((k_apply_p1<D, T, V, P, A>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, \
+                     u, A ? m.Ka : m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial, \
+                     bits)
     if (m.dim == 2) {
-      if (kfac) { if (pack) INSP_P1(2, true, true); else INSP_P1(2, true, false); }
-      else { if (pack) INSP_P1(2, false, true); else INSP_P1(2, false, false); }
+      if (kfac) { if (pack) INSP_P1(2, true, true, false); else INSP_P1(2, true, false, false); }
+      else { if (pack) INSP_P1(2, false, true, false); else INSP_P1(2, false, false, false); }
+    } else if (m.Ka) {
+      if (kfac) { if (pack) INSP_P1(3, true, true, true); else INSP_P1(3, true, false, true); }
+      else { if (pack) INSP_P1(3, false, true, true); else INSP_P1(3, false, false, true); }
     } else {
-      if (kfac) { if (pack) INSP_P1(3, true, true); else INSP_P1(3, true, false); }
-      else { if (pack) INSP_P1(3, false, true); else INSP_P1(3, false, false); }
+      if (kfac) { if (pack) INSP_P1(3, true, true, false); else INSP_P1(3, true, false, false); }
+      else { if (pack) INSP_P1(3, false, true, false); else INSP_P1(3, false, false, false); }
     }
 #undef INSP_P1
     launch_check("k_apply_p1");
@@ -483,16 +487,19 @@ bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const 
   const int* nw = pack ? m.nbc : m.nbr;
   if (l.N == 1) {
     const P1NodeBits& bits = p1_node_bits(m.dim);
-#define INSP_P1C(D, V, P)                                                                         \
-  hipLaunchKernelGGL((k_apply_p1_cheb<D, T, V, P>), grid_for(m.E), dim3(256), 0, s, m.E, m.stride, \
-                     u, m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, inv, d, x_next, c1, c2, \
-                     bits)
+#define INSP_P1C(D, V, P, A)                                                                      \
+  hipLaunchKernelGGL((k_apply_p1_cheb<D, T, V, P, A>), grid_for(m.E), dim3(256), 0, s, m.E,       \
+                     m.stride, u, A ? m.Ka : m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, \
+                     inv, d, x_next, c1, c2, bits)
     if (m.dim == 2) {
-      if (kfac) { if (pack) INSP_P1C(2, true, true); else INSP_P1C(2, true, false); }
-      else { if (pack) INSP_P1C(2, false, true); else INSP_P1C(2, false, false); }
+      if (kfac) { if (pack) INSP_P1C(2, true, true, false); else INSP_P1C(2, true, false, false); }
+      else { if (pack) INSP_P1C(2, false, true, false); else INSP_P1C(2, false, false, false); }
+    } else if (m.Ka) {
+      if (kfac) { if (pack) INSP_P1C(3, true, true, true); else INSP_P1C(3, true, false, true); }
+      else { if (pack) INSP_P1C(3, false, true, true); else INSP_P1C(3, false, false, true); }
     } else {
-      if (kfac) { if (pack) INSP_P1C(3, true, true); else INSP_P1C(3, true, false); }
-      else { if (pack) INSP_P1C(3, false, true); else INSP_P1C(3, false, false); }
+      if (kfac) { if (pack) INSP_P1C(3, true, true, false); else INSP_P1C(3, true, false, false); }
+      else { if (pack) INSP_P1C(3, false, true, false); else INSP_P1C(3, false, false, false); }
     }
 #undef INSP_P1C
     launch_check("k_apply_p1_cheb");

@@ -123,6 +123,7 @@ PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
     }
   }
   Kt_.upload(kt.data(), kt.size());
+  if (d == 3) Ka_.upload(mesh_.Kt.data(), (size_t)E * NK);
   J_.upload(jj.data(), jj.size());
   nbr_.upload(nb.data(), nb.size());
   code_.upload(cd.data(), cd.size());
@@ -161,6 +162,7 @@ MeshView PoissonSolver::mesh_view() const {
   m.code = code_.p;
   m.bctype = bctype_.p;
   m.nbc = nbc_.empty() ? nullptr : nbc_.p;
+  m.Ka = Ka_.empty() ? nullptr : Ka_.p;
   return m;
 }
 

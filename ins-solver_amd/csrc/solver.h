@@ -270,6 +270,7 @@ class PoissonSolver {
   int64_t stride_;
   // device mesh
   DBuf<double> Kt_, J_;
+  DBuf<double> Ka_;  // 3D: the metric cell by cell, for the order-1 pass
   DBuf<int> nbr_, code_, bctype_;
   DBuf<int> nbc_;  // packed neighbour words of the element passes (empty: not packed)
   DBuf<double> bc_;
