@@ -64,13 +64,13 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
                   T* out, hipStream_t s, double* partial = nullptr);
 int apply_blocks(int64_t E);
 // Element pass (vector kernels, order 1 included) with one Chebyshev step of the smoother in
-// place of the store: res = inv (f - A u), d = c1 d + c2 res (d not read when c1 = 0),
-// x_next = u + d. x_next must differ from u. Returns false, launching nothing, where the level
-// takes the matrix-core element pass.
+// place of the store, in three-term form: res = inv (f - A u), x_next = u + c1 (u - prev) +
+// c2 res (prev not read when c1 = 0, zero when null). x_next must differ from u and may be prev.
+// Returns false, launching nothing, where the level takes the matrix-core element pass.
 template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
-                       const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
-                       hipStream_t s);
+                       const T* kfac, const T* f, const T* inv, const T* prev, T* x_next, T c1,
+                       T c2, hipStream_t s);
 // Element pass of component slip.comp of the vector (velocity) operator, FP64: the scalar
 // operator of that component, except that faces of bc type 2 are slip faces, which read the
 // other components' values and face traces of the cell (the traces of every component from

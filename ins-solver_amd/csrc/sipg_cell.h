@@ -102,19 +102,21 @@ __device__ __forceinline__ void store_out(const O& o, int64_t k, T acc, T u) {
   o(k, acc, u);
 }
 
-// One Chebyshev step taken in place of the store: res = D^-1 (f - A x), d = c1 d + c2 res (d is
-// not read when c1 = 0), x_next = x + d into another buffer (the pass still reads x).
+// One Chebyshev step taken in place of the store, in three-term form: res = D^-1 (f - A x),
+// x_next = x + c1 (x - x_prev) + c2 res (x_prev not read when c1 = 0, zero when null). x_next goes
+// to another buffer than x (the pass still reads x); it may be x_prev's, each entry being read
+// before it is written by the same thread.
 template <typename T>
 struct ChebOut {
   const T* f;
   const T* inv;
-  T* d;
+  const T* prev;
   T* x_next;
   T c1, c2;
   __device__ __forceinline__ void operator()(int64_t k, T acc, T u) const {
     const T res = inv[k] * (f[k] - acc);
-    const T di = c1 == (T)0 ? c2 * res : fma(c1, d[k], c2 * res);
-    d[k] = di;
+    T di = c2 * res;
+    if (c1 != (T)0) di = fma(c1, prev ? u - prev[k] : u, di);
     x_next[k] = u + di;
   }
 };

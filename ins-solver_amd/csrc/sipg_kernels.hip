@@ -99,12 +99,12 @@ k_apply_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const T* __rest
              const int* __restrict__ nbr, const int* __restrict__ code,
              const int* __restrict__ bctype, const T* __restrict__ kfac, const T* __restrict__ C,
              const int* __restrict__ tab, const T* __restrict__ f, const T* __restrict__ inv,
-             T* __restrict__ d, T* __restrict__ x_next, T c1, T c2) {
+             const T* prev, T* x_next, T c1, T c2) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   if (e < E)
     (void)apply_cell<DIM, N, T, VAR, PACK>(e, stride, u, tr, Kt, tauS, nbr, code, bctype, kfac,
                                            as_const(C), as_const(tab),
-                                           ChebOut<T>{f, inv, d, x_next, c1, c2});
+                                           ChebOut<T>{f, inv, prev, x_next, c1, c2});
 }
 
 // Order-1 element pass with the Chebyshev step fused into its output.
@@ -114,13 +114,13 @@ k_apply_p1_cheb(int64_t E, int64_t stride, const T* __restrict__ u, const double
                 const double* __restrict__ tauS, const int* __restrict__ nbr,
                 const int* __restrict__ code, const int* __restrict__ bctype,
                 const T* __restrict__ kfac, const T* __restrict__ C, const int* __restrict__ tab,
-                const T* __restrict__ f, const T* __restrict__ inv, T* __restrict__ d,
-                T* __restrict__ x_next, T c1, T c2, P1NodeBits bits) {
+                const T* __restrict__ f, const T* __restrict__ inv, const T* prev, T* x_next,
+                T c1, T c2, P1NodeBits bits) {
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   if (e < E)
     (void)apply_p1_cell<DIM, T, VAR, PACK, ChebOut<T>, NoSlip, BitNodes, KAOS>(
         e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab),
-        ChebOut<T>{f, inv, d, x_next, c1, c2}, NoSlip{}, BitNodes{bits});
+        ChebOut<T>{f, inv, prev, x_next, c1, c2}, NoSlip{}, BitNodes{bits});
 }
 
 // Element pass of one component of the vector (velocity) operator with slip faces (FP64;
@@ -477,8 +477,8 @@ void launch_apply_slip(const MeshView& m, const LevelView& l, const double* u, c
 
 template <typename T>
 bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const T* tr,
-                       const T* kfac, const T* f, const T* inv, T* d, T* x_next, T c1, T c2,
-                       hipStream_t s) {
+                       const T* kfac, const T* f, const T* inv, const T* prev, T* x_next, T c1,
+                       T c2, hipStream_t s) {
   if (l.frag && mfma_apply_supported(m.dim, l.N)) return false;
   if (x_next == u) throw std::invalid_argument("fused Chebyshev step: x_next must not alias x");
   if (m.E == 0) return true;
@@ -490,7 +490,7 @@ bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const 
 #define INSP_P1C(D, V, P, A)                                                                      \
   hipLaunchKernelGGL((k_apply_p1_cheb<D, T, V, P, A>), grid_for(m.E), dim3(256), 0, s, m.E,       \
                      m.stride, u, A ? m.Ka : m.Kt, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, f, \
-                     inv, d, x_next, c1, c2, bits)
+                     inv, prev, x_next, c1, c2, bits)
     if (m.dim == 2) {
       if (kfac) { if (pack) INSP_P1C(2, true, true, false); else INSP_P1C(2, true, false, false); }
       else { if (pack) INSP_P1C(2, false, true, false); else INSP_P1C(2, false, false, false); }
@@ -509,7 +509,7 @@ bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const 
   INSP_DISPATCH_N(m.dim, l.N,                                                                     \
                   hipLaunchKernelGGL((k_apply_cheb<D_, N_, T, V, P>), grid_for(m.E), dim3(256), 0, \
                                      s, m.E, m.stride, u, tr, m.Kt, l.tauS, nw, m.code, m.bctype, \
-                                     kfac, C, l.tab, f, inv, d, x_next, c1, c2))
+                                     kfac, C, l.tab, f, inv, prev, x_next, c1, c2))
   if (kfac) { if (pack) INSP_CHEB(true, true); else INSP_CHEB(true, false); }
   else { if (pack) INSP_CHEB(false, true); else INSP_CHEB(false, false); }
 #undef INSP_CHEB
@@ -518,10 +518,10 @@ bool launch_apply_cheb(const MeshView& m, const LevelView& l, const T* u, const 
 }
 template bool launch_apply_cheb<double>(const MeshView&, const LevelView&, const double*,
                                         const double*, const double*, const double*, const double*,
-                                        double*, double*, double, double, hipStream_t);
+                                        const double*, double*, double, double, hipStream_t);
 template bool launch_apply_cheb<float>(const MeshView&, const LevelView&, const float*,
                                        const float*, const float*, const float*, const float*,
-                                       float*, float*, float, float, hipStream_t);
+                                       const float*, float*, float, float, hipStream_t);
 
 template void launch_trace<double>(const MeshView&, const LevelView&, const double*, const double*,
                                    double*, hipStream_t);

@@ -765,18 +765,23 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   // A step after an apply: fused into the element pass where the level has the vector pass and
-  // no mass term (the next iterate goes to x2, then x and x2 trade places), else apply + stream.
+  // no mass term, else apply + stream. The fused step is the three-term form
+  // x_next = x + c1 (x - x_prev) + c2 D^-1 (f - A x): x_next overwrites x_prev in x2, then x and
+  // x2 trade places, so x2 holds the previous iterate at every step (d = x - x_prev is not stored).
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
   const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && !(oi_ && l == 0) &&
                     !(v.frag && mfma_apply_supported(mv.dim, v.N));
+  bool prev_zero = false;  // x_prev = 0: the first iterate came from x = 0 by the stream step
   auto step = [&](T c1, T c2) {
     ++smoother_applies_;
     if (fuse) {
       if (w.tr.empty()) w.tr.alloc((size_t)lv.L.Nf * lv.L.Nfp * stride_);
       const T* k = var_factor_ ? w.k.p : nullptr;
       launch_trace<T>(mv, v, x, k, w.tr.p, stream_);
-      if (!launch_apply_cheb<T>(mv, v, x, w.tr.p, k, f, w.inv.p, w.d.p, w.x2.p, c1, c2, stream_))
+      const T* prev = prev_zero ? nullptr : w.x2.p;
+      prev_zero = false;
+      if (!launch_apply_cheb<T>(mv, v, x, w.tr.p, k, f, w.inv.p, prev, w.x2.p, c1, c2, stream_))
         throw std::logic_error("fused Chebyshev step not available on this level");
       std::swap(w.x, w.x2);
       x = w.x.p;
@@ -785,9 +790,10 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
     apply_level<T>(l, x, w.ax.p);
     cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, c1, c2, false, stream_);
   };
-  if (zero_init)
+  if (zero_init) {
     cheb_step<T>(n, f, nullptr, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), true, stream_);
-  else
+    prev_zero = true;
+  } else
     step((T)0, (T)(1.0 / theta));
   for (int k = 2; k <= K; ++k) {
     const double rho = 1.0 / (2.0 * sigma - rho_old);
