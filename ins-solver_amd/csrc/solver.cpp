@@ -765,12 +765,14 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   // A step after an apply: fused into the element pass where the level has the vector pass and
-  // no mass term, else apply + stream. The fused step is the three-term form
-  // x_next = x + c1 (x - x_prev) + c2 D^-1 (f - A x): x_next overwrites x_prev in x2, then x and
-  // x2 trade places, so x2 holds the previous iterate at every step (d = x - x_prev is not stored).
+  // no mass term, else apply + stream. Where x is the level's own iterate the step is the
+  // three-term form x_next = x + c1 (x - x_prev) + c2 D^-1 (f - A x): x_next overwrites x_prev in
+  // x2, then x and x2 trade places, so x2 holds the previous iterate at every step
+  // (d = x - x_prev is not stored).
   const MeshView mv = mesh_view();
   const LevelView v = level_view(l, std::is_same<T, float>::value);
-  const bool fuse = x == w.x.p && !w.x2.empty() && !has_mass_term() && !(oi_ && l == 0) &&
+  const bool three = x == w.x.p && !w.x2.empty();
+  const bool fuse = three && !has_mass_term() && !(oi_ && l == 0) &&
                     !(v.frag && mfma_apply_supported(mv.dim, v.N));
   bool prev_zero = false;  // x_prev = 0: the first iterate came from x = 0 by the stream step
   auto step = [&](T c1, T c2) {
@@ -788,10 +790,20 @@ void PoissonSolver::chebyshev(int l, const T* f, T* x, bool zero_init) {
       return;
     }
     apply_level<T>(l, x, w.ax.p);
+    if (three) {
+      const T* prev = prev_zero ? nullptr : w.x2.p;
+      prev_zero = false;
+      cheb_step3<T>(n, f, w.ax.p, w.inv.p, prev, x, w.x2.p, c1, c2, stream_);
+      std::swap(w.x, w.x2);
+      x = w.x.p;
+      return;
+    }
     cheb_step<T>(n, f, w.ax.p, w.inv.p, w.d.p, x, c1, c2, false, stream_);
   };
   if (zero_init) {
-    cheb_step<T>(n, f, nullptr, w.inv.p, w.d.p, x, (T)0, (T)(1.0 / theta), true, stream_);
+    // the first iterate from x = 0; the three-term steps need no d
+    cheb_step<T>(n, f, nullptr, w.inv.p, three ? nullptr : w.d.p, x, (T)0, (T)(1.0 / theta), true,
+                 stream_);
     prev_zero = true;
   } else
     step((T)0, (T)(1.0 / theta));

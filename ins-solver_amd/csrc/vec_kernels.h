@@ -72,10 +72,15 @@ void cg_update_x2p(int64_t n, T* x, T* p0, const T* p1, const T* z, const double
 template <typename T>
 void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s);
 // Chebyshev step: r = invdiag*(f - Ax) (Ax may be null = 0); d = c1*d + c2*r; x += d
-// (x = d when x_zero).
+// (x = d when x_zero). d may be null when c1 = 0 (then it is neither read nor written).
 template <typename T>
 void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x, T c1, T c2,
                bool x_zero, hipStream_t s);
+// Chebyshev step in three-term form: x_next = x + c1 (x - prev) + c2 invdiag (f - Ax), prev not
+// read when c1 = 0 and zero when null. x_next must differ from x and may be prev.
+template <typename T>
+void cheb_step3(int64_t n, const T* f, const T* Ax, const T* invdiag, const T* prev, const T* x,
+                T* x_next, T c1, T c2, hipStream_t s);
 // y = x converted
 template <typename Tin, typename Tout>
 void convert(int64_t n, const Tin* x, Tout* y, hipStream_t s);

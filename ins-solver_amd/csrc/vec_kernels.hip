@@ -243,8 +243,23 @@ k_cheb_step(int64_t n, const T* __restrict__ f, const T* __restrict__ Ax,
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const T res = invdiag[i] * (Ax ? f[i] - Ax[i] : f[i]);
     const T di = c1 == (T)0 ? c2 * res : fma(c1, d[i], c2 * res);
-    d[i] = di;
+    if (d) d[i] = di;
     x[i] = x_zero ? di : x[i] + di;
+  }
+}
+
+// Three-term form: x_next = x + c1 (x - prev) + c2 D^-1 (f - Ax), prev not read when c1 = 0 and
+// zero when null; x_next may be prev (each entry is read before it is written).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cheb_step3(int64_t n, const T* __restrict__ f, const T* __restrict__ Ax,
+             const T* __restrict__ invdiag, const T* prev, const T* __restrict__ x, T* x_next, T c1,
+             T c2) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T xi = x[i];
+    T di = c2 * (invdiag[i] * (f[i] - Ax[i]));
+    if (c1 != (T)0) di = fma(c1, prev ? xi - prev[i] : xi, di);
+    x_next[i] = xi + di;
   }
 }
 
@@ -458,6 +473,14 @@ void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x,
                      d, x, c1, c2, x_zero); launch_check("k_cheb_step");
 }
 
+template <typename T>
+void cheb_step3(int64_t n, const T* f, const T* Ax, const T* invdiag, const T* prev, const T* x,
+                T* x_next, T c1, T c2, hipStream_t s) {
+  if (x_next == x) throw std::invalid_argument("three-term Chebyshev step: x_next must not alias x");
+  hipLaunchKernelGGL((k_cheb_step3<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, f, Ax, invdiag,
+                     prev, x, x_next, c1, c2); launch_check("k_cheb_step3");
+}
+
 template <typename Tin, typename Tout>
 void convert(int64_t n, const Tin* x, Tout* y, hipStream_t s) {
   hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, y); launch_check("k_convert");
@@ -545,6 +568,8 @@ void scal_copy(double* scal, int dst, int src, hipStream_t s) {
   template void jacobi<T>(int64_t, const T*, const T*, T*, hipStream_t);                          \
   template void cheb_step<T>(int64_t, const T*, const T*, const T*, T*, T*, T, T, bool,           \
                              hipStream_t);                                                        \
+  template void cheb_step3<T>(int64_t, const T*, const T*, const T*, const T*, const T*, T*, T, T, \
+                              hipStream_t);                                                       \
   template void fill<T>(int64_t, T*, T, hipStream_t);                                             \
   template void copy<T>(int64_t, const T*, T*, hipStream_t);                                      \
   template void axpby<T>(int64_t, T, const T*, T, T*, hipStream_t);                               \
