@@ -373,18 +373,48 @@ __global__ void __launch_bounds__(TB) kb_vertex_prolong(BatchView b) {
   }
 }
 
+// CSR products over a 256-row tile of AMG level k: groups of kCsrLanes adjacent lanes share a
+// row (lane l takes the row's nonzeros l, l + kCsrLanes, ...) and add their partial sums by
+// shuffles in a fixed order; each group walks kCsrLanes rows of the tile. Returns the tile's
+// problem slot and first row, or slot -1 when the block has no live tile.
+constexpr int kCsrLanes = 8;
+__device__ __forceinline__ int2 locate_tile(const BatchView& b, int k) {
+  const int t = (int)xcd_block();
+  if (t >= b.lcount[k]) return make_int2(-1, 0);
+  const int2 tt = b.ltiles[(int64_t)k * b.ltile_cap + t];
+  return frozen(b.P[tt.x]) ? make_int2(-1, 0) : tt;
+}
+
+template <typename T>
+__device__ __forceinline__ T csr_row_dot(const int* __restrict__ ptr, const int* __restrict__ col,
+                                         const T* __restrict__ val, const T* __restrict__ x, int r,
+                                         bool live) {
+  const int l = (int)threadIdx.x % kCsrLanes;
+  T s = 0;
+  if (live)
+    for (int q = ptr[r] + l; q < ptr[r + 1]; q += kCsrLanes) s = fma(val[q], x[col[q]], s);
+#pragma unroll
+  for (int o = kCsrLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kCsrLanes);
+  return s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kB) kb_amg_spmv(BatchView b, int k, int residual) {
-  const Row c = locate_row(b, k);
-  if (!c.valid) return;
-  const BatchProb& P = b.P[c.slot];
+  const int2 tt = locate_tile(b, k);
+  if (tt.x < 0) return;  // the whole block
+  const BatchProb& P = b.P[tt.x];
   if (k >= P.amg.nlev - 1) return;  // the last level is solved densely
   const BatchAmgLevel& L = P.amg.lev[k];
   const T* val = (const T*)L.Aval;
   const T* x = (const T*)L.x;
-  T s = 0;
-  for (int q = L.Aptr[c.r]; q < L.Aptr[c.r + 1]; ++q) s = fma(val[q], x[L.Acol[q]], s);
-  ((T*)L.r)[c.r] = residual ? ((const T*)L.b)[c.r] - s : s;
+  constexpr int G = kB / kCsrLanes;
+  const int g = (int)threadIdx.x / kCsrLanes;
+  for (int i = 0; i < kCsrLanes; ++i) {
+    const int r = tt.y + g + G * i;
+    const bool live = r < L.n;
+    const T s = csr_row_dot<T>(L.Aptr, L.Acol, val, x, r, live);
+    if (live && threadIdx.x % kCsrLanes == 0) ((T*)L.r)[r] = residual ? ((const T*)L.b)[r] - s : s;
+  }
 }
 
 template <typename T>
@@ -406,15 +436,21 @@ __global__ void __launch_bounds__(kB) kb_amg_cheb(BatchView b, int k, int step, 
 // Rows of level k+1: b_{k+1} = R_k r_k.
 template <typename T>
 __global__ void __launch_bounds__(kB) kb_amg_restrict(BatchView b, int k) {
-  const Row c = locate_row(b, k + 1);
-  if (!c.valid) return;
-  const BatchProb& P = b.P[c.slot];
+  const int2 tt = locate_tile(b, k + 1);
+  if (tt.x < 0) return;  // the whole block
+  const BatchProb& P = b.P[tt.x];
   const BatchAmgLevel& L = P.amg.lev[k];
+  const int nc = P.amg.lev[k + 1].n;
   const T* val = (const T*)L.Rval;
-  const T* r = (const T*)L.r;
-  T s = 0;
-  for (int q = L.Rptr[c.r]; q < L.Rptr[c.r + 1]; ++q) s = fma(val[q], r[L.Rcol[q]], s);
-  ((T*)P.amg.lev[k + 1].b)[c.r] = s;
+  const T* res = (const T*)L.r;
+  constexpr int G = kB / kCsrLanes;
+  const int g = (int)threadIdx.x / kCsrLanes;
+  for (int i = 0; i < kCsrLanes; ++i) {
+    const int r = tt.y + g + G * i;
+    const bool live = r < nc;
+    const T s = csr_row_dot<T>(L.Rptr, L.Rcol, val, res, r, live);
+    if (live && threadIdx.x % kCsrLanes == 0) ((T*)P.amg.lev[k + 1].b)[r] = s;
+  }
 }
 
 // Rows of level k: x_k += P_k x_{k+1}.
