@@ -63,6 +63,11 @@ template <typename T>
 void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr, const T* kfac,
                   T* out, hipStream_t s, double* partial = nullptr);
 int apply_blocks(int64_t E);
+// Order-1 apply (3D, with the cell-by-cell metric) on values held cell by cell: u and out are
+// [E][4] (the coarse Jacobi-CG runs on such copies); the coefficient stays in rows.
+template <typename T>
+void launch_apply_p1_cells(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* out,
+                           hipStream_t s, double* partial = nullptr);
 // Element pass (vector kernels, order 1 included) with one Chebyshev step of the smoother in
 // place of the store, in three-term form: res = inv (f - A u), x_next = u + c1 (u - prev) +
 // c2 res (prev not read when c1 = 0, zero when null). x_next must differ from u and may be prev.

@@ -361,7 +361,7 @@ struct BitNodes {
 };
 
 template <int DIM, typename T, int VAR, bool PACK = false, class Out = T*, class Slip = NoSlip,
-          class Nodes = TabNodes, bool KAOS = false>
+          class Nodes = TabNodes, bool KAOS = false, bool VCM = false>
 __device__ __forceinline__ double
 apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
               const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -384,10 +384,28 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
       c[f] = nb[f] >= 0 ? code[f * stride + e] : 0;
     }
   }
+  // VCM: u and out hold the values cell by cell ([E][Np], 3D: 16 or 32 bytes per cell) instead of
+  // in rows; the coefficient stays in rows
+  static_assert(!VCM || Np == 4, "cell-by-cell values: 3D order 1");
+  auto values = [&](int64_t q, T* dst) {
+    if constexpr (VCM) {
+      if constexpr (sizeof(T) == 8) {
+        const double2* uq = reinterpret_cast<const double2*>(u) + q * 2;
+        const double2 a = uq[0], b = uq[1];
+        dst[0] = a.x; dst[1] = a.y; dst[2] = b.x; dst[3] = b.y;
+      } else {
+        const float4 a = reinterpret_cast<const float4*>(u)[q];
+        dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < Np; ++m) dst[m] = u[m * stride + q];
+    }
+  };
   T uu[Np], acc[Np], kk[Np];
+  values(e, uu);
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
-    uu[m] = u[m * stride + e];
     acc[m] = 0;
     kk[m] = kval<VAR>(kfac, m * stride + e);
   }
@@ -420,11 +438,9 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
     const bool in = nb[f] >= 0;
     const int64_t q = in ? (int64_t)nb[f] : e;
     const int cf = in ? c[f] : 0;
+    values(q, un[f]);
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      un[f][m] = u[m * stride + q];
-      fn_k[f][m] = kval<VAR>(kfac, m * stride + q);
-    }
+    for (int m = 0; m < Np; ++m) fn_k[f][m] = kval<VAR>(kfac, m * stride + q);
     metric(q, kn[f]);
 #pragma unroll
     for (int j = 0; j < Nfp; ++j) {
@@ -561,11 +577,21 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
 #pragma unroll
     for (int r = 0; r < DIM; ++r) acc[r + 1] = fma((T)0.5, V[r], acc[r + 1]);
   }
+  if constexpr (VCM) {
+    static_assert(std::is_same<Out, T*>::value, "cell-by-cell values: plain output");
+    if constexpr (sizeof(T) == 8) {
+      double2* oq = reinterpret_cast<double2*>(out) + e * 2;
+      oq[0] = make_double2(acc[0], acc[1]);
+      oq[1] = make_double2(acc[2], acc[3]);
+    } else {
+      reinterpret_cast<float4*>(out)[e] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+  } else {
 #pragma unroll
-  for (int i = 0; i < Np; ++i) {
-    store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
-    dotv += (double)uu[i] * (double)acc[i];
+    for (int i = 0; i < Np; ++i) store_out(out, (int64_t)i * stride + e, acc[i], uu[i]);
   }
+#pragma unroll
+  for (int i = 0; i < Np; ++i) dotv += (double)uu[i] * (double)acc[i];
   return dotv;
 }
 

@@ -72,7 +72,7 @@ k_apply(int64_t E, int64_t stride, const T* __restrict__ u, const T* __restrict_
 }
 
 
-template <int DIM, typename T, bool VAR, bool PACK, bool KAOS>
+template <int DIM, typename T, bool VAR, bool PACK, bool KAOS, bool VCM = false>
 __global__ void __launch_bounds__(256)
 k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __restrict__ Kt,
            const double* __restrict__ tauS, const int* __restrict__ nbr,
@@ -82,7 +82,7 @@ k_apply_p1(int64_t E, int64_t stride, const T* __restrict__ u, const double* __r
   const int64_t e = xcd_block() * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (e < E)
-    dotv = apply_p1_cell<DIM, T, VAR, PACK, T*, NoSlip, BitNodes, KAOS>(
+    dotv = apply_p1_cell<DIM, T, VAR, PACK, T*, NoSlip, BitNodes, KAOS, VCM>(
         e, stride, u, Kt, tauS, nbr, code, bctype, kfac, as_const(C), as_const(tab), out, NoSlip{},
         BitNodes{bits});
   if (partial) {
@@ -461,6 +461,30 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
   else { if (pack) INSP_APPLY(false, true); else INSP_APPLY(false, false); }
 #undef INSP_APPLY
 }
+
+template <typename T>
+void launch_apply_p1_cells(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* out,
+                           hipStream_t s, double* partial) {
+  if (m.dim != 3 || l.N != 1 || !m.Ka)
+    throw std::invalid_argument("cell-by-cell order-1 apply: 3D order 1 with the cell-by-cell metric");
+  if (m.E == 0) return;
+  const T* C = static_cast<const T*>(l.C);
+  const bool pack = m.nbc != nullptr;
+  const int* nw = pack ? m.nbc : m.nbr;
+  const P1NodeBits& bits = p1_node_bits(3);
+#define INSP_P1V(V, P)                                                                            \
+  hipLaunchKernelGGL((k_apply_p1<3, T, V, P, true, true>), grid_for(m.E), dim3(256), 0, s, m.E,   \
+                     m.stride, u, m.Ka, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial, \
+                     bits)
+  if (kfac) { if (pack) INSP_P1V(true, true); else INSP_P1V(true, false); }
+  else { if (pack) INSP_P1V(false, true); else INSP_P1V(false, false); }
+#undef INSP_P1V
+  launch_check("k_apply_p1");
+}
+template void launch_apply_p1_cells<double>(const MeshView&, const LevelView&, const double*,
+                                            const double*, double*, hipStream_t, double*);
+template void launch_apply_p1_cells<float>(const MeshView&, const LevelView&, const float*,
+                                           const float*, float*, hipStream_t, double*);
 
 int apply_blocks(int64_t E) { return (int)((E + 255) / 256); }
 

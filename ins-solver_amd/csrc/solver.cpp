@@ -670,7 +670,7 @@ hipEvent_t PoissonSolver::ev_take() {
 }
 
 template <typename T>
-void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial, T* tr) {
+void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial, T* tr, bool cells) {
   Level& lv = *levels_[l];
   Work<T>& w = lv.work<T>();
   const bool own_tr = tr == nullptr;  // a caller-given trace buffer: no allocation, no events
@@ -692,7 +692,11 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial, T* t
     e0 = ev_take(); e2 = ev_take();
     INSP_HIP(hipEventRecord(e0, stream_));
   }
-  if (oi_ && l == 0) {
+  if (cells) {
+    // u and out cell by cell (the coarse Jacobi-CG's copies; 3D order 1, no mass term)
+    if (prof) INSP_HIP(hipEventRecord(e1, stream_));
+    launch_apply_p1_cells<T>(mv, v, u, k, out, stream_, partial);
+  } else if (oi_ && l == 0) {
     if (prof) INSP_HIP(hipEventRecord(e1, stream_));
     launch_apply_oi<T>(mv, v, oi_view<T>(), u, out, stream_, partial);
   } else {
@@ -700,9 +704,11 @@ void PoissonSolver::apply_level(int l, const T* u, T* out, double* partial, T* t
     if (prof) INSP_HIP(hipEventRecord(e1, stream_));
     launch_apply<T>(mv, v, u, tr, k, out, stream_, partial);
   }
-  if (has_mass_term())
+  if (has_mass_term()) {
+    if (cells) throw std::logic_error("cell-by-cell apply with a mass term");
     launch_mass_add<T>(mv, v, lv.L.Np, var_mm_ ? w.mm.p : nullptr, (T)mm_const_, u, out, stream_,
                        partial);
+  }
   if (prof) {
     INSP_HIP(hipEventRecord(e2, stream_));
     ev_trace_.push_back({e0, e1});
@@ -825,10 +831,25 @@ void PoissonSolver::coarse_cg(int l) {
   // brought up to date every second iteration, from the two last directions (two buffers): the
   // same operations in the same order as one x update per iteration, with 4.5 instead of 5
   // vector passes per iteration.
+  //
+  // In 3D at order 1 the iteration runs on cell-by-cell copies (v[e*4 + i]): the apply then reads a
+  // cell's four values, its own and each neighbour's, with one or two 16-byte loads instead of four
+  // rows. b and D^-1 are copied in (into the level's ax and d, unused by this solve) and x is
+  // copied out at the end; the vector kernels are the same over the flat range.
+  const bool cells = mesh_.dim == 3 && lv.N == 1 && !has_mass_term() && !(oi_ && l == 0) &&
+                     !matrix_pass(l) && !Ka_.empty();
+  const T* bb = w.b.p;
+  const T* inv = w.inv.p;
+  if (cells) {
+    rows_to_cells<T>(mesh_.n_cells, stride_, lv.L.Np, w.b.p, w.ax.p, stream_);
+    rows_to_cells<T>(mesh_.n_cells, stride_, lv.L.Np, w.inv.p, w.d.p, stream_);
+    bb = w.ax.p;
+    inv = w.d.p;
+  }
   fill<T>(n, w.x.p, (T)0, stream_);
-  jacobi<T>(n, w.b.p, w.inv.p, w.z.p, stream_);
+  jacobi<T>(n, bb, inv, w.z.p, stream_);
   copy<T>(n, w.z.p, w.p.p, stream_);
-  dot<T>(n, w.b.p, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
+  dot<T>(n, bb, w.z.p, nullptr, nullptr, 1, scal_.p, S_CG_A, rs, stream_);
   int a = S_CG_A, b = S_CG_B;
   T* pc = w.p.p;   // the current direction
   T* pq = w.p2.p;  // the previous one (odd iterations) or the buffer for the next one
@@ -838,11 +859,11 @@ void PoissonSolver::coarse_cg(int l) {
       apply_level<T>(l, pc, w.Ap.p);
       dot<T>(n, pc, w.Ap.p, nullptr, nullptr, 1, scal_.p, S_CG_PAP, rs, stream_);
     } else {
-      apply_level<T>(l, pc, w.Ap.p, partial_.p);  // p.Ap fused into the element pass
+      apply_level<T>(l, pc, w.Ap.p, partial_.p, nullptr, cells);  // p.Ap fused into the pass
       finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_CG_PAP, stream_);
     }
     ++smoother_applies_;
-    cg_update_z<T>(n, w.z.p, w.Ap.p, w.inv.p, scal_.p, a, S_CG_PAP, b, rs, stream_);
+    cg_update_z<T>(n, w.z.p, w.Ap.p, inv, scal_.p, a, S_CG_PAP, b, rs, stream_);
     const bool last = it + 1 == pmg_.coarse_iters;
     if (it % 2 == 1) {
       // x += alpha_{k-1} p_{k-1} + alpha_k p_k; the next direction overwrites p_{k-1}
@@ -856,6 +877,11 @@ void PoissonSolver::coarse_cg(int l) {
       std::swap(pc, pq);
     }
     std::swap(a, b);
+  }
+  if (cells) {
+    // x back to rows through ax, then x and ax trade places
+    cells_to_rows<T>(mesh_.n_cells, stride_, lv.L.Np, w.x.p, w.ax.p, stream_);
+    std::swap(w.x, w.ax);
   }
 }
 

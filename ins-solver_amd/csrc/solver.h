@@ -232,7 +232,8 @@ class PoissonSolver {
   void compute_diagonals();
   void estimate_eigenvalues();
   template <typename T>
-  void apply_level(int l, const T* u, T* out, double* partial = nullptr, T* tr = nullptr);
+  void apply_level(int l, const T* u, T* out, double* partial = nullptr, T* tr = nullptr,
+                   bool cells = false);
   // finest level: coefficient and mass-term factor (null when constant), face nodes per cell
   const double* fine_factor() const;
   const double* fine_mm() const;

@@ -76,6 +76,12 @@ void jacobi(int64_t n, const T* r, const T* invdiag, T* z, hipStream_t s);
 template <typename T>
 void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x, T c1, T c2,
                bool x_zero, hipStream_t s);
+// Np values per cell from rows (v[i*stride + e]) to cell by cell (v[e*Np + i]) and back, out of
+// place, over the whole Np*stride range (padding cells and the cell-by-cell tail written as zero).
+template <typename T>
+void rows_to_cells(int64_t E, int64_t stride, int Np, const T* src, T* dst, hipStream_t s);
+template <typename T>
+void cells_to_rows(int64_t E, int64_t stride, int Np, const T* src, T* dst, hipStream_t s);
 // Chebyshev step in three-term form: x_next = x + c1 (x - prev) + c2 invdiag (f - Ax), prev not
 // read when c1 = 0 and zero when null. x_next must differ from x and may be prev.
 template <typename T>

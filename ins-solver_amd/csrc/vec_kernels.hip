@@ -263,6 +263,28 @@ k_cheb_step3(int64_t n, const T* __restrict__ f, const T* __restrict__ Ax,
   }
 }
 
+// Rows (v[i*stride + e]) to cell by cell (v[e*Np + i]) and back, over the whole Np*stride range:
+// entries of padding cells, and the cell-by-cell tail past Np*E, are written as zero.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_rows_to_cells(int64_t E, int64_t stride, int Np, const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t o = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (o >= Np * stride) return;
+  const int64_t e = o / Np;
+  const int i = (int)(o - e * Np);
+  dst[o] = e < E ? src[i * stride + e] : (T)0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cells_to_rows(int64_t E, int64_t stride, int Np, const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t o = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (o >= Np * stride) return;
+  const int i = (int)(o / stride);
+  const int64_t e = o - i * stride;
+  dst[o] = e < E ? src[e * Np + i] : (T)0;
+}
+
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(kBlock)
 k_convert(int64_t n, const Tin* __restrict__ x, Tout* __restrict__ y) {
@@ -474,6 +496,20 @@ void cheb_step(int64_t n, const T* f, const T* Ax, const T* invdiag, T* d, T* x,
 }
 
 template <typename T>
+void rows_to_cells(int64_t E, int64_t stride, int Np, const T* src, T* dst, hipStream_t s) {
+  if (src == dst) throw std::invalid_argument("rows_to_cells: the copy is out of place");
+  hipLaunchKernelGGL((k_rows_to_cells<T>), dim3(grid_stream(Np * stride)), dim3(kBlock), 0, s, E, stride,
+                     Np, src, dst); launch_check("k_rows_to_cells");
+}
+
+template <typename T>
+void cells_to_rows(int64_t E, int64_t stride, int Np, const T* src, T* dst, hipStream_t s) {
+  if (src == dst) throw std::invalid_argument("cells_to_rows: the copy is out of place");
+  hipLaunchKernelGGL((k_cells_to_rows<T>), dim3(grid_stream(Np * stride)), dim3(kBlock), 0, s, E, stride,
+                     Np, src, dst); launch_check("k_cells_to_rows");
+}
+
+template <typename T>
 void cheb_step3(int64_t n, const T* f, const T* Ax, const T* invdiag, const T* prev, const T* x,
                 T* x_next, T c1, T c2, hipStream_t s) {
   if (x_next == x) throw std::invalid_argument("three-term Chebyshev step: x_next must not alias x");
@@ -570,6 +606,8 @@ void scal_copy(double* scal, int dst, int src, hipStream_t s) {
                              hipStream_t);                                                        \
   template void cheb_step3<T>(int64_t, const T*, const T*, const T*, const T*, const T*, T*, T, T, \
                               hipStream_t);                                                       \
+  template void rows_to_cells<T>(int64_t, int64_t, int, const T*, T*, hipStream_t);               \
+  template void cells_to_rows<T>(int64_t, int64_t, int, const T*, T*, hipStream_t);               \
   template void fill<T>(int64_t, T*, T, hipStream_t);                                             \
   template void copy<T>(int64_t, const T*, T*, hipStream_t);                                      \
   template void axpby<T>(int64_t, T, const T*, T, T*, hipStream_t);                               \
