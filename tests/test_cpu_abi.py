@@ -37,6 +37,38 @@ def test_product_reference_element_matches_oracle(ip, oracle, dim, N):
         assert np.abs(oracle.interp(dim, N, nc) - ip.ref_interp(dim, N, nc)).max() < 1e-12
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_order1_tables_are_the_p1_closed_forms(ip, oracle, dim):
+    """The order-1 kernels compile in the P1 basis on the reference simplex (vertices
+    (-1,..,-1), e_k): phi_0 = -(1 + sum xi)/2, phi_{k+1} = (1 + xi_k)/2, the mass column sums
+    |T|/(dim+1) and the face mass |F|/(dim(dim+1)) (1 + delta_ij) in face coordinates. Both
+    the library's tables and the oracle's must be exactly these (a sign, index or factor slip in
+    either fails here; the library also checks its tables when it builds them)."""
+    np_ = dim + 1
+    dphi = np.zeros((dim, np_))
+    dphi[:, 0] = -0.5
+    for r in range(dim):
+        dphi[r, r + 1] = 0.5
+    vol = 2.0 if dim == 2 else 4.0 / 3.0       # reference triangle / tetrahedron
+    face = 2.0  # the face in face coordinates: the segment [-1, 1] or the reference triangle
+    nfp = dim
+    mface = face / (nfp * (nfp + 1)) * (np.ones((nfp, nfp)) + np.eye(nfp))
+    for R in (ip.ref_element(dim, 1), oracle.ref_element(dim, 1)):
+        D = np.asarray(R["D"]).reshape(dim, np_, np_)
+        for r in range(dim):
+            assert np.abs(D[r] - dphi[r][None, :]).max() < 1e-13  # every row: constant gradient
+        M = np.asarray(R["M"]).reshape(np_, np_)
+        assert np.abs(M.sum(axis=0) - vol / np_).max() < 1e-13
+        Mf = np.asarray(R["Mface"]).reshape(dim + 1, nfp, nfp)
+        assert np.abs(Mf - mface[None]).max() < 1e-13
+    # the closed forms themselves: gradients of the P1 basis at the vertices reproduce the
+    # vertex coordinates' differences, and the face mass integrates 1 to the face size
+    verts = np.vstack([-np.ones(dim), -np.ones((dim, dim)) + 2 * np.eye(dim)])
+    for r in range(dim):
+        assert np.allclose(dphi[r] @ verts[:, r], 1.0)
+    assert np.isclose(mface.sum(), face)
+
+
 @pytest.mark.parametrize("kind", ["square", "cube"])
 def test_mesh_topology_matches_oracle(ip, oracle, kind):
     m = ip.Mesh.square(9, 7, jitter=0.1, seed=3) if kind == "square" else ip.Mesh.cube(4, 0.1, 3)
