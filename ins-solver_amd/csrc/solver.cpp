@@ -17,6 +17,7 @@ enum : int {
   S_MEAN = 16, S_BB = 17,
   S_COUNT = 32
 };
+static_assert(S_BNUM == S_RZN + 1, "dot_rz_flex writes r.z and r.(z - zold) to adjacent slots");
 
 int64_t round_up(int64_t v, int64_t m) { return ((v + m - 1) / m) * m; }
 }  // namespace
@@ -1245,8 +1246,7 @@ int PoissonSolver::fcg_mixed(double tol, int* iters, double* rr_out, int64_t* fi
     if (it >= max_iter_) break;
     std::swap(z32_, zold32_);
     precondition32();
-    dot<float>(n, r, z32_.p, nullptr, nullptr, 1, scal_.p, S_RZN, rs, stream_);
-    dot_flex<float>(n, r, z32_.p, zold32_.p, scal_.p, S_BNUM, rs, stream_);
+    dot_rz_flex<float>(n, r, z32_.p, zold32_.p, scal_.p, S_RZN, rs, stream_);  // and S_BNUM
     cg_update_p<float>(n, p32_.p, z32_.p, scal_.p, S_BNUM, S_RZ, stream_);
     scal_copy(scal_.p, S_RZ, S_RZN, stream_);
   }
@@ -1369,8 +1369,7 @@ int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveSta
         if (std::sqrt(rr) <= tol) break;
         std::swap(z_.p, zold_.p);
         precondition(r_.p, z_.p);
-        dot<double>(n, r_.p, z_.p, nullptr, nullptr, 1, scal_.p, S_RZN, rs, stream_);
-        dot_flex<double>(n, r_.p, z_.p, zold_.p, scal_.p, S_BNUM, rs, stream_);
+        dot_rz_flex<double>(n, r_.p, z_.p, zold_.p, scal_.p, S_RZN, rs, stream_);  // and S_BNUM
         cg_update_p<double>(n, p_.p, z_.p, scal_.p, S_BNUM, S_RZ, stream_);
         scal_copy(scal_.p, S_RZ, S_RZN, stream_);
       }

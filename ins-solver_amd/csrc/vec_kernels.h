@@ -30,6 +30,11 @@ void dot(int64_t n, const T* a, const T* b, const T* c, const T* d, int nd, doub
 template <typename T>
 void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, int slot,
               RedScratch rs, hipStream_t s);
+// scal[slot] = sum(r * z), scal[slot + 1] = sum(r * (z - zold)), in one pass (each sum in the
+// order dot and dot_flex use)
+template <typename T>
+void dot_rz_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, int slot,
+                 RedScratch rs, hipStream_t s);
 // r = b - Ax, scal[slot] = |r|^2
 void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
               RedScratch rs, hipStream_t s);

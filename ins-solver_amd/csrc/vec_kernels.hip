@@ -93,6 +93,19 @@ k_dot_flex(int64_t n, const T* __restrict__ r, const T* __restrict__ z, const T*
   write_partials<1>(v, partial);
 }
 
+// Both inner products of a flexible-CG step in one pass: r.z and r.(z - zold).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dot_rz_flex(int64_t n, const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ zold,
+              double* __restrict__ partial) {
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    v[0] += (double)r[i] * (double)z[i];
+    v[1] += (double)r[i] * ((double)z[i] - (double)zold[i]);
+  }
+  write_partials<2>(v, partial);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ Ax,
            double* __restrict__ r, double* __restrict__ partial) {
@@ -416,6 +429,14 @@ void dot_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, in
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, slot); launch_check("k_finalize");
 }
 
+template <typename T>
+void dot_rz_flex(int64_t n, const T* r, const T* z, const T* zold, double* scal, int slot,
+                 RedScratch rs, hipStream_t s) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL((k_dot_rz_flex<T>), dim3(nb), dim3(kBlock), 0, s, n, r, z, zold, rs.partial); launch_check("k_dot_rz_flex");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 2, rs.partial, scal, slot); launch_check("k_finalize");
+}
+
 void residual(int64_t n, const double* b, const double* Ax, double* r, double* scal, int slot,
               RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
@@ -590,6 +611,8 @@ void scal_copy(double* scal, int dst, int src, hipStream_t s) {
                        RedScratch, hipStream_t);                                                  \
   template void dot_flex<T>(int64_t, const T*, const T*, const T*, double*, int, RedScratch,      \
                             hipStream_t);                                                         \
+  template void dot_rz_flex<T>(int64_t, const T*, const T*, const T*, double*, int, RedScratch,   \
+                               hipStream_t);                                                      \
   template void cg_update_xr<T>(int64_t, T*, T*, const T*, const T*, const T*, T*, double*, int,  \
                                 int, int, RedScratch, hipStream_t);                               \
   template void cg_update_p<T>(int64_t, T*, const T*, const double*, int, int, hipStream_t);      \
