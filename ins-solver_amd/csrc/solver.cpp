@@ -1163,9 +1163,17 @@ void PoissonSolver::precondition(const double* r, double* z) {
     launch_inv_mass(np(), mesh_.n_cells, stride_, J_.p, minv_.p, inv_mass_factor_,
                     inv_mass_recp_.empty() ? nullptr : inv_mass_recp_.p, r, z, stream_);
   } else if (prec_ == Precision::FP64) {
-    copy<double>(n, r, l0.w64.b.p, stream_);
+    // the flexible CG keeps its residual in the finest level's right-hand side (which the
+    // V-cycle only reads) and takes the V-cycle's result by trading buffers with z (every
+    // V-cycle writes its iterate before reading it)
+    if (r != l0.w64.b.p) copy<double>(n, r, l0.w64.b.p, stream_);
     vcycle<double>(0);
-    copy<double>(n, l0.w64.x.p, z, stream_);
+    if (z == z_.p) {
+      std::swap(z_.p, l0.w64.x.p);
+      z = z_.p;
+    } else {
+      copy<double>(n, l0.w64.x.p, z, stream_);
+    }
   } else {
     convert<double, float>(n, r, l0.w32.b.p, stream_);
     vcycle<float>(0);
@@ -1344,23 +1352,26 @@ int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveSta
     status = fcg_mixed(std::max(rtol_ * bnorm, atol_), &it, &rr_true, &fine_applies);
   } else {
     const double tol = std::max(rtol_ * bnorm, atol_);
+    // FP64 p-multigrid: the residual lives in the finest level's right-hand side (no copy into
+    // the V-cycle)
+    double* const r = type_ == SolverType::PMULTIGRID ? levels_[0]->w64.b.p : r_.p;
     for (int restart = 0;; ++restart) {
       apply_level<double>(0, x_.p, Ap_.p);
       ++fine_applies;
-      residual(n, b_.p, Ap_.p, r_.p, scal_.p, S_RR, rs, stream_);
+      residual(n, b_.p, Ap_.p, r, scal_.p, S_RR, rs, stream_);
       rr_true = read_scalar(S_RR);
       if (!std::isfinite(rr_true)) { status = -3; break; }
       if (std::sqrt(rr_true) <= tol) { status = 0; break; }
       if (it >= max_iter_ || restart > 50) { status = 1; break; }
-      precondition(r_.p, z_.p);
-      dot<double>(n, r_.p, z_.p, nullptr, nullptr, 1, scal_.p, S_RZ, rs, stream_);
+      precondition(r, z_.p);
+      dot<double>(n, r, z_.p, nullptr, nullptr, 1, scal_.p, S_RZ, rs, stream_);
       copy<double>(n, z_.p, p_.p, stream_);
       bool breakdown = false;
       while (it < max_iter_) {
         apply_level<double>(0, p_.p, Ap_.p, partial_.p);  // p.Ap fused into the element pass
         ++fine_applies;
         finalize(apply_blocks(mesh_.n_cells), 1, partial_.p, scal_.p, S_PAP, stream_);
-        cg_update_xr<double>(n, x_.p, r_.p, p_.p, Ap_.p, nullptr, nullptr, scal_.p, S_RZ, S_PAP,
+        cg_update_xr<double>(n, x_.p, r, p_.p, Ap_.p, nullptr, nullptr, scal_.p, S_RZ, S_PAP,
                              S_RR, rs, stream_);
         ++it;
         read_scalar(0);
@@ -1368,8 +1379,8 @@ int PoissonSolver::solve(const double* rhs, double* ans, Layout layout, SolveSta
         if (!(pap > 0) || !std::isfinite(rr) || !std::isfinite(pap)) { breakdown = true; break; }
         if (std::sqrt(rr) <= tol) break;
         std::swap(z_.p, zold_.p);
-        precondition(r_.p, z_.p);
-        dot_rz_flex<double>(n, r_.p, z_.p, zold_.p, scal_.p, S_RZN, rs, stream_);  // and S_BNUM
+        precondition(r, z_.p);
+        dot_rz_flex<double>(n, r, z_.p, zold_.p, scal_.p, S_RZN, rs, stream_);  // and S_BNUM
         cg_update_p<double>(n, p_.p, z_.p, scal_.p, S_BNUM, S_RZ, stream_);
         scal_copy(scal_.p, S_RZ, S_RZN, stream_);
       }
