@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 
@@ -241,6 +242,56 @@ k_cg_update_x2p(int64_t n, T* __restrict__ x, T* __restrict__ p0, const T* __res
   }
 }
 
+// Two entries per thread with 16-byte (FP64) or 8-byte (FP32) accesses: the direction updates of
+// the coarse CG, whose vectors are even-length and aligned. Same arithmetic per entry as the
+// one-entry kernels.
+template <typename T> struct Pair2;
+template <> struct Pair2<double> { using type = double2; };
+template <> struct Pair2<float> { using type = float2; };
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_p_pair(int64_t n2, T* __restrict__ p_next, const T* __restrict__ p, const T* __restrict__ z,
+                   double* __restrict__ scal, int anum, int aden, int bnum, int asave) {
+  using V = typename Pair2<T>::type;
+  const T beta = (T)safe_div(scal[bnum], scal[anum]);
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i < n2) {
+    const V pi = reinterpret_cast<const V*>(p)[i], zi = reinterpret_cast<const V*>(z)[i];
+    V o;
+    o.x = fma(beta, pi.x, zi.x);
+    o.y = fma(beta, pi.y, zi.y);
+    reinterpret_cast<V*>(p_next)[i] = o;
+  }
+  // no block reads scal[asave] in this launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[asave] = safe_div(scal[anum], scal[aden]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_x2p_pair(int64_t n2, T* __restrict__ x, T* __restrict__ p0, const T* __restrict__ p1,
+                     const T* __restrict__ z, const double* __restrict__ scal, int asave, int anum,
+                     int aden, int bnum, bool last) {
+  using V = typename Pair2<T>::type;
+  const T a0 = (T)scal[asave];
+  const T a1 = (T)safe_div(scal[anum], scal[aden]);
+  const T beta = (T)safe_div(scal[bnum], scal[anum]);
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i >= n2) return;
+  const V q0 = reinterpret_cast<const V*>(p0)[i], q1 = reinterpret_cast<const V*>(p1)[i];
+  V xi = reinterpret_cast<V*>(x)[i];
+  xi.x = fma(a1, q1.x, fma(a0, q0.x, xi.x));
+  xi.y = fma(a1, q1.y, fma(a0, q0.y, xi.y));
+  reinterpret_cast<V*>(x)[i] = xi;
+  if (!last) {
+    const V zi = reinterpret_cast<const V*>(z)[i];
+    V o;
+    o.x = fma(beta, q1.x, zi.x);
+    o.y = fma(beta, q1.y, zi.y);
+    reinterpret_cast<V*>(p0)[i] = o;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_jacobi(int64_t n, const T* __restrict__ r, const T* __restrict__ invdiag, T* __restrict__ z) {
@@ -393,6 +444,15 @@ __global__ void k_scal_copy(double* scal, int dst, int src) {
   if (threadIdx.x == 0) scal[dst] = scal[src];
 }
 
+// Vectors that the pair kernels may take: an even length and every base aligned to two entries.
+template <typename T>
+bool pairs_ok(int64_t n, std::initializer_list<const void*> ptrs) {
+  if (n & 1) return false;
+  for (const void* q : ptrs)
+    if (reinterpret_cast<uintptr_t>(q) % (2 * sizeof(T)) != 0) return false;
+  return true;
+}
+
 // Streams take one element per thread over a grid that covers the vector: on an MI355X that
 // streams at 6.1 TB/s where grid-stride loops over a few thousand blocks reach 4.4-4.9.
 int grid_stream(int64_t n) {
@@ -493,6 +553,11 @@ void cg_update_xp(int64_t n, T* x, T* p, const T* z, const double* scal, int anu
 template <typename T>
 void cg_update_p(int64_t n, T* p_next, const T* p, const T* z, double* scal, int anum, int aden,
                  int bnum, int asave, hipStream_t s) {
+  if (pairs_ok<T>(n, {p_next, p, z})) {
+    hipLaunchKernelGGL((k_cg_update_p_pair<T>), dim3(grid_stream(n / 2)), dim3(kBlock), 0, s, n / 2,
+                       p_next, p, z, scal, anum, aden, bnum, asave); launch_check("k_cg_update_p_pair");
+    return;
+  }
   hipLaunchKernelGGL((k_cg_update_p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p_next, p, z, scal,
                      anum, aden, bnum, asave); launch_check("k_cg_update_p");
 }
@@ -500,6 +565,12 @@ void cg_update_p(int64_t n, T* p_next, const T* p, const T* z, double* scal, int
 template <typename T>
 void cg_update_x2p(int64_t n, T* x, T* p0, const T* p1, const T* z, const double* scal, int asave,
                    int anum, int aden, int bnum, bool last, hipStream_t s) {
+  if (pairs_ok<T>(n, {x, p0, p1, z})) {
+    hipLaunchKernelGGL((k_cg_update_x2p_pair<T>), dim3(grid_stream(n / 2)), dim3(kBlock), 0, s, n / 2,
+                       x, p0, p1, z, scal, asave, anum, aden, bnum, last);
+    launch_check("k_cg_update_x2p_pair");
+    return;
+  }
   hipLaunchKernelGGL((k_cg_update_x2p<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, x, p0, p1, z,
                      scal, asave, anum, aden, bnum, last); launch_check("k_cg_update_x2p");
 }
