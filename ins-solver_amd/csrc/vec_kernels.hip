@@ -294,6 +294,27 @@ k_cg_update_x2p_pair(int64_t n2, T* __restrict__ x, T* __restrict__ p0, const T*
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
+k_cg_update_z_pair(int64_t n2, T* __restrict__ z, const T* __restrict__ Ap,
+                   const T* __restrict__ invdiag, const double* __restrict__ scal, int num, int den,
+                   double* __restrict__ partial) {
+  using V = typename Pair2<T>::type;
+  const T a = (T)safe_div(scal[num], scal[den]);
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kBlock) {
+    const V di = reinterpret_cast<const V*>(invdiag)[i], api = reinterpret_cast<const V*>(Ap)[i];
+    V zi = reinterpret_cast<V*>(z)[i];
+    zi.x = fma(-a * di.x, api.x, zi.x);
+    zi.y = fma(-a * di.y, api.y, zi.y);
+    reinterpret_cast<V*>(z)[i] = zi;
+    const double zx = (double)zi.x, zy = (double)zi.y;
+    v[0] += di.x != (T)0 ? zx * zx / (double)di.x : 0.0;
+    v[0] += di.y != (T)0 ? zy * zy / (double)di.y : 0.0;
+  }
+  write_partials<1>(v, partial);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
 k_jacobi(int64_t n, const T* __restrict__ r, const T* __restrict__ invdiag, T* __restrict__ z) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     z[i] = invdiag[i] * r[i];
@@ -538,8 +559,13 @@ template <typename T>
 void cg_update_z(int64_t n, T* z, const T* Ap, const T* invdiag, double* scal, int num, int den,
                  int out, RedScratch rs, hipStream_t s) {
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL((k_cg_update_z<T>), dim3(nb), dim3(kBlock), 0, s, n, z, Ap, invdiag, scal, num,
-                     den, rs.partial); launch_check("k_cg_update_z");
+  if (pairs_ok<T>(n, {z, Ap, invdiag})) {
+    hipLaunchKernelGGL((k_cg_update_z_pair<T>), dim3(nb), dim3(kBlock), 0, s, n / 2, z, Ap, invdiag,
+                       scal, num, den, rs.partial); launch_check("k_cg_update_z_pair");
+  } else {
+    hipLaunchKernelGGL((k_cg_update_z<T>), dim3(nb), dim3(kBlock), 0, s, n, z, Ap, invdiag, scal, num,
+                       den, rs.partial); launch_check("k_cg_update_z");
+  }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, nb, 1, rs.partial, scal, out); launch_check("k_finalize");
 }
 
