@@ -390,6 +390,62 @@ k_axpby(int64_t n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
     y[i] = fma(a, x[i], b * y[i]);
 }
 
+// 16-byte accesses for the smoother's streams: two FP64 or four FP32 entries per thread, same
+// arithmetic per entry as the one-entry kernels.
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
+template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
+
+template <typename T>
+__device__ __forceinline__ void ld16(const T* p, int64_t i, T (&o)[Vec16<T>::n]) {
+  const typename Vec16<T>::type v = reinterpret_cast<const typename Vec16<T>::type*>(p)[i];
+  __builtin_memcpy(o, &v, 16);
+}
+template <typename T>
+__device__ __forceinline__ void st16(T* p, int64_t i, const T (&o)[Vec16<T>::n]) {
+  typename Vec16<T>::type v;
+  __builtin_memcpy(&v, o, 16);
+  reinterpret_cast<typename Vec16<T>::type*>(p)[i] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_axpby16(int64_t n16, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+  constexpr int W = Vec16<T>::n;
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i >= n16) return;
+  T xv[W], yv[W];
+  ld16(x, i, xv);
+  ld16(y, i, yv);
+#pragma unroll
+  for (int k = 0; k < W; ++k) yv[k] = fma(a, xv[k], b * yv[k]);
+  st16(y, i, yv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cheb_step3_16(int64_t n16, const T* __restrict__ f, const T* __restrict__ Ax,
+                const T* __restrict__ invdiag, const T* prev, const T* __restrict__ x, T* x_next,
+                T c1, T c2) {
+  constexpr int W = Vec16<T>::n;
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i >= n16) return;
+  T fv[W], av[W], dv[W], xv[W], pv[W], o[W];
+  ld16(f, i, fv);
+  ld16(Ax, i, av);
+  ld16(invdiag, i, dv);
+  ld16(x, i, xv);
+  const bool rd = c1 != (T)0 && prev;
+  if (rd) ld16(prev, i, pv);
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    T di = c2 * (dv[k] * (fv[k] - av[k]));
+    if (c1 != (T)0) di = fma(c1, rd ? xv[k] - pv[k] : xv[k], di);
+    o[k] = xv[k] + di;
+  }
+  st16(x_next, i, o);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_sub_mean(int rows, int64_t E, int64_t stride, T* __restrict__ x, const double* __restrict__ scal,
@@ -471,6 +527,16 @@ bool pairs_ok(int64_t n, std::initializer_list<const void*> ptrs) {
   if (n & 1) return false;
   for (const void* q : ptrs)
     if (reinterpret_cast<uintptr_t>(q) % (2 * sizeof(T)) != 0) return false;
+  return true;
+}
+
+// Vectors that the 16-byte kernels may take: a length that is a multiple of the entries per
+// access and every base (null ones aside) 16-byte aligned.
+template <typename T>
+bool vec16_ok(int64_t n, std::initializer_list<const void*> ptrs) {
+  if (n % (16 / (int64_t)sizeof(T)) != 0) return false;
+  for (const void* q : ptrs)
+    if (reinterpret_cast<uintptr_t>(q) % 16 != 0) return false;
   return true;
 }
 
@@ -631,6 +697,12 @@ template <typename T>
 void cheb_step3(int64_t n, const T* f, const T* Ax, const T* invdiag, const T* prev, const T* x,
                 T* x_next, T c1, T c2, hipStream_t s) {
   if (x_next == x) throw std::invalid_argument("three-term Chebyshev step: x_next must not alias x");
+  if (vec16_ok<T>(n, {f, Ax, invdiag, prev, x, x_next})) {
+    const int64_t n16 = n / Vec16<T>::n;
+    hipLaunchKernelGGL((k_cheb_step3_16<T>), dim3(grid_stream(n16)), dim3(kBlock), 0, s, n16, f, Ax,
+                       invdiag, prev, x, x_next, c1, c2); launch_check("k_cheb_step3_16");
+    return;
+  }
   hipLaunchKernelGGL((k_cheb_step3<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, f, Ax, invdiag,
                      prev, x, x_next, c1, c2); launch_check("k_cheb_step3");
 }
@@ -652,6 +724,12 @@ void copy(int64_t n, const T* x, T* y, hipStream_t s) {
 
 template <typename T>
 void axpby(int64_t n, T a, const T* x, T b, T* y, hipStream_t s) {
+  if (vec16_ok<T>(n, {x, y})) {
+    const int64_t n16 = n / Vec16<T>::n;
+    hipLaunchKernelGGL((k_axpby16<T>), dim3(grid_stream(n16)), dim3(kBlock), 0, s, n16, a, x, b, y);
+    launch_check("k_axpby16");
+    return;
+  }
   hipLaunchKernelGGL((k_axpby<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, x, b, y); launch_check("k_axpby");
 }
 
