@@ -24,6 +24,9 @@ struct MeshView {
   // [E][NK] the metric cell by cell (3D; null otherwise): the order-1 pass gathers a
   // neighbour's metric as three 16-byte loads from one or two cache lines instead of six rows
   const double* Ka = nullptr;
+  // [E][4] the packed words cell by cell (3D, when packed; null otherwise), for the coarse CG's
+  // order-1 apply
+  const int* nbc_cells = nullptr;
 };
 
 // Boundary faces of the vector (velocity) operator with the slip condition (bc type 2 on
@@ -51,6 +54,7 @@ struct LevelView {
   const int* tab;
   const double* tauS;  // [Nf][stride] penalty * sJ
   const void* frag = nullptr;  // matrix-core operator fragments in the kernel precision, or null
+  const double* tau_cells = nullptr;  // [E][4] tauS cell by cell (3D order 1), or null
 };
 
 // SIPG operator: trace pass + element pass (order 1: element pass only). With

@@ -371,12 +371,18 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
   constexpr OpLayout L = op_layout(DIM, 1);
   constexpr int Np = L.Np, Nfp = L.Nfp, Nf = L.Nf, NK = L.NK;
   constexpr FmaskTable FM = make_fmask(DIM, 1);
-  // round 1
+  // round 1; VCM: nbr holds the packed words and tauS the penalties cell by cell ([E][4])
+  static_assert(!VCM || PACK, "cell-by-cell values: packed words");
   int nb[Nf], c[Nf];
+  int wcell[Nf];
+  if constexpr (VCM) {
+    const int4 w4 = reinterpret_cast<const int4*>(nbr)[e];
+    wcell[0] = w4.x; wcell[1] = w4.y; wcell[2] = w4.z; wcell[3] = w4.w;
+  }
 #pragma unroll
   for (int f = 0; f < Nf; ++f) {
     if constexpr (PACK) {
-      const int w = nbr[f * stride + e];
+      const int w = VCM ? wcell[f] : nbr[f * stride + e];
       nb[f] = w >= 0 ? w >> 5 : w;
       c[f] = w >= 0 ? (w & 31) : 0;
     } else {
@@ -428,8 +434,14 @@ apply_p1_cell(int64_t e, int64_t stride, const T* __restrict__ u, const double* 
   T K[NK];
   metric(e, K);
   T ts[Nf];
+  if constexpr (VCM) {
+    const double2* tq = reinterpret_cast<const double2*>(tauS) + e * 2;
+    const double2 t01 = tq[0], t23 = tq[1];
+    ts[0] = (T)t01.x; ts[1] = (T)t01.y; ts[2] = (T)t23.x; ts[3] = (T)t23.y;
+  } else {
 #pragma unroll
-  for (int f = 0; f < Nf; ++f) ts[f] = (T)tauS[f * stride + e];
+    for (int f = 0; f < Nf; ++f) ts[f] = (T)tauS[f * stride + e];
+  }
   // round 2
   T un[Nf][Np], kn[Nf][NK], fn_k[Nf][Np];
   int nd[Nf][Nfp], bt[Nf];

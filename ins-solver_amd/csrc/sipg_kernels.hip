@@ -465,19 +465,18 @@ void launch_apply(const MeshView& m, const LevelView& l, const T* u, const T* tr
 template <typename T>
 void launch_apply_p1_cells(const MeshView& m, const LevelView& l, const T* u, const T* kfac, T* out,
                            hipStream_t s, double* partial) {
-  if (m.dim != 3 || l.N != 1 || !m.Ka)
-    throw std::invalid_argument("cell-by-cell order-1 apply: 3D order 1 with the cell-by-cell metric");
+  if (m.dim != 3 || l.N != 1 || !m.Ka || !m.nbc_cells || !l.tau_cells)
+    throw std::invalid_argument("cell-by-cell order-1 apply: 3D order 1 with the cell-by-cell metric, "
+                                "packed words and penalties");
   if (m.E == 0) return;
   const T* C = static_cast<const T*>(l.C);
-  const bool pack = m.nbc != nullptr;
-  const int* nw = pack ? m.nbc : m.nbr;
   const P1NodeBits& bits = p1_node_bits(3);
-#define INSP_P1V(V, P)                                                                            \
-  hipLaunchKernelGGL((k_apply_p1<3, T, V, P, true, true>), grid_for(m.E), dim3(256), 0, s, m.E,   \
-                     m.stride, u, m.Ka, l.tauS, nw, m.code, m.bctype, kfac, C, l.tab, out, partial, \
-                     bits)
-  if (kfac) { if (pack) INSP_P1V(true, true); else INSP_P1V(true, false); }
-  else { if (pack) INSP_P1V(false, true); else INSP_P1V(false, false); }
+#define INSP_P1V(V)                                                                               \
+  hipLaunchKernelGGL((k_apply_p1<3, T, V, true, true, true>), grid_for(m.E), dim3(256), 0, s, m.E, \
+                     m.stride, u, m.Ka, l.tau_cells, m.nbc_cells, m.code, m.bctype, kfac, C, l.tab, \
+                     out, partial, bits)
+  if (kfac) INSP_P1V(true);
+  else INSP_P1V(false);
 #undef INSP_P1V
   launch_check("k_apply_p1");
 }

@@ -43,6 +43,7 @@ struct PoissonSolver::Level {
   DBuf<double> C64;
   DBuf<int> tab;
   DBuf<double> tauS, diag, Ik;
+  DBuf<double> tau_cells;  // 3D order 1: tauS cell by cell, for the coarse CG's apply
   double lmax = 0;
   Work<double> w64;
   Work<float> w32;
@@ -132,6 +133,12 @@ PoissonSolver::PoissonSolver(HostMesh mesh, int order, hipStream_t stream)
     for (size_t i = 0; i < nb.size(); ++i)
       if (nb[i] >= 0) nb[i] = (nb[i] << 5) | cd[i];
     nbc_.upload(nb.data(), nb.size());
+    if (d == 3) {
+      std::vector<int> nc((size_t)E * Nf);
+      for (int64_t e = 0; e < E; ++e)
+        for (int f = 0; f < Nf; ++f) nc[(size_t)e * Nf + f] = nb[(size_t)f * stride_ + e];
+      nbc_cells_.upload(nc.data(), nc.size());
+    }
   }
   std::vector<int> bt((size_t)std::max<int64_t>(mesh_.n_bfaces, 1), 0);
   bctype_.upload(bt.data(), bt.size());
@@ -164,6 +171,7 @@ MeshView PoissonSolver::mesh_view() const {
   m.bctype = bctype_.p;
   m.nbc = nbc_.empty() ? nullptr : nbc_.p;
   m.Ka = Ka_.empty() ? nullptr : Ka_.p;
+  m.nbc_cells = nbc_cells_.empty() ? nullptr : nbc_cells_.p;
   return m;
 }
 
@@ -189,6 +197,7 @@ LevelView PoissonSolver::level_view(int l, bool fp32) const {
   v.C64 = lv.C64.p;
   v.tab = lv.tab.p;
   v.tauS = lv.tauS.p;
+  v.tau_cells = lv.tau_cells.empty() ? nullptr : lv.tau_cells.p;
   v.frag = fp32 ? (const void*)lv.w32.frag.p : (const void*)lv.w64.frag.p;
   return v;
 }
@@ -615,6 +624,12 @@ void PoissonSolver::init() {
   for (size_t l = 0; l < levels_.size(); ++l)
     launch_tau(mv, level_view((int)l, false), var_factor_ ? levels_[l]->w64.k.p : nullptr, N_,
                levels_[l]->tauS.p, stream_);
+  for (size_t l = 0; l < levels_.size(); ++l)
+    if (mesh_.dim == 3 && levels_[l]->N == 1 && !nbc_cells_.empty()) {
+      Level& lv = *levels_[l];
+      lv.tau_cells.alloc((size_t)lv.L.Nf * stride_, false);
+      rows_to_cells<double>(mesh_.n_cells, stride_, lv.L.Nf, lv.tauS.p, lv.tau_cells.p, stream_);
+    }
   compute_diagonals();
   // work vectors
   const bool mixed = prec_ == Precision::MIXED;
@@ -838,7 +853,8 @@ void PoissonSolver::coarse_cg(int l) {
   // rows. b and D^-1 are copied in (into the level's ax and d, unused by this solve) and x is
   // copied out at the end; the vector kernels are the same over the flat range.
   const bool cells = mesh_.dim == 3 && lv.N == 1 && !has_mass_term() && !(oi_ && l == 0) &&
-                     !matrix_pass(l) && !Ka_.empty();
+                     !matrix_pass(l) && !Ka_.empty() &&
+                     !nbc_cells_.empty() && !lv.tau_cells.empty();
   const T* bb = w.b.p;
   const T* inv = w.inv.p;
   if (cells) {

@@ -274,6 +274,7 @@ class PoissonSolver {
   DBuf<double> Ka_;  // 3D: the metric cell by cell, for the order-1 pass
   DBuf<int> nbr_, code_, bctype_;
   DBuf<int> nbc_;  // packed neighbour words of the element passes (empty: not packed)
+  DBuf<int> nbc_cells_;  // 3D: the same words cell by cell, for the coarse CG's order-1 apply
   DBuf<double> bc_;
   bool have_bc_ = false;
   bool var_factor_ = false;
